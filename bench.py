@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the branch-site model A likelihood path on one MI355X.
+
+Metric: lnL+gradient evaluations per second at 128 taxa x 10,000 codons.
+One step = one evaluation of -lnL and its gradient in codeml's parameter
+vector (np = 2*128-3 + 5 = 258 parameters).  The path timed here is the plain
+FP64 one with ming2's forward-difference gradient (gradientB: np+1 = 259
+likelihood evaluations per step), every likelihood evaluation running the
+full HIP pipeline (P(t) batch, level-batched MFMA pruning, root reduction).
+
+Inputs are synthetic (seeded simulation under model A, config-2 parameter
+draws) and resident in HBM before timing starts.  The CPU baseline is the
+reference's own CPU codeml (built from its sources into oracle/_ref/codeml),
+timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cucodeml_amd import engine, formats, simulate  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def make_workload(ns, ncodon, seed):
+    rng = np.random.default_rng(seed)
+    tree = simulate.yule_tree(ns, 0.05, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    p = simulate.uniform_params(rng)
+    seqs, _ = simulate.simulate_model_a(tree, ncodon, p["kappa"], p["w0"], p["w2"], p["p0"],
+                                        p["p1"], pi, rng)
+    data = formats.load_codon_data(tree.names, seqs)
+    return tree, data, p, seqs
+
+
+CTL = """seqfile = seq.phy
+treefile = tree.nwk
+outfile = out
+noisy = 0
+verbose = 0
+runmode = 0
+seqtype = 1
+CodonFreq = 2
+clock = 0
+model = 2
+NSsites = 2
+icode = 0
+fix_kappa = 0
+kappa = 2
+fix_omega = 1
+omega = {w2}
+fix_alpha = 1
+alpha = 0
+getSE = 0
+RateAncestor = 0
+cleandata = 0
+method = 0
+"""
+
+
+def cpu_baseline(tree, seqs, p, ncodon_full, npatt_full, np_full, sample_codons):
+    """Reference CPU codeml at fixed parameters on the first `sample_codons`
+    codons; one run = 3 full likelihood evaluations (initial, NEB, final)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "codeml")
+    if not os.path.exists(ref):
+        return None
+    sub = [s[:3 * sample_codons] for s in seqs]
+    d_sub = formats.load_codon_data(tree.names, sub)
+    work = tempfile.mkdtemp(prefix="bsa_cpu_")
+    try:
+        formats.write_phylip(os.path.join(work, "seq.phy"), tree.names, sub)
+        with open(os.path.join(work, "tree.nwk"), "w") as f:
+            f.write(f"{tree.ns} 1\n{tree.newick(digits=10)}\n")
+        x = list(tree.branch_vector()) + [p["kappa"], p["p0"], p["p1"], p["w0"]]
+        with open(os.path.join(work, "in.codeml"), "w") as f:
+            f.write("-1 " + " ".join(f"{v:.12f}" for v in x) + "\n")
+        with open(os.path.join(work, "codeml.ctl"), "w") as f:
+            f.write(CTL.format(w2=p["w2"]))
+        t0 = time.perf_counter()
+        r = subprocess.run([ref, "codeml.ctl"], cwd=work, capture_output=True, text=True,
+                           input="\n", timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    per_lnl_sample = wall / 3.0
+    per_lnl_full = per_lnl_sample * npatt_full / d_sub.npatt
+    evals_per_s = 1.0 / ((np_full + 1) * per_lnl_full)
+    return {"value": evals_per_s, "unit": "lnL+gradient evals/s", "cores": 1,
+            "kind": "reference",
+            "sample": (f"reference codeml (CPU, -O3, 1 thread) at fixed parameters on "
+                       f"{tree.ns} taxa x {sample_codons} codons ({d_sub.npatt} patterns): "
+                       f"{wall:.2f} s per run of 3 lnL evaluations incl. set-up -> "
+                       f"{per_lnl_sample:.3f} s/lnL, scaled by patterns to {npatt_full} "
+                       f"patterns ({per_lnl_full:.2f} s/lnL) x (np+1)={np_full + 1} "
+                       f"evaluations per forward-difference gradient")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--taxa", type=int, default=128)
+    ap.add_argument("--codons", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--kernel", choices=["mfma", "valu"], default="mfma")
+    ap.add_argument("--cpu-sample-codons", type=int, default=1000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding over site patterns "
+              "is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    tree, data, p, seqs = make_workload(args.taxa, args.codons, args.seed)
+    eng = engine.BranchSiteEngine(tree, data, precision=args.precision, kernel=args.kernel)
+    x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"],
+                             p["w2"])
+    npar = len(x)
+    for _ in range(args.warmup):
+        eng.gradientB(x)
+    eng.timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f, g = eng.gradientB(x)
+    elapsed = time.perf_counter() - t0
+    tm = eng.timing(reset=True)
+    work = eng.work()
+    nev = tm["n_eval"]
+    prune_ms = tm["sum_prune"]
+    prune_s_per_eval = prune_ms / 1e3 / nev
+    bytes_per_eval = work[1]
+    achieved = bytes_per_eval / prune_s_per_eval / 1e9
+    flops_ach = work[0] / prune_s_per_eval / 1e12
+    launches = tm["n_launch"]
+
+    w = 8 if args.precision == "fp64" else 4
+    out = {
+        "metric": "lnL+gradient evals/sec at 128 taxa x 10k codons (branch-site model A)",
+        "value": args.steps / elapsed,
+        "unit": "lnL+gradient evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic: seeded model-A simulation (Yule tree, F3x4 Dirichlet(1))",
+        "config": {"workload": f"model A, {args.taxa} taxa x {args.codons} codons "
+                               f"({data.npatt} patterns), {args.precision} partials, "
+                               f"forward-difference gradient ({npar + 1} lnL per step)",
+                   "taxa": args.taxa, "codons": args.codons, "patterns": data.npatt,
+                   "np": npar, "gradient": "fd-forward", "kernel": args.kernel},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_prune_{args.kernel}<{'double' if w == 8 else 'float'}>",
+                     "bytes_per_lnl": bytes_per_eval, "flops_per_lnl": work[0],
+                     "launches_per_lnl": work[4],
+                     "avg_launch_us": prune_ms * 1e3 / max(launches, 1),
+                     "fp64_tflops": flops_ach, "fp64_frac": flops_ach / FP64_PEAK_TFLOPS},
+        "lnl_ms": {"wall": tm["sum_wall"] / nev, "pmat": tm["sum_pmat"] / nev,
+                   "prune": prune_ms / nev, "root": tm["sum_root"] / nev},
+        "lnL": -f,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(tree, seqs, p, args.codons, data.npatt, npar,
+                                           args.cpu_sample_codons)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

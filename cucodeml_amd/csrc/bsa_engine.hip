@@ -1,0 +1,654 @@
+// C ABI of the branch-site model A engine: problem set-up (tree schedule,
+// device buffers), one likelihood evaluation per call, finite-difference
+// gradients, per-site outputs.  Declarations and the reference entry points
+// each call replaces are in include/bsa.h.
+#include "bsa.h"
+#include "bsa_kernels.h"
+#include "bsa_model.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace bsa;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(BSA_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                          \
+    } while (0)
+
+constexpr int kUVRSet = 64 * 64 * 2 + 64;
+
+struct bsa_engine {
+    int dev = 0;
+    int ns = 0, nnode = 0, root = 0, nbranch = 0, npatt = 0, npad = 0, ncodes = 0, namb = 0;
+    int precision = BSA_FP64, kernel = BSA_KERNEL_MFMA;
+    std::vector<int> father, label, ibranch, height;
+    std::vector<std::vector<int>> sons;
+    std::vector<char> onpath;
+    std::vector<double> pi, fpatt;
+    bsa_model_opts opts{0, 1.0, 1};
+
+    // schedule
+    std::vector<PmatJob> pjobs;
+    std::vector<PruneJob> prune;          // all jobs, grouped by level
+    std::vector<int> level_off;           // level l: prune[level_off[l] .. level_off[l+1])
+    int nslot = 0;
+    int root_slot[4];
+
+    // device
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *d_uvr = nullptr;
+    PmatJob *d_pjobs = nullptr;
+    PruneJob *d_prune = nullptr;
+    void *d_PT = nullptr, *d_AT = nullptr, *d_L = nullptr;
+    int *d_E = nullptr;
+    unsigned char *d_z = nullptr, *d_mask_amb = nullptr;
+    double *d_pi = nullptr, *d_fpatt = nullptr, *d_fh = nullptr, *d_site = nullptr;
+    double *d_partial = nullptr, *d_out = nullptr;
+    int nroot_blk = 0;
+
+    // host staging (pinned)
+    double *h_uvr = nullptr;
+    double *h_out = nullptr;
+    PmatJob *h_pjobs = nullptr;
+
+    // model cache and last evaluation
+    ModelA model;
+    bool have_eigen = false;
+    double cache_kappa = NAN, cache_w[3] = {NAN, NAN, NAN};
+    double last_lnl = NAN;
+    double timing[5] = {0, 0, 0, 0, 0};
+    double timing_sum[5] = {0, 0, 0, 0, 0};
+    double n_eval = 0, n_launch = 0;
+    bool evaluated = false;
+    double work[6] = {0, 0, 0, 0, 0, 0};
+};
+
+static size_t tsize(const bsa_engine *e) { return e->precision == BSA_FP32 ? 4 : 8; }
+
+static void free_all(bsa_engine *e)
+{
+    if (e->stream) hipSetDevice(e->dev);
+    void *dev_ptrs[] = {e->d_uvr, e->d_pjobs, e->d_prune, e->d_PT, e->d_AT, e->d_L, e->d_E,
+                        e->d_z, e->d_mask_amb, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
+                        e->d_partial, e->d_out};
+    for (void *p : dev_ptrs)
+        if (p) hipFree(p);
+    if (e->h_uvr) hipHostFree(e->h_uvr);
+    if (e->h_out) hipHostFree(e->h_out);
+    if (e->h_pjobs) hipHostFree(e->h_pjobs);
+    for (auto &v : e->ev)
+        if (v) hipEventDestroy(v);
+    if (e->stream) hipStreamDestroy(e->stream);
+}
+
+// ---------------------------------------------------------------- set-up
+
+static int build_schedule(bsa_engine *e)
+{
+    const int ns = e->ns, nn = e->nnode;
+    // heights (tips 0), levels of interior nodes
+    e->height.assign(nn, 0);
+    std::vector<int> order;   // postorder
+    {
+        std::vector<std::pair<int, int>> st{{e->root, 0}};
+        while (!st.empty()) {
+            auto &[v, k] = st.back();
+            if (k < (int)e->sons[v].size()) {
+                int c = e->sons[v][k++];
+                st.push_back({c, 0});
+            } else {
+                order.push_back(v);
+                st.pop_back();
+            }
+        }
+    }
+    if ((int)order.size() != nn) return fail(BSA_EINVAL, "tree is not connected from the root");
+    for (int v : order) {
+        int h = 0;
+        for (int c : e->sons[v]) h = std::max(h, e->height[c] + 1);
+        e->height[v] = (v < ns) ? 0 : h;
+    }
+    // nodes whose site classes 2/3 differ from 0/1: proper ancestors of a
+    // foreground branch
+    e->onpath.assign(nn, 0);
+    int nfg = 0;
+    for (int v = 0; v < nn; v++) {
+        if (v == e->root || e->label[v] != 1) continue;
+        nfg++;
+        for (int a = e->father[v]; a >= 0; a = e->father[a]) e->onpath[a] = 1;
+    }
+    if (nfg == 0) return fail(BSA_EINVAL, "branch-site model A needs a foreground (#1) branch");
+    // partial slots
+    std::vector<std::array<int, 4>> slot(nn, std::array<int, 4>{-1, -1, -1, -1});
+    int ns_ = 0;
+    for (int v = ns; v < nn; v++) {
+        slot[v][0] = ns_++;
+        slot[v][1] = ns_++;
+        if (e->onpath[v]) {
+            slot[v][2] = ns_++;
+            slot[v][3] = ns_++;
+        } else {
+            slot[v][2] = slot[v][0];
+            slot[v][3] = slot[v][1];
+        }
+    }
+    e->nslot = ns_;
+    for (int c = 0; c < 4; c++) e->root_slot[c] = slot[e->root][c];
+    // pruning jobs grouped by height
+    int hmax = e->height[e->root];
+    e->prune.clear();
+    e->level_off.assign(1, 0);
+    for (int lev = 1; lev <= hmax; lev++) {
+        for (int v = ns; v < nn; v++) {
+            if (e->height[v] != lev) continue;
+            int ncls = e->onpath[v] ? 4 : 2;
+            for (int c = 0; c < ncls; c++) {
+                PruneJob j{};
+                j.out_slot = slot[v][c];
+                j.nch = (int)e->sons[v].size();
+                for (int k = 0; k < j.nch; k++) {
+                    int s = e->sons[v][k];
+                    j.ch_tip[k] = s < ns ? s : -1;
+                    j.ch_slot[k] = s < ns ? -1 : slot[s][c];
+                    j.ch_pmat[k] = s * 3 + uvr_set(c, e->label[s]);
+                }
+                e->prune.push_back(j);
+            }
+        }
+        e->level_off.push_back((int)e->prune.size());
+    }
+    // work per evaluation of the pruning pass, counted on 61 states
+    {
+        const double w = e->precision == BSA_FP32 ? 4.0 : 8.0, np_ = e->npatt;
+        double fl = 0, by = 0;
+        for (const PruneJob &j : e->prune) {
+            int nint = 0, ntip = 0;
+            for (int k = 0; k < j.nch; k++) (j.ch_tip[k] >= 0 ? ntip : nint)++;
+            fl += 2.0 * kS * kS * np_ * nint;
+            by += np_ * (kS * w * (nint + 1) + ntip + 4.0 * (nint + 1));
+        }
+        int nint_nodes = nn - ns, nint_edges = 0;
+        for (int v = ns; v < nn; v++)
+            if (v != e->root) nint_edges++;
+        e->work[0] = fl;
+        e->work[1] = by;
+        e->work[2] = 4.0 * np_ * nint_edges * 2.0 * kS * kS;
+        e->work[3] = 4.0 * np_ * kS * w * (nint_nodes + nint_edges);
+        e->work[4] = (double)(e->level_off.size() - 1);
+        e->work[5] = (double)e->prune.size();
+    }
+    // P(t) jobs: every branch for eigen sets w0 and 1; foreground also w2
+    e->pjobs.clear();
+    for (int v = 0; v < nn; v++) {
+        if (v == e->root) continue;
+        int nset = e->label[v] == 1 ? 3 : 2;
+        for (int s = 0; s < nset; s++) {
+            PmatJob p{};
+            p.node = v;
+            p.set = s;
+            p.pm = v * 3 + s;
+            p.tip = v < ns ? v : -1;
+            p.t = 0;
+            e->pjobs.push_back(p);
+        }
+    }
+    return BSA_OK;
+}
+
+static int upload_static(bsa_engine *e, const bsa_problem *p)
+{
+    const size_t ts = tsize(e);
+    const int np = e->npad;
+    HIPCHK(hipSetDevice(e->dev));
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto &v : e->ev) HIPCHK(hipEventCreate(&v));
+    HIPCHK(hipMalloc(&e->d_uvr, sizeof(double) * 3 * kUVRSet));
+    HIPCHK(hipMalloc(&e->d_pjobs, sizeof(PmatJob) * e->pjobs.size()));
+    HIPCHK(hipMalloc(&e->d_prune, sizeof(PruneJob) * std::max<size_t>(1, e->prune.size())));
+    HIPCHK(hipMalloc(&e->d_PT, ts * 4096 * (size_t)e->nnode * 3));
+    HIPCHK(hipMemset(e->d_PT, 0, ts * 4096 * (size_t)e->nnode * 3));
+    if (e->namb > 0) {
+        HIPCHK(hipMalloc(&e->d_AT, ts * 64 * (size_t)e->namb * e->ns * 3));
+        HIPCHK(hipMalloc(&e->d_mask_amb, 64 * (size_t)e->namb));
+    }
+    HIPCHK(hipMalloc(&e->d_L, ts * 64 * (size_t)np * e->nslot));
+    HIPCHK(hipMalloc(&e->d_E, sizeof(int) * (size_t)np * e->nslot));
+    HIPCHK(hipMalloc(&e->d_z, (size_t)np * e->ns));
+    HIPCHK(hipMalloc(&e->d_pi, sizeof(double) * 64));
+    HIPCHK(hipMalloc(&e->d_fpatt, sizeof(double) * np));
+    HIPCHK(hipMalloc(&e->d_fh, sizeof(double) * 4 * (size_t)np));
+    HIPCHK(hipMalloc(&e->d_site, sizeof(double) * np));
+    e->nroot_blk = (e->npatt + 63) / 64;
+    HIPCHK(hipMalloc(&e->d_partial, sizeof(double) * 2 * e->nroot_blk));
+    HIPCHK(hipMalloc(&e->d_out, sizeof(double) * 8));
+    HIPCHK(hipHostMalloc((void **)&e->h_uvr, sizeof(double) * 3 * kUVRSet, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&e->h_out, sizeof(double) * 8, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
+                         hipHostMallocDefault));
+
+    // static inputs
+    std::vector<unsigned char> z((size_t)np * e->ns, 0);
+    for (int s = 0; s < e->ns; s++)
+        std::memcpy(&z[(size_t)s * np], p->z + (size_t)s * e->npatt, e->npatt);
+    HIPCHK(hipMemcpy(e->d_z, z.data(), z.size(), hipMemcpyHostToDevice));
+    if (e->namb > 0) {
+        std::vector<unsigned char> m(64 * (size_t)e->namb, 0);
+        for (int a = 0; a < e->namb; a++)
+            for (int k = 0; k < kS; k++) m[a * 64 + k] = p->code_mask[(size_t)(kS + a) * kS + k] ? 1 : 0;
+        HIPCHK(hipMemcpy(e->d_mask_amb, m.data(), m.size(), hipMemcpyHostToDevice));
+    }
+    std::vector<double> pi64(64, 0.0), fp(np, 0.0);
+    for (int i = 0; i < kS; i++) pi64[i] = e->pi[i];
+    for (int h = 0; h < e->npatt; h++) fp[h] = e->fpatt[h];
+    HIPCHK(hipMemcpy(e->d_pi, pi64.data(), sizeof(double) * 64, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_fpatt, fp.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+    if (!e->prune.empty())
+        HIPCHK(hipMemcpy(e->d_prune, e->prune.data(), sizeof(PruneJob) * e->prune.size(),
+                         hipMemcpyHostToDevice));
+    return BSA_OK;
+}
+
+extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
+{
+    if (!p || !out) return fail(BSA_EINVAL, "null argument");
+    *out = nullptr;
+    if (p->ns < 3) return fail(BSA_EINVAL, "need at least 3 sequences (ns=%d)", p->ns);
+    if (p->nnode <= p->ns || p->nnode > 2 * p->ns - 1)
+        return fail(BSA_EINVAL, "nnode=%d inconsistent with ns=%d", p->nnode, p->ns);
+    if (p->root < p->ns || p->root >= p->nnode) return fail(BSA_EINVAL, "root must be interior");
+    if (p->npatt <= 0) return fail(BSA_EINVAL, "npatt must be positive");
+    if (p->ncodes < kS || p->ncodes > 256) return fail(BSA_EINVAL, "ncodes=%d out of range", p->ncodes);
+    if (!p->father || !p->label || !p->ibranch || !p->z || !p->fpatt || !p->pi)
+        return fail(BSA_EINVAL, "null array in problem");
+    if (p->ncodes > kS && !p->code_mask) return fail(BSA_EINVAL, "ambiguity codes need code_mask");
+    if (p->precision != BSA_FP64 && p->precision != BSA_FP32)
+        return fail(BSA_EINVAL, "unknown precision %d", p->precision);
+    if (p->kernel != BSA_KERNEL_MFMA && p->kernel != BSA_KERNEL_VALU)
+        return fail(BSA_EINVAL, "unknown kernel %d", p->kernel);
+
+    bsa_engine *e = new bsa_engine();
+    e->dev = p->device;
+    e->ns = p->ns;
+    e->nnode = p->nnode;
+    e->root = p->root;
+    e->nbranch = p->nnode - 1;
+    e->npatt = p->npatt;
+    e->npad = (p->npatt + kTileP - 1) / kTileP * kTileP;
+    e->ncodes = p->ncodes;
+    e->namb = p->ncodes - kS;
+    e->precision = p->precision;
+    e->kernel = p->kernel;
+    e->father.assign(p->father, p->father + p->nnode);
+    e->label.assign(p->label, p->label + p->nnode);
+    e->ibranch.assign(p->ibranch, p->ibranch + p->nnode);
+    e->sons.assign(p->nnode, {});
+    std::vector<char> seen_b(e->nbranch, 0);
+    for (int v = 0; v < p->nnode; v++) {
+        int f = e->father[v];
+        if (v == p->root) {
+            if (f != -1) { delete e; return fail(BSA_EINVAL, "father[root] must be -1"); }
+            continue;
+        }
+        if (f < p->ns || f >= p->nnode) {
+            delete e;
+            return fail(BSA_EINVAL, "node %d has invalid father %d", v, f);
+        }
+        if (e->label[v] != 0 && e->label[v] != 1) {
+            delete e;
+            return fail(BSA_EINVAL, "branch label of node %d is %d; model A allows 0 and 1", v,
+                        e->label[v]);
+        }
+        int b = e->ibranch[v];
+        if (b < 0 || b >= e->nbranch || seen_b[b]) {
+            delete e;
+            return fail(BSA_EINVAL, "ibranch of node %d (%d) is not a permutation", v, b);
+        }
+        seen_b[b] = 1;
+        e->sons[f].push_back(v);
+    }
+    for (int v = p->ns; v < p->nnode; v++) {
+        if (e->sons[v].empty()) { delete e; return fail(BSA_EINVAL, "interior node %d has no sons", v); }
+        if ((int)e->sons[v].size() > kMaxCh) {
+            delete e;
+            return fail(BSA_EINVAL, "node %d has %zu sons (max %d)", v, e->sons[v].size(), kMaxCh);
+        }
+    }
+    for (size_t i = 0; i < (size_t)p->ns * p->npatt; i++)
+        if (p->z[i] >= p->ncodes) { delete e; return fail(BSA_EINVAL, "code %d >= ncodes", p->z[i]); }
+    e->pi.assign(p->pi, p->pi + kS);
+    e->fpatt.assign(p->fpatt, p->fpatt + p->npatt);
+    int rc = build_schedule(e);
+    if (rc) { delete e; return rc; }
+    rc = upload_static(e, p);
+    if (rc) { std::string m = g_err; free_all(e); delete e; g_err = m; return rc; }
+    *out = e;
+    return BSA_OK;
+}
+
+extern "C" int bsa_destroy(bsa_engine *e)
+{
+    if (!e) return BSA_OK;
+    free_all(e);
+    delete e;
+    return BSA_OK;
+}
+
+extern "C" int bsa_set_model(bsa_engine *e, const bsa_model_opts *o)
+{
+    if (!e || !o) return fail(BSA_EINVAL, "null argument");
+    if (o->fix_omega && !(o->omega_fix > 0)) return fail(BSA_EINVAL, "omega_fix must be > 0");
+    e->opts = *o;
+    return BSA_OK;
+}
+
+extern "C" int bsa_num_params(const bsa_engine *e)
+{
+    if (!e) return -1;
+    return e->nbranch + (e->opts.fix_omega ? 4 : 5);
+}
+
+// ---------------------------------------------------------------- evaluation
+
+// x (codeml variables) -> natural parameters; f_and_x, tools.c:1280-1305
+static void natural_params(const bsa_engine *e, const double *x, double *kappa, double *p0,
+                           double *p1, double *w0, double *w2)
+{
+    const int nb = e->nbranch;
+    *kappa = x[nb];
+    if (e->opts.p_transformed) {
+        double a = std::exp(x[nb + 1]), b = std::exp(x[nb + 2]), tot = 1 + a + b;
+        *p0 = a / tot;
+        *p1 = b / tot;
+    } else {
+        *p0 = x[nb + 1];
+        *p1 = x[nb + 2];
+    }
+    *w0 = x[nb + 3];
+    *w2 = e->opts.fix_omega ? e->opts.omega_fix : x[nb + 4];
+}
+
+template <typename T>
+static int run_device(bsa_engine *e)
+{
+    hipStream_t s = e->stream;
+    HIPCHK(hipMemcpyAsync(e->d_uvr, e->h_uvr, sizeof(double) * 3 * kUVRSet, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->d_pjobs, e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(e->ev[0], s));
+    launch_pmat<T>(s, e->d_uvr, e->d_pjobs, (int)e->pjobs.size(), (T *)e->d_PT, e->d_mask_amb,
+                   e->namb, (T *)e->d_AT);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev[1], s));
+    for (size_t l = 0; l + 1 < e->level_off.size(); l++) {
+        int a = e->level_off[l], b = e->level_off[l + 1];
+        launch_prune<T>(s, e->kernel, e->d_prune + a, b - a, e->npad, e->d_z, (const T *)e->d_PT,
+                        (const T *)e->d_AT, e->namb, (T *)e->d_L, e->d_E);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev[2], s));
+    RootArgs ra;
+    ra.npatt = e->npatt;
+    ra.npad = e->npad;
+    for (int c = 0; c < 4; c++) {
+        ra.root_slot[c] = e->root_slot[c];
+        ra.freqK[c] = e->model.freqK[c];
+    }
+    launch_root<T>(s, ra, (const T *)e->d_L, e->d_E, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
+                   e->d_partial, e->nroot_blk, e->d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev[3], s));
+    HIPCHK(hipMemcpyAsync(e->h_out, e->d_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+    e->timing[0] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));
+    e->timing[1] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[2], e->ev[3]));
+    e->timing[2] = ms;
+    e->timing[3] = 0;
+    for (int i = 0; i < 3; i++) e->timing_sum[i] += e->timing[i];
+    e->n_launch += (double)(e->level_off.size() - 1);
+    return BSA_OK;
+}
+
+static int evaluate_inner(bsa_engine *e, const double *x, double *lnl);
+
+static int evaluate(bsa_engine *e, const double *x, double *lnl)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = evaluate_inner(e, x, lnl);
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == BSA_OK) {
+        e->timing[4] = ms;
+        e->timing_sum[4] += ms;
+        e->n_eval += 1;
+    }
+    return rc;
+}
+
+static int evaluate_inner(bsa_engine *e, const double *x, double *lnl)
+{
+    double kappa, p0, p1, w0, w2;
+    natural_params(e, x, &kappa, &p0, &p1, &w0, &w2);
+    for (int b = 0; b < e->nbranch; b++)
+        if (!std::isfinite(x[b])) return fail(BSA_EINVAL, "branch length %d not finite", b);
+    if (!(kappa > 0) || !(w0 >= 0) || !(w2 >= 0) || !(p0 >= 0) || !(p1 >= 0) || !(p0 + p1 > 0) ||
+        p0 + p1 > 1 + 1e-12)
+        return fail(BSA_EINVAL, "parameters out of range: kappa=%g p0=%g p1=%g w0=%g w2=%g", kappa,
+                    p0, p1, w0, w2);
+    const bool same = e->have_eigen && kappa == e->cache_kappa && w0 == e->cache_w[0] &&
+                      w2 == e->cache_w[2];
+    if (same) {
+        // eigen systems unchanged: only proportions and Qfactors are recomputed
+        if (!setup_model_a(kappa, p0, p1, w0, w2, e->pi.data(), &e->model, false))
+            return fail(BSA_ENUMERIC, "model set-up failed");
+    } else {
+        if (!setup_model_a(kappa, p0, p1, w0, w2, e->pi.data(), &e->model, true)) {
+            e->have_eigen = false;
+            return fail(BSA_ENUMERIC, "eigen decomposition failed");
+        }
+        e->have_eigen = true;
+        e->cache_kappa = kappa;
+        e->cache_w[0] = w0;
+        e->cache_w[1] = 1.0;
+        e->cache_w[2] = w2;
+        // padded U, V, root per eigen set
+        for (int s = 0; s < 3; s++) {
+            double *U = e->h_uvr + s * kUVRSet, *V = U + 4096, *R = U + 8192;
+            std::memset(U, 0, sizeof(double) * kUVRSet);
+            for (int i = 0; i < kS; i++)
+                for (int k = 0; k < kS; k++) {
+                    U[i * 64 + k] = e->model.U[s][i * kS + k];
+                    V[i * 64 + k] = e->model.V[s][i * kS + k];
+                }
+            for (int k = 0; k < kS; k++) R[k] = e->model.root[s][k];
+        }
+    }
+    for (size_t i = 0; i < e->pjobs.size(); i++) {
+        PmatJob p = e->pjobs[i];
+        p.t = x[e->ibranch[p.node]] * e->model.Qfactor[e->label[p.node]];
+        e->h_pjobs[i] = p;
+    }
+    int rc = e->precision == BSA_FP32 ? run_device<float>(e) : run_device<double>(e);
+    if (rc) return rc;
+    *lnl = e->h_out[0];
+    e->last_lnl = *lnl;
+    e->evaluated = true;
+    return BSA_OK;
+}
+
+extern "C" int bsa_lnl(bsa_engine *e, const double *x, int np, double *neg_lnl)
+{
+    if (!e || !x || !neg_lnl) return fail(BSA_EINVAL, "null argument");
+    if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    HIPCHK(hipSetDevice(e->dev));
+    double lnl;
+    int rc = evaluate(e, x, &lnl);
+    if (rc) return rc;
+    *neg_lnl = -lnl;
+    return BSA_OK;
+}
+
+// finite-difference gradient of -lnL in codeml's variables, as gradientB
+// (tools.c:6326-6350): step eh = small*(|x_i|+1); forward difference, or
+// central with eh^0.67 when `central`.
+extern "C" int bsa_lnl_grad_fd(bsa_engine *e, const double *x, int np, int central, double small,
+                               double *neg_lnl, double *grad)
+{
+    if (!e || !x || !neg_lnl || !grad) return fail(BSA_EINVAL, "null argument");
+    if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    if (!(small > 0)) small = 1e-6;
+    HIPCHK(hipSetDevice(e->dev));
+    std::vector<double> x1(x, x + np);
+    double f0, f1, f2;
+    int rc = evaluate(e, x, &f0);
+    if (rc) return rc;
+    for (int i = 0; i < np; i++) {
+        double eh = small * (std::fabs(x[i]) + 1);
+        if (central) {
+            eh = std::pow(eh, 0.67);
+            x1[i] = x[i] + eh;
+            if ((rc = evaluate(e, x1.data(), &f1))) return rc;
+            x1[i] = x[i] - eh;
+            if ((rc = evaluate(e, x1.data(), &f2))) return rc;
+            grad[i] = (-f1 + f2) / (2 * eh);
+        } else {
+            x1[i] = x[i] + eh;
+            if ((rc = evaluate(e, x1.data(), &f1))) return rc;
+            grad[i] = (-f1 + f0) / eh;
+        }
+        x1[i] = x[i];
+    }
+    // leave the engine's per-site state at x
+    if ((rc = evaluate(e, x, &f0))) return rc;
+    *neg_lnl = -f0;
+    return BSA_OK;
+}
+
+extern "C" int bsa_class_lnl(bsa_engine *e, double *fhK, double *site)
+{
+    if (!e) return fail(BSA_EINVAL, "null engine");
+    if (!e->evaluated) return fail(BSA_EINVAL, "no evaluation yet");
+    HIPCHK(hipSetDevice(e->dev));
+    if (fhK)
+        for (int c = 0; c < 4; c++)
+            HIPCHK(hipMemcpy(fhK + (size_t)c * e->npatt, e->d_fh + (size_t)c * e->npad,
+                             sizeof(double) * e->npatt, hipMemcpyDeviceToHost));
+    if (site)
+        HIPCHK(hipMemcpy(site, e->d_site, sizeof(double) * e->npatt, hipMemcpyDeviceToHost));
+    return BSA_OK;
+}
+
+extern "C" int bsa_last_params(const bsa_engine *e, double *o)
+{
+    if (!e || !o) return fail(BSA_EINVAL, "null argument");
+    if (!e->evaluated) return fail(BSA_EINVAL, "no evaluation yet");
+    const ModelA &m = e->model;
+    double v[11] = {m.kappa, m.p0, m.p1, m.w0, m.w2, m.Qfactor[0], m.Qfactor[1],
+                    m.freqK[0], m.freqK[1], m.freqK[2], m.freqK[3]};
+    std::memcpy(o, v, sizeof v);
+    return BSA_OK;
+}
+
+extern "C" int bsa_timing(bsa_engine *e, double *o, int reset)
+{
+    if (!e || !o) return fail(BSA_EINVAL, "null argument");
+    std::memcpy(o, e->timing, sizeof e->timing);
+    std::memcpy(o + 5, e->timing_sum, sizeof e->timing_sum);
+    o[10] = e->n_eval;
+    o[11] = e->n_launch;
+    if (reset) {
+        for (double &v : e->timing_sum) v = 0;
+        e->n_eval = 0;
+        e->n_launch = 0;
+    }
+    return BSA_OK;
+}
+
+extern "C" int bsa_model_a_setup(double kappa, double p0, double p1, double w0, double w2,
+                                 const double *pi, double *freqK, double *Qfactor, double *Root,
+                                 double *U, double *V)
+{
+    if (!pi || !freqK || !Qfactor) return fail(BSA_EINVAL, "null argument");
+    static thread_local ModelA m;
+    bool eig = Root && U && V;
+    if (!setup_model_a(kappa, p0, p1, w0, w2, pi, &m, eig))
+        return fail(BSA_ENUMERIC, "model set-up failed");
+    std::memcpy(freqK, m.freqK, sizeof m.freqK);
+    std::memcpy(Qfactor, m.Qfactor, sizeof m.Qfactor);
+    if (eig) {
+        std::memcpy(Root, m.root, sizeof m.root);
+        std::memcpy(U, m.U, sizeof m.U);
+        std::memcpy(V, m.V, sizeof m.V);
+    }
+    return BSA_OK;
+}
+
+extern "C" int bsa_work(const bsa_engine *e, double *o)
+{
+    if (!e || !o) return fail(BSA_EINVAL, "null argument");
+    std::memcpy(o, e->work, sizeof e->work);
+    return BSA_OK;
+}
+
+extern "C" const char *bsa_last_error(void) { return g_err.c_str(); }
+extern "C" const char *bsa_version(void) { return "bsa 0.1 gfx950"; }
+
+// In-bounds check of the matrix-core lane layout: runs one 16x16x4 MFMA in
+// FP64 and FP32 on A (16x4) and B (4x16) and returns each lane's 4 outputs.
+extern "C" int bsa_debug_mfma_probe(int device, const double *A, const double *B, double *D64,
+                                    double *D32)
+{
+    HIPCHK(hipSetDevice(device));
+    double *dA, *dB, *dD;
+    float *fA, *fB, *fD;
+    HIPCHK(hipMalloc(&dA, 64 * 8));
+    HIPCHK(hipMalloc(&dB, 64 * 8));
+    HIPCHK(hipMalloc(&dD, 256 * 8));
+    HIPCHK(hipMalloc(&fA, 64 * 4));
+    HIPCHK(hipMalloc(&fB, 64 * 4));
+    HIPCHK(hipMalloc(&fD, 256 * 4));
+    float hA[64], hB[64], hD[256];
+    for (int i = 0; i < 64; i++) { hA[i] = (float)A[i]; hB[i] = (float)B[i]; }
+    HIPCHK(hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(fA, hA, 64 * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(fB, hB, 64 * 4, hipMemcpyHostToDevice));
+    launch_mfma_probe(nullptr, dA, dB, dD, fA, fB, fD);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(D64, dD, 256 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hD, fD, 256 * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 256; i++) D32[i] = hD[i];
+    hipFree(dA); hipFree(dB); hipFree(dD); hipFree(fA); hipFree(fB); hipFree(fD);
+    return BSA_OK;
+}

@@ -1,0 +1,313 @@
+// Host FP64 model set-up for branch-site model A: GY94 rate matrices, the
+// symmetric eigen decomposition they are exponentiated through, Qfactor
+// branch scaling and the site-class table.  Everything here is O(61^3) per
+// likelihood evaluation and runs on the host in double precision.
+#include "bsa_model.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace bsa {
+
+static const char kAA64[] =
+    "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
+
+CodonTables::CodonTables()
+{
+    int k = 0;
+    for (int c = 0; c < 64; c++) {
+        from64[c] = -1;
+        if (kAA64[c] != '*') { from61[k] = c; from64[c] = k; k++; }
+    }
+    // kind[i][j]: 0 = not a single-nucleotide change; else bit0 transition,
+    // bit1 nonsynonymous, +4 marks a one-step neighbour
+    for (int i = 0; i < kS; i++)
+        for (int j = 0; j < kS; j++) {
+            int a = from61[i], b = from61[j];
+            int fa[3] = {a / 16, (a / 4) % 4, a % 4}, fb[3] = {b / 16, (b / 4) % 4, b % 4};
+            int nd = 0, pos = 0;
+            for (int p = 0; p < 3; p++)
+                if (fa[p] != fb[p]) { nd++; pos = p; }
+            unsigned char kd = 0;
+            if (i != j && nd == 1) {
+                kd = 4;
+                int s = fa[pos] + fb[pos];
+                if (s == 1 || s == 5) kd |= 1;          // T<->C or A<->G
+                if (kAA64[a] != kAA64[b]) kd |= 2;
+            }
+            kind[i][j] = kd;
+        }
+}
+
+const CodonTables &tables()
+{
+    static const CodonTables t;
+    return t;
+}
+
+// Q_ij = q_ij pi_j with q = kappa^[transition] * omega^[nonsynonymous] for
+// one-step neighbours (codeml.c:3147-3186); returns the mean rate
+// -sum_i pi_i Q_ii.
+double build_q(double kappa, double omega, const double *pi, double *Q)
+{
+    const CodonTables &T = tables();
+    for (int i = 0; i < kS; i++) {
+        double rs = 0;
+        for (int j = 0; j < kS; j++) {
+            unsigned char kd = T.kind[i][j];
+            double q = 0;
+            if (kd) {
+                q = 1;
+                if (kd & 1) q = kappa;
+                if (kd & 2) q *= omega;
+                q *= pi[j];
+            }
+            Q[i * kS + j] = q;
+            rs += q;
+        }
+        Q[i * kS + i] = -rs;
+    }
+    double mr = 0;
+    for (int i = 0; i < kS; i++) mr -= pi[i] * Q[i * kS + i];
+    return mr;
+}
+
+double mean_rate(double kappa, double omega, const double *pi)
+{
+    const CodonTables &T = tables();
+    double mr = 0;
+    for (int i = 0; i < kS; i++) {
+        double rs = 0;
+        for (int j = 0; j < kS; j++) {
+            unsigned char kd = T.kind[i][j];
+            if (!kd) continue;
+            double q = (kd & 1) ? kappa : 1.0;
+            if (kd & 2) q *= omega;
+            rs += q * pi[j];
+        }
+        mr += pi[i] * rs;
+    }
+    return mr;
+}
+
+// Householder reduction of the symmetric matrix in z (n x n, row-major) to
+// tridiagonal form (diagonal d, sub-diagonal e), accumulating the orthogonal
+// transform in z; then the implicit-shift QL iteration on the tridiagonal
+// matrix, rotating z into the eigenvectors (columns).  This is the classic
+// Householder + QL pair of algorithms (Wilkinson & Reinsch, Handbook for
+// Automatic Computation vol. II, procedures tred2 and tql2).
+static bool sym_eigen(int n, double *z, double *d, double *e)
+{
+#define Z(i, j) z[(i) * n + (j)]
+    for (int j = 0; j < n; j++) d[j] = Z(n - 1, j);
+    for (int i = n - 1; i > 0; i--) {
+        double scale = 0, h = 0;
+        for (int k = 0; k < i; k++) scale += std::fabs(d[k]);
+        if (scale == 0) {
+            e[i] = d[i - 1];
+            for (int j = 0; j < i; j++) {
+                d[j] = Z(i - 1, j);
+                Z(i, j) = 0;
+                Z(j, i) = 0;
+            }
+        } else {
+            for (int k = 0; k < i; k++) {
+                d[k] /= scale;
+                h += d[k] * d[k];
+            }
+            double f = d[i - 1];
+            double g = std::sqrt(h);
+            if (f > 0) g = -g;
+            e[i] = scale * g;
+            h -= f * g;
+            d[i - 1] = f - g;
+            for (int j = 0; j < i; j++) e[j] = 0;
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                Z(j, i) = f;
+                g = e[j] + Z(j, j) * f;
+                for (int k = j + 1; k <= i - 1; k++) {
+                    g += Z(k, j) * d[k];
+                    e[k] += Z(k, j) * f;
+                }
+                e[j] = g;
+            }
+            f = 0;
+            for (int j = 0; j < i; j++) {
+                e[j] /= h;
+                f += e[j] * d[j];
+            }
+            double hh = f / (h + h);
+            for (int j = 0; j < i; j++) e[j] -= hh * d[j];
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                g = e[j];
+                for (int k = j; k <= i - 1; k++) Z(k, j) -= (f * e[k] + g * d[k]);
+                d[j] = Z(i - 1, j);
+                Z(i, j) = 0;
+            }
+        }
+        d[i] = h;
+    }
+    for (int i = 0; i < n - 1; i++) {
+        Z(n - 1, i) = Z(i, i);
+        Z(i, i) = 1;
+        double h = d[i + 1];
+        if (h != 0) {
+            for (int k = 0; k <= i; k++) d[k] = Z(k, i + 1) / h;
+            for (int j = 0; j <= i; j++) {
+                double g = 0;
+                for (int k = 0; k <= i; k++) g += Z(k, i + 1) * Z(k, j);
+                for (int k = 0; k <= i; k++) Z(k, j) -= g * d[k];
+            }
+        }
+        for (int k = 0; k <= i; k++) Z(k, i + 1) = 0;
+    }
+    for (int j = 0; j < n; j++) {
+        d[j] = Z(n - 1, j);
+        Z(n - 1, j) = 0;
+    }
+    Z(n - 1, n - 1) = 1;
+    e[0] = 0;
+
+    // QL with implicit shifts
+    for (int i = 1; i < n; i++) e[i - 1] = e[i];
+    e[n - 1] = 0;
+    double f = 0, tst1 = 0;
+    const double eps = std::ldexp(1.0, -52);
+    for (int l = 0; l < n; l++) {
+        tst1 = std::fmax(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+        int m = l;
+        while (m < n) {
+            if (std::fabs(e[m]) <= eps * tst1) break;
+            m++;
+        }
+        if (m > l) {
+            int iter = 0;
+            do {
+                if (++iter > 60) return false;
+                double g = d[l];
+                double p = (d[l + 1] - g) / (2 * e[l]);
+                double r = std::hypot(p, 1.0);
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
+                double dl1 = d[l + 1];
+                double h = g - d[l];
+                for (int i = l + 2; i < n; i++) d[i] -= h;
+                f += h;
+                p = d[m];
+                double c = 1, c2 = 1, c3 = 1, el1 = e[l + 1], s = 0, s2 = 0;
+                for (int i = m - 1; i >= l; i--) {
+                    c3 = c2;
+                    c2 = c;
+                    s2 = s;
+                    g = c * e[i];
+                    h = c * p;
+                    r = std::hypot(p, e[i]);
+                    e[i + 1] = s * r;
+                    s = e[i] / r;
+                    c = p / r;
+                    p = c * d[i] - s * g;
+                    d[i + 1] = h + s * (c * g + s * d[i]);
+                    for (int k = 0; k < n; k++) {
+                        h = Z(k, i + 1);
+                        Z(k, i + 1) = s * Z(k, i) + c * h;
+                        Z(k, i) = c * Z(k, i) - s * h;
+                    }
+                }
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = c * p;
+            } while (std::fabs(e[l]) > eps * tst1);
+        }
+        d[l] = d[l] + f;
+        e[l] = 0;
+    }
+    // sort descending (eigenvalues and vector columns)
+    for (int i = 0; i < n - 1; i++) {
+        int k = i;
+        double p = d[i];
+        for (int j = i + 1; j < n; j++)
+            if (d[j] > p) { k = j; p = d[j]; }
+        if (k != i) {
+            d[k] = d[i];
+            d[i] = p;
+            for (int j = 0; j < n; j++) {
+                double t = Z(j, i);
+                Z(j, i) = Z(j, k);
+                Z(j, k) = t;
+            }
+        }
+    }
+    return true;
+#undef Z
+}
+
+// Q = U diag(root) V with UV = I, via the symmetric S = Pi^1/2 Q Pi^-1/2;
+// codons with pi = 0 are split off with root 0 and identity rows
+// (eigenQREV, tools.c:4850-4937).  U, V: kS x kS row-major.
+bool eigen_rev(const double *Q, const double *pi, double *root, double *U, double *V)
+{
+    int idx[kS], m = 0;
+    double sq[kS];
+    for (int i = 0; i < kS; i++)
+        if (pi[i] > 1e-100) { idx[m] = i; sq[m] = std::sqrt(pi[i]); m++; }
+    double A[kS * kS], d[kS], e[kS];
+    for (int i = 0; i < m; i++) {
+        A[i * m + i] = Q[idx[i] * kS + idx[i]];
+        for (int j = 0; j < i; j++) {
+            double s = Q[idx[i] * kS + idx[j]] * sq[i] / sq[j];
+            A[i * m + j] = s;
+            A[j * m + i] = s;
+        }
+    }
+    if (!sym_eigen(m, A, d, e)) return false;
+    std::memset(U, 0, sizeof(double) * kS * kS);
+    std::memset(V, 0, sizeof(double) * kS * kS);
+    for (int i = 0; i < kS; i++) root[i] = 0;
+    for (int i = 0; i < kS; i++)
+        if (pi[i] <= 1e-100) { U[i * kS + i] = 1; V[i * kS + i] = 1; }
+    for (int k = 0; k < m; k++) root[idx[k]] = d[k];
+    for (int i = 0; i < m; i++)
+        for (int k = 0; k < m; k++) {
+            U[idx[i] * kS + idx[k]] = A[i * m + k] / sq[i];
+            V[idx[k] * kS + idx[i]] = A[i * m + k] * sq[i];
+        }
+    for (int i = 0; i < kS; i++)
+        if (!std::isfinite(root[i])) return false;
+    return true;
+}
+
+// Model A at natural parameters (SetParametersNSsites, codeml.c:2470-2520):
+// proportions (p0, p1, (1-p0-p1)p0/(p0+p1), (1-p0-p1)p1/(p0+p1)); Qfactor of
+// background branches from the mean rate at the background-average omega
+// (p0 w0 + p1)/(p0+p1), of foreground branches at p0 w0 + p1 + (1-p0-p1) w2;
+// eigen systems for omega = w0, 1, w2.
+bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, const double *pi,
+                   ModelA *m, bool need_eigen)
+{
+    double t = p0 + p1;
+    if (!(t > 1e-100)) return false;
+    m->kappa = kappa; m->p0 = p0; m->p1 = p1; m->w0 = w0; m->w2 = w2;
+    m->freqK[0] = p0;
+    m->freqK[1] = p1;
+    m->freqK[2] = (1 - t) * p0 / t;
+    m->freqK[3] = (1 - t) * p1 / t;
+    m->wbar[0] = (p0 / t) * w0 + (p1 / t) * 1.0;
+    m->wbar[1] = p0 * w0 + p1 * 1.0 + (1 - t) * w2;
+    m->mr[0] = mean_rate(kappa, m->wbar[0], pi);
+    m->mr[1] = mean_rate(kappa, m->wbar[1], pi);
+    m->Qfactor[0] = 1 / m->mr[0];
+    m->Qfactor[1] = 1 / m->mr[1];
+    if (!need_eigen) return true;
+    const double w[3] = {w0, 1.0, w2};
+    for (int s = 0; s < 3; s++) {
+        double Q[kS * kS];
+        build_q(kappa, w[s], pi, Q);
+        if (!eigen_rev(Q, pi, m->root[s], m->U[s], m->V[s])) return false;
+    }
+    return true;
+}
+
+}  // namespace bsa

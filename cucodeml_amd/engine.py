@@ -1,0 +1,204 @@
+"""Python front end (ctypes) of the C ABI in include/bsa.h.
+
+``BranchSiteEngine`` mirrors the reference's likelihood call surface for
+branch-site model A: ``lfundG(x, np)`` returns -lnL at codeml's parameter
+vector x (CUDA_lfundG, treesub.c:7397), ``fhK`` holds the per-class
+log f(x_h | class) (com.fhK, cuda-codeml.cu:1328) and ``gradientB`` the
+finite-difference gradient ming2 uses (tools.c:6326-6350).  There is no
+fallback: if libbsa.so is missing or a HIP call fails, the call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import codons
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libbsa.so")
+
+BSA_FP64, BSA_FP32 = 0, 1
+BSA_KERNEL_MFMA, BSA_KERNEL_VALU = 0, 1
+
+EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
+            "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
+            "bsa_last_error", "bsa_version", "bsa_debug_mfma_probe", "bsa_work",
+            "bsa_model_a_setup"]
+
+
+class BsaError(RuntimeError):
+    pass
+
+
+class _Problem(C.Structure):
+    _fields_ = [("ns", C.c_int), ("nnode", C.c_int), ("root", C.c_int),
+                ("father", C.POINTER(C.c_int)), ("label", C.POINTER(C.c_int)),
+                ("ibranch", C.POINTER(C.c_int)), ("npatt", C.c_int),
+                ("z", C.POINTER(C.c_ubyte)), ("ncodes", C.c_int),
+                ("code_mask", C.POINTER(C.c_ubyte)), ("fpatt", C.POINTER(C.c_double)),
+                ("pi", C.POINTER(C.c_double)), ("precision", C.c_int), ("kernel", C.c_int),
+                ("device", C.c_int)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("fix_omega", C.c_int), ("omega_fix", C.c_double), ("p_transformed", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise BsaError(f"{LIBPATH} not built: run `python -m cucodeml_amd.build`")
+        L = C.CDLL(LIBPATH)
+        dp = C.POINTER(C.c_double)
+        L.bsa_create.argtypes = [C.POINTER(_Problem), C.POINTER(C.c_void_p)]
+        L.bsa_destroy.argtypes = [C.c_void_p]
+        L.bsa_set_model.argtypes = [C.c_void_p, C.POINTER(_Opts)]
+        L.bsa_num_params.argtypes = [C.c_void_p]
+        L.bsa_lnl.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.bsa_lnl_grad_fd.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, C.c_double, dp, dp]
+        L.bsa_class_lnl.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.bsa_last_params.argtypes = [C.c_void_p, dp]
+        L.bsa_timing.argtypes = [C.c_void_p, dp, C.c_int]
+        L.bsa_work.argtypes = [C.c_void_p, dp]
+        L.bsa_model_a_setup.argtypes = [C.c_double] * 5 + [dp] * 6
+        L.bsa_last_error.restype = C.c_char_p
+        L.bsa_version.restype = C.c_char_p
+        L.bsa_debug_mfma_probe.argtypes = [C.c_int, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BsaError(f"bsa error {rc}: {lib().bsa_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def x_from_params(nbranch_t, kappa, p0, p1, w0, w2=None, transformed=True):
+    """codeml parameter vector [t..., kappa, x_p0, x_p1, w0 (, w2)]."""
+    if transformed:
+        tot = 1.0 - p0 - p1
+        a, b = np.log(p0 / tot), np.log(p1 / tot)
+    else:
+        a, b = p0, p1
+    tail = [kappa, a, b, w0] + ([] if w2 is None else [w2])
+    return np.concatenate([np.asarray(nbranch_t, dtype=np.float64), tail])
+
+
+class BranchSiteEngine:
+    def __init__(self, tree, data, precision="fp64", kernel="mfma", device=0,
+                 fix_omega=False, omega_fix=1.0, p_transformed=True, pi=None):
+        self.tree, self.data = tree, data
+        self._keep = []
+        ns, nn = tree.ns, tree.nnode
+
+        def iarr(a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            self._keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_int))
+
+        z = np.ascontiguousarray(data.z, dtype=np.uint8)
+        mask = np.ascontiguousarray(data.code_table().astype(np.uint8))
+        fp = np.ascontiguousarray(data.fpatt, dtype=np.float64)
+        pv = np.ascontiguousarray(data.pi if pi is None else pi, dtype=np.float64)
+        self._keep += [z, mask, fp, pv]
+        lab = np.array([max(int(v), 0) for v in tree.label])
+        prob = _Problem(ns=ns, nnode=nn, root=tree.root, father=iarr(tree.father),
+                        label=iarr(lab), ibranch=iarr(tree.ibranch), npatt=data.npatt,
+                        z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
+                        code_mask=mask.ctypes.data_as(C.POINTER(C.c_ubyte)), fpatt=_dptr(fp),
+                        pi=_dptr(pv), precision=BSA_FP32 if precision == "fp32" else BSA_FP64,
+                        kernel=BSA_KERNEL_VALU if kernel == "valu" else BSA_KERNEL_MFMA,
+                        device=device)
+        h = C.c_void_p()
+        _check(lib().bsa_create(C.byref(prob), C.byref(h)))
+        self._h = h
+        self.set_model(fix_omega, omega_fix, p_transformed)
+
+    def set_model(self, fix_omega=False, omega_fix=1.0, p_transformed=True):
+        o = _Opts(int(bool(fix_omega)), float(omega_fix), int(bool(p_transformed)))
+        _check(lib().bsa_set_model(self._h, C.byref(o)))
+        self.np = lib().bsa_num_params(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bsa_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lnl(self, x):
+        """-lnL at codeml variables x (as lfundG)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = C.c_double()
+        _check(lib().bsa_lnl(self._h, _dptr(x), len(x), C.byref(out)))
+        return out.value
+
+    lfundG = lnl
+
+    def gradientB(self, x, central=False, small=1e-6):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros(len(x))
+        out = C.c_double()
+        _check(lib().bsa_lnl_grad_fd(self._h, _dptr(x), len(x), int(central), small,
+                                     C.byref(out), _dptr(g)))
+        return out.value, g
+
+    def fhK(self):
+        """(per-class log f (4, npatt), per-pattern mixture log f (npatt,))"""
+        f = np.zeros(4 * self.data.npatt)
+        s = np.zeros(self.data.npatt)
+        _check(lib().bsa_class_lnl(self._h, f.ctypes.data, s.ctypes.data))
+        return f.reshape(4, -1), s
+
+    def last_params(self):
+        o = np.zeros(11)
+        _check(lib().bsa_last_params(self._h, _dptr(o)))
+        return o
+
+    def work(self):
+        o = np.zeros(6)
+        _check(lib().bsa_work(self._h, _dptr(o)))
+        return o
+
+    def timing(self, reset=False):
+        """dict of last / summed device times (ms) and counts"""
+        o = np.zeros(12)
+        _check(lib().bsa_timing(self._h, _dptr(o), int(reset)))
+        keys = ["pmat", "prune", "root", "adjoint", "wall"]
+        d = {f"last_{k}": o[i] for i, k in enumerate(keys)}
+        d.update({f"sum_{k}": o[5 + i] for i, k in enumerate(keys)})
+        d["n_eval"], d["n_launch"] = o[10], o[11]
+        return d
+
+
+def model_a_setup(kappa, p0, p1, w0, w2, pi):
+    """Host-side model set-up: (freqK[4], Qfactor[2], Root[3,61], U[3,61,61], V[3,61,61])."""
+    pi = np.ascontiguousarray(pi, dtype=np.float64)
+    fk, qf = np.zeros(4), np.zeros(2)
+    R, U, V = np.zeros(3 * 61), np.zeros(3 * 61 * 61), np.zeros(3 * 61 * 61)
+    _check(lib().bsa_model_a_setup(kappa, p0, p1, w0, w2, _dptr(pi), _dptr(fk), _dptr(qf),
+                                   _dptr(R), _dptr(U), _dptr(V)))
+    return fk, qf, R.reshape(3, 61), U.reshape(3, 61, 61), V.reshape(3, 61, 61)
+
+
+def mfma_probe(A, B, device=0):
+    A = np.ascontiguousarray(A, dtype=np.float64).ravel()
+    B = np.ascontiguousarray(B, dtype=np.float64).ravel()
+    d64, d32 = np.zeros(256), np.zeros(256)
+    _check(lib().bsa_debug_mfma_probe(device, _dptr(A), _dptr(B), _dptr(d64), _dptr(d32)))
+    return d64.reshape(64, 4), d32.reshape(64, 4)
+
+
+__all__ = ["BranchSiteEngine", "BsaError", "x_from_params", "mfma_probe", "lib", "codons"]

@@ -1,0 +1,151 @@
+/*
+ * bsa.h — C ABI of the MI355X engine for the likelihood of codeml's
+ * branch-site model A (model = 2, NSsites = 2).
+ *
+ * One engine handle holds one gene (tree + compressed codon alignment) on
+ * one GPU.  All arrays cross the boundary as plain host pointers; every call
+ * returns a status code (0 = success) and never exits the process; the text
+ * of the last error on the calling thread is in bsa_last_error().
+ *
+ * Reference interface replaced (the reference passes everything through the
+ * globals com/tree/nodes, codeml.h:87-105):
+ *   bsa_create      <- CUDA_Init(double *x)                cuda-codeml.cu:1106
+ *                      (+ deviceInit 762-837, prepareTree 851-919)
+ *   bsa_destroy     <- CUDA_FreeAll(void)                  cuda-codeml.cu:1180
+ *   bsa_lnl         <- CUDA_lfundG(double x[], int np)     treesub.c:7397
+ *                      (CUDA_fx_r treesub.c:7554 and its four
+ *                      CUDA_ScheduleConditionalPNode calls, cuda-codeml.cu:1336)
+ *   bsa_class_lnl   <- com.fhK[ir*npatt+h] filled by CUDA_ScheduleConditionalPNode
+ *                      (cuda-codeml.cu:1328) and the lfundG site loop
+ *                      (dfsites, treesub.c:7387)
+ *   bsa_lnl_grad_fd <- gradientB (tools.c:6326-6350), the finite-difference
+ *                      gradient ming2 uses because the reference passes
+ *                      dfun = NULL (codeml.c:688)
+ */
+#ifndef BSA_H
+#define BSA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BSA_OK            0
+#define BSA_EINVAL        1   /* bad argument / inconsistent input       */
+#define BSA_ENOMEM        2   /* host or device allocation failed        */
+#define BSA_EDEVICE       3   /* HIP runtime error                       */
+#define BSA_ENUMERIC      4   /* eigen solver failed / non-finite value  */
+
+/* precision of the conditional likelihoods */
+#define BSA_FP64          0   /* plain FP64 partials                       */
+#define BSA_FP32          1   /* FP32 partials, power-of-two site scaling  */
+
+/* pruning kernel variant */
+#define BSA_KERNEL_MFMA   0   /* 16x16x4 matrix-core contraction (default) */
+#define BSA_KERNEL_VALU   1   /* plain vector FMA contraction              */
+
+typedef struct bsa_engine bsa_engine;
+
+typedef struct {
+    /* tree: tips are nodes 0..ns-1 (alignment order), interior ns..nnode-1 */
+    int ns;
+    int nnode;
+    int root;
+    const int *father;        /* [nnode], father[root] = -1                 */
+    const int *label;         /* [nnode], branch type of the edge above
+                                 each node: 0 background, 1 foreground      */
+    const int *ibranch;       /* [nnode], index of each edge in the branch-
+                                 length part of x (codeml's tree.branches
+                                 order); root -1                            */
+    /* alignment, compressed to site patterns */
+    int npatt;
+    const unsigned char *z;   /* [ns*npatt], codes 0..60 = sense codons,
+                                 61..ncodes-1 = ambiguity codes             */
+    int ncodes;
+    const unsigned char *code_mask; /* [ncodes*61], 1 where code allows the
+                                       sense codon (CharaMap, treesub.c:1146) */
+    const double *fpatt;      /* [npatt] pattern counts                     */
+    const double *pi;         /* [61] codon frequencies                     */
+    /* options */
+    int precision;            /* BSA_FP64 or BSA_FP32                       */
+    int kernel;               /* BSA_KERNEL_MFMA or BSA_KERNEL_VALU         */
+    int device;               /* HIP device ordinal                         */
+} bsa_problem;
+
+/* Parameter vector x in codeml's optimisation variables (codeml.c:1980-1981,
+ * 2080-2098):  x = [t_0 .. t_{nbranch-1}, kappa, x_p0, x_p1, w0 (, w2)]
+ * with  p0 = e^{x_p0}/(1+e^{x_p0}+e^{x_p1}),  p1 = e^{x_p1}/(...)
+ * (f_and_x, tools.c:1280-1305).  With fix_omega the vector has no w2 and
+ * w2 = omega_fix (H0 uses omega_fix = 1, codeml.c:1648-1649, 2425).
+ * With p_transformed = 0, x carries p0, p1 themselves (codeml's LASTROUND,
+ * used by in.codeml files starting with -1). */
+typedef struct {
+    int fix_omega;
+    double omega_fix;
+    int p_transformed;
+} bsa_model_opts;
+
+int  bsa_create(const bsa_problem *prob, bsa_engine **out);
+int  bsa_destroy(bsa_engine *eng);
+int  bsa_set_model(bsa_engine *eng, const bsa_model_opts *opts);
+int  bsa_num_params(const bsa_engine *eng);          /* np, or -1          */
+
+/* -lnL at x (np values), as codeml's lfundG returns it */
+int  bsa_lnl(bsa_engine *eng, const double *x, int np, double *neg_lnl);
+
+/* -lnL and its finite-difference gradient d(-lnL)/dx in the same variables,
+ * as gradientB (tools.c:6326-6350): step small*(|x_i|+1) (small <= 0 means
+ * 1e-6, Small_Diff), forward differences (np+1 evaluations) or, with
+ * central != 0, central differences with step^0.67 (2np+1 evaluations) */
+int  bsa_lnl_grad_fd(bsa_engine *eng, const double *x, int np, int central, double small,
+                     double *neg_lnl, double *grad);
+
+/* after bsa_lnl / bsa_lnl_grad: per-class log f(x_h | class) [4*npatt]
+ * (class-major, as com.fhK with node scaling) and the mixture log f_h
+ * [npatt]; either pointer may be NULL */
+int  bsa_class_lnl(bsa_engine *eng, double *fhK, double *site_lnf);
+
+/* the parameters of the last evaluation in natural form:
+ * out[0..4] = kappa, p0, p1, w0, w2; out[5..6] = Qfactor background,
+ * foreground; out[7..10] = class proportions */
+int  bsa_last_params(const bsa_engine *eng, double *out11);
+
+/* timing in ms, from HIP events on the engine's stream: out[0] P(t) and tip
+ * tables, out[1] pruning (all levels), out[2] root reduction, out[3]
+ * adjoint (0 if none), out[4] host wall time of the whole evaluation, all
+ * for the last evaluation; out[5..9] the same summed since creation or the
+ * last reset; out[10] evaluations and out[11] pruning launches in that sum.
+ * reset != 0 clears the sums after reading them. */
+int  bsa_timing(bsa_engine *eng, double *out12, int reset);
+
+/* host only (no GPU): model A at natural parameters, as SetParametersNSsites
+ * (codeml.c:2470-2520) with eigenQcodon/eigenQREV (codeml.c:3103-3272,
+ * tools.c:4850-4937): freqK[4] class proportions, Qfactor[2] (background,
+ * foreground), and when Root, U, V are non-NULL the eigen systems for
+ * omega = w0, 1, w2: Root[3][61], U[3][61*61], V[3][61*61], Q = U diag(Root) V */
+int  bsa_model_a_setup(double kappa, double p0, double p1, double w0, double w2,
+                       const double *pi61, double *freqK4, double *Qfactor2, double *Root,
+                       double *U, double *V);
+
+/* work of one evaluation's pruning pass, counted on 61 states:
+ * out[0] flops executed (site classes 2/3 share partials with 0/1 off the
+ * foreground path), out[1] algorithmic HBM bytes executed (partials read
+ * and written, tip codes, scale exponents), out[2] flops and out[3] bytes of
+ * the reference algorithm (4 classes over the whole tree), out[4] pruning
+ * launches (tree levels), out[5] (node, class) jobs */
+int  bsa_work(const bsa_engine *eng, double *out6);
+
+const char *bsa_last_error(void);
+
+/* diagnostic: one 16x16x4 matrix-core product in FP64 and in FP32 of
+ * A (16x4, row-major) and B (4x16) on `device`; D64/D32 get each lane's four
+ * accumulator values, lane-major [64][4] (no counterpart in the reference;
+ * used by the tests to pin the lane layout the pruning kernel assumes) */
+int  bsa_debug_mfma_probe(int device, const double *A, const double *B, double *D64,
+                          double *D32);
+const char *bsa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

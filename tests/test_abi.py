@@ -1,0 +1,144 @@
+"""The C ABI library: it loads, exports every entry point include/bsa.h
+declares, and rejects bad problems with a status code and a message (these
+checks run before any HIP call, so they need no GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cucodeml_amd import engine, formats, simulate
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "bsa.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\s*\*)\s*(bsa_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = engine.lib()
+    names = header_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in bsa.h but not exported"
+    assert sorted(engine.EXPORTED) == names
+    assert b"gfx950" in L.bsa_version()
+
+
+def _problem(tree, data, **over):
+    keep = []
+
+    def ia(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_int))
+    z = np.ascontiguousarray(data.z, dtype=np.uint8)
+    m = np.ascontiguousarray(data.code_table().astype(np.uint8))
+    fp = np.ascontiguousarray(data.fpatt)
+    pi = np.ascontiguousarray(data.pi)
+    keep += [z, m, fp, pi]
+    kw = dict(ns=tree.ns, nnode=tree.nnode, root=tree.root, father=ia(tree.father),
+              label=ia(np.maximum(tree.label, 0)), ibranch=ia(tree.ibranch), npatt=data.npatt,
+              z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
+              code_mask=m.ctypes.data_as(C.POINTER(C.c_ubyte)),
+              fpatt=fp.ctypes.data_as(C.POINTER(C.c_double)),
+              pi=pi.ctypes.data_as(C.POINTER(C.c_double)), precision=0, kernel=0, device=0)
+    for k, v in over.items():
+        kw[k] = ia(v) if isinstance(v, np.ndarray) else v
+    return engine._Problem(**kw), keep
+
+
+@pytest.fixture(scope="module")
+def small():
+    rng = np.random.default_rng(4)
+    tree = simulate.yule_tree(6, 0.1, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    seqs, _ = simulate.simulate_model_a(tree, 40, 2, 0.1, 3, 0.5, 0.3, pi, rng)
+    return tree, formats.load_codon_data(tree.names, seqs)
+
+
+@pytest.mark.parametrize("change, msg", [
+    (dict(ns=2), "at least 3"),
+    (dict(root=0), "interior"),
+    (dict(npatt=0), "npatt"),
+    (dict(precision=7), "precision"),
+    (dict(kernel=5), "kernel"),
+    (dict(ncodes=300), "ncodes"),
+])
+def test_create_rejects_bad_problems(small, change, msg):
+    tree, data = small
+    prob, keep = _problem(tree, data, **change)
+    h = C.c_void_p()
+    rc = engine.lib().bsa_create(C.byref(prob), C.byref(h))
+    assert rc == 1 and not h.value
+    assert msg in engine.lib().bsa_last_error().decode()
+
+
+def test_create_rejects_bad_trees(small):
+    tree, data = small
+    lab = np.maximum(tree.label, 0).copy()
+    lab[lab == 1] = 0                       # no foreground branch
+    prob, keep = _problem(tree, data, label=lab)
+    h = C.c_void_p()
+    assert engine.lib().bsa_create(C.byref(prob), C.byref(h)) == 1
+    assert "foreground" in engine.lib().bsa_last_error().decode()
+    lab = np.maximum(tree.label, 0).copy()
+    lab[0] = 2                              # model A has two branch types
+    prob, keep = _problem(tree, data, label=lab)
+    assert engine.lib().bsa_create(C.byref(prob), C.byref(h)) == 1
+    ib = tree.ibranch.copy()
+    ib[0] = ib[1]                           # not a permutation
+    prob, keep = _problem(tree, data, ibranch=ib)
+    assert engine.lib().bsa_create(C.byref(prob), C.byref(h)) == 1
+    assert "permutation" in engine.lib().bsa_last_error().decode()
+    fa = tree.father.copy()
+    fa[0] = 1                               # a tip as father
+    prob, keep = _problem(tree, data, father=fa)
+    assert engine.lib().bsa_create(C.byref(prob), C.byref(h)) == 1
+
+
+def test_null_arguments_are_errors():
+    L = engine.lib()
+    assert L.bsa_create(None, None) == 1
+    assert L.bsa_lnl(None, None, 0, None) == 1
+    assert L.bsa_destroy(None) == 0
+    assert L.bsa_num_params(None) == -1
+
+
+def test_x_from_params_roundtrip():
+    x = engine.x_from_params([0.1, 0.2, 0.3], 2.0, 0.5, 0.3, 0.1, 3.0)
+    a, b = np.exp(x[4]), np.exp(x[5])
+    assert np.allclose([a / (1 + a + b), b / (1 + a + b)], [0.5, 0.3])
+    assert len(engine.x_from_params([0.1] * 3, 2.0, 0.5, 0.3, 0.1)) == 7
+
+
+def test_host_model_setup_matches_oracle():
+    """The product's host eigen route (Householder + QL) against the oracle's
+    (Jacobi): proportions, Qfactors, eigenvalues and P(t) = U e^{Rt} V."""
+    from oracle import oracle
+    rng = np.random.default_rng(8)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    par = (2.7, 0.55, 0.25, 0.07, 4.5)
+    fk, qf, R, U, V = engine.model_a_setup(*par, pi)
+    fko, qfo, Ro, Uo, Vo = oracle.model(*par, pi)
+    assert np.allclose(fk, fko, rtol=1e-15) and np.allclose(qf, qfo, rtol=1e-14)
+    for s in range(3):
+        assert np.max(np.abs(np.sort(R[s]) - np.sort(Ro[s]))) < 1e-13
+        assert np.max(np.abs(U[s] @ V[s] - np.eye(61))) < 1e-12
+        for t in (0.01, 0.2, 1.5):
+            P = (U[s] * np.exp(R[s] * t)) @ V[s]
+            Po = oracle.pmat(Uo[s], Vo[s], Ro[s], t)
+            assert np.max(np.abs(np.clip(P, 0, None) - Po)) < 1e-13
+    # a codon with zero frequency is split off with root 0 and identity rows
+    pi0 = pi.copy()
+    pi0[[5, 30]] = 0
+    pi0 /= pi0.sum()
+    fk, qf, R, U, V = engine.model_a_setup(*par, pi0)
+    Q, _ = oracle.q_matrix(par[0], par[3], pi0)
+    keep = pi0 > 0
+    QQ = U[0] @ np.diag(R[0]) @ V[0]
+    assert np.max(np.abs((QQ - Q)[np.ix_(keep, keep)])) < 1e-12
+    assert U[0][5, 5] == 1 and V[0][30, 30] == 1 and R[0][5] == 0
