@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--kernel", choices=["mfma", "valu"], default="mfma")
+    ap.add_argument("--tile", type=int, default=0, help="patterns per pruning block (64/128)")
     ap.add_argument("--cpu-sample-codons", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -127,7 +128,8 @@ def main():
         sys.exit(2)
 
     tree, data, p, seqs = make_workload(args.taxa, args.codons, args.seed)
-    eng = engine.BranchSiteEngine(tree, data, precision=args.precision, kernel=args.kernel)
+    eng = engine.BranchSiteEngine(tree, data, precision=args.precision, kernel=args.kernel,
+                                  tile=args.tile)
     x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"],
                              p["w2"])
     npar = len(x)
@@ -166,10 +168,12 @@ def main():
                                f"({data.npatt} patterns), {args.precision} partials, "
                                f"forward-difference gradient ({npar + 1} lnL per step)",
                    "taxa": args.taxa, "codons": args.codons, "patterns": data.npatt,
-                   "np": npar, "gradient": "fd-forward", "kernel": args.kernel},
+                   "np": npar, "gradient": "fd-forward", "kernel": args.kernel,
+                   "tile": args.tile or 64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_prune_{args.kernel}<{'double' if w == 8 else 'float'}>",
+                     "kernel": f"k_prune_{args.kernel}<{'double' if w == 8 else 'float'}>"
+                               " (FP64 arithmetic, storage as named)",
                      "bytes_per_lnl": bytes_per_eval, "flops_per_lnl": work[0],
                      "launches_per_lnl": work[4],
                      "avg_launch_us": prune_ms * 1e3 / max(launches, 1),

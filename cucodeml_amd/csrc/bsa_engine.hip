@@ -47,7 +47,7 @@ constexpr int kUVRSet = 64 * 64 * 2 + 64;
 struct bsa_engine {
     int dev = 0;
     int ns = 0, nnode = 0, root = 0, nbranch = 0, npatt = 0, npad = 0, ncodes = 0, namb = 0;
-    int precision = BSA_FP64, kernel = BSA_KERNEL_MFMA;
+    int precision = BSA_FP64, kernel = BSA_KERNEL_MFMA, tile = 64;
     std::vector<int> father, label, ibranch, height;
     std::vector<std::vector<int>> sons;
     std::vector<char> onpath;
@@ -233,10 +233,10 @@ static int upload_static(bsa_engine *e, const bsa_problem *p)
     HIPCHK(hipMalloc(&e->d_uvr, sizeof(double) * 3 * kUVRSet));
     HIPCHK(hipMalloc(&e->d_pjobs, sizeof(PmatJob) * e->pjobs.size()));
     HIPCHK(hipMalloc(&e->d_prune, sizeof(PruneJob) * std::max<size_t>(1, e->prune.size())));
-    HIPCHK(hipMalloc(&e->d_PT, ts * 4096 * (size_t)e->nnode * 3));
-    HIPCHK(hipMemset(e->d_PT, 0, ts * 4096 * (size_t)e->nnode * 3));
+    HIPCHK(hipMalloc(&e->d_PT, sizeof(double) * 4096 * (size_t)e->nnode * 3));
+    HIPCHK(hipMemset(e->d_PT, 0, sizeof(double) * 4096 * (size_t)e->nnode * 3));
     if (e->namb > 0) {
-        HIPCHK(hipMalloc(&e->d_AT, ts * 64 * (size_t)e->namb * e->ns * 3));
+        HIPCHK(hipMalloc(&e->d_AT, sizeof(double) * 64 * (size_t)e->namb * e->ns * 3));
         HIPCHK(hipMalloc(&e->d_mask_amb, 64 * (size_t)e->namb));
     }
     HIPCHK(hipMalloc(&e->d_L, ts * 64 * (size_t)np * e->nslot));
@@ -246,7 +246,7 @@ static int upload_static(bsa_engine *e, const bsa_problem *p)
     HIPCHK(hipMalloc(&e->d_fpatt, sizeof(double) * np));
     HIPCHK(hipMalloc(&e->d_fh, sizeof(double) * 4 * (size_t)np));
     HIPCHK(hipMalloc(&e->d_site, sizeof(double) * np));
-    e->nroot_blk = (e->npatt + 63) / 64;
+    e->nroot_blk = (e->npatt + 15) / 16;   // k_root: 16 patterns per block
     HIPCHK(hipMalloc(&e->d_partial, sizeof(double) * 2 * e->nroot_blk));
     HIPCHK(hipMalloc(&e->d_out, sizeof(double) * 8));
     HIPCHK(hipHostMalloc((void **)&e->h_uvr, sizeof(double) * 3 * kUVRSet, hipHostMallocDefault));
@@ -301,7 +301,13 @@ extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
     e->root = p->root;
     e->nbranch = p->nnode - 1;
     e->npatt = p->npatt;
-    e->npad = (p->npatt + kTileP - 1) / kTileP * kTileP;
+    e->npad = (p->npatt + kPadP - 1) / kPadP * kPadP;
+    e->tile = p->tile == 0 ? 64 : p->tile;
+    if (e->tile != 64 && e->tile != 128) {
+        int t = e->tile;
+        delete e;
+        return fail(BSA_EINVAL, "tile %d: 64 or 128 patterns per block", t);
+    }
     e->ncodes = p->ncodes;
     e->namb = p->ncodes - kS;
     e->precision = p->precision;
@@ -403,14 +409,15 @@ static int run_device(bsa_engine *e)
     HIPCHK(hipMemcpyAsync(e->d_pjobs, e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
                           hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(e->ev[0], s));
-    launch_pmat<T>(s, e->d_uvr, e->d_pjobs, (int)e->pjobs.size(), (T *)e->d_PT, e->d_mask_amb,
-                   e->namb, (T *)e->d_AT);
+    launch_pmat(s, e->d_uvr, e->d_pjobs, (int)e->pjobs.size(), (double *)e->d_PT, e->d_mask_amb,
+                e->namb, (double *)e->d_AT);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev[1], s));
     for (size_t l = 0; l + 1 < e->level_off.size(); l++) {
         int a = e->level_off[l], b = e->level_off[l + 1];
-        launch_prune<T>(s, e->kernel, e->d_prune + a, b - a, e->npad, e->d_z, (const T *)e->d_PT,
-                        (const T *)e->d_AT, e->namb, (T *)e->d_L, e->d_E);
+        launch_prune<T>(s, e->kernel, e->tile, e->d_prune + a, b - a, e->npad, e->d_z,
+                        (const double *)e->d_PT, (const double *)e->d_AT, e->namb, (T *)e->d_L,
+                        e->d_E);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev[2], s));

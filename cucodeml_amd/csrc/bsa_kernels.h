@@ -4,7 +4,7 @@
 
 namespace bsa {
 
-constexpr int kTileP = 64;      // patterns per pruning tile
+constexpr int kPadP = 128;      // patterns are padded to a multiple of this
 constexpr int kMaxCh = 8;       // children per node handled by one job
 
 // One (node, site class) job of the pruning pass: multiply the child
@@ -30,14 +30,15 @@ struct RootArgs {
     double freqK[4];
 };
 
-// launchers; T = double or float
-template <typename T>
-void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, T *PT,
-                 const unsigned char *mask_amb, int namb, T *AT);
+// launchers; T = storage type of the conditional likelihoods (double or float)
+void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
+                 const unsigned char *mask_amb, int namb, double *AT);
 
+// kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64)
 template <typename T>
-void launch_prune(hipStream_t s, int kernel, const PruneJob *jobs, int njobs, int npad,
-                  const unsigned char *z, const T *PT, const T *AT, int namb, T *L, int *E);
+void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
+                  const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
+                  int *E);
 
 template <typename T>
 void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,

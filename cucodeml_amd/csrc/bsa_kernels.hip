@@ -1,19 +1,28 @@
 // HIP kernels for gfx950 (MI355X) of the branch-site model A likelihood:
-//   k_pmat        batched P(t) = U diag(exp(root t)) V for every (branch, eigen set)
+//   k_pmat        batched P(t) - I = U diag(exp(root t)) V - I, every (branch, eigen set)
 //   k_amb         per-tip rows for ambiguous codons (sums of P columns)
-//   k_prune_mfma  one tree level of Felsenstein pruning on the matrix cores
+//   k_prune_mfma  one tree level of Felsenstein pruning on the FP64 matrix cores
 //   k_prune_valu  the same on plain vector FMA
 //   k_root        per-pattern class log-likelihoods, mixture and weighted sum
 //
-// Layouts (T = double or float):
-//   PT[pm][k][j]    = P_{j,k}(t) for matrix pm = node*3 + set, 64 x 64, rows and
-//                     columns >= 61 are zero (k: child state, j: parent state)
-//   L[slot][h][j]   conditional likelihood of a node / site class, pattern h,
-//                   parent-side state j (64, lanes >= 61 stay exactly zero)
+// Arithmetic is FP64 throughout; T (double or float) is only the storage type
+// of the conditional likelihoods.  Layouts:
+//   PT[pm][k][j]    = P_{j,k}(t) - delta_{jk} (double) for matrix pm = node*3 + set,
+//                     64 x 64, rows and columns >= 61 zero (k: child state,
+//                     j: parent state).  The identity part is added exactly
+//                     after the product, so small off-diagonal terms are
+//                     summed on their own scale instead of being swamped by
+//                     the near-1 diagonal.
+//   L[slot][h][j]   (T) conditional likelihood of a node / site class, pattern
+//                   h, parent-side state j (64, lanes >= 61 stay exactly zero)
 //   E[slot][h]      accumulated power-of-two exponent: true value = L * 2^E
+// FP32 storage rounds stochastically with a hash of (slot, pattern, state) as
+// the random bits: unbiased, uncorrelated across patterns that share
+// sub-trees, and bitwise repeatable.
 #include "bsa_kernels.h"
 
 #include <math.h>
+#include <stdint.h>
 
 namespace bsa {
 
@@ -23,16 +32,17 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kUVR = 64 * 64 * 2 + 64;   // per eigen set: U 64x64, V 64x64, root 64
 constexpr double kLn2 = 0.69314718055994530942;
+constexpr double kLogFloor = -690.77552789821368;   // log(1e-300)
+constexpr int kRootPerWave = 4;    // patterns per wave in k_root (16 per block)
 
 // ------------------------------------------------------------------ P(t)
 
 // One block per (branch, set).  U is staged transposed so that the 64 lanes
 // of a wave (64 consecutive parent states j) read consecutive LDS words; the
 // wave's child-state index is uniform, so V reads are broadcasts.
-template <typename T>
 __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
                                               const PmatJob *__restrict__ jobs,
-                                              T *__restrict__ PT)
+                                              double *__restrict__ PT)
 {
     __shared__ double Ut[64 * 64];   // Ut[m][j] = U[j][m]
     __shared__ double Ve[64 * 64];   // Ve[m][k] = exp(root_m t) V[m][k]
@@ -40,13 +50,10 @@ __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
     const double *U = uvr + (size_t)jb.set * kUVR;
     const double *V = U + 4096;
     const double *R = U + 8192;
-    T *out = PT + (size_t)jb.pm * 4096;
+    double *out = PT + (size_t)jb.pm * 4096;
     const int tid = threadIdx.x;
     if (jb.t < 1e-100) {   // PMatUVRoot returns the identity (tools.c:527-530)
-        for (int q = tid; q < 4096; q += 256) {
-            int k = q >> 6, j = q & 63;
-            out[q] = (T)((k == j && k < 61) ? 1.0 : 0.0);
-        }
+        for (int q = tid; q < 4096; q += 256) out[q] = 0.0;
         return;
     }
     for (int q = tid; q < 4096; q += 256) {
@@ -66,53 +73,62 @@ __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
     }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
+        const int k = kq + 4 * q;
         double p = acc[q];
         if (p < 0) p = 0;   // negatives clamped (tools.c:536-538)
-        out[(kq + 4 * q) * 64 + j] = (T)p;
+        if (k == j && k < 61) p -= 1.0;
+        out[k * 64 + j] = p;
     }
 }
 
-// Rows of ambiguous codes for tip branches: AT[pm][a][j] = sum_{k in a} P_{j,k}.
-template <typename T>
+// Rows of ambiguous codes for tip branches: AT[pm][a][j] = sum_{k in a} P_{j,k}
+// (full P, not P - I).
 __global__ __launch_bounds__(256) void k_amb(const PmatJob *__restrict__ jobs,
-                                             const T *__restrict__ PT,
+                                             const double *__restrict__ PT,
                                              const unsigned char *__restrict__ mask_amb, int namb,
-                                             T *__restrict__ AT)
+                                             double *__restrict__ AT)
 {
     const PmatJob jb = jobs[blockIdx.x];
     if (jb.tip < 0) return;
-    const T *P = PT + (size_t)jb.pm * 4096;
-    T *out = AT + (size_t)jb.pm * namb * 64;
+    const double *P = PT + (size_t)jb.pm * 4096;
+    double *out = AT + (size_t)jb.pm * namb * 64;
     const int j = threadIdx.x & 63;
     for (int a = threadIdx.x >> 6; a < namb; a += 4) {
         double s = 0;
         for (int k = 0; k < 61; k++)
-            if (mask_amb[a * 64 + k]) s += (double)P[k * 64 + j];
-        out[a * 64 + j] = (T)s;
+            if (mask_amb[a * 64 + k]) s += P[k * 64 + j] + (k == j ? 1.0 : 0.0);
+        out[a * 64 + j] = s;
     }
 }
 
 // ------------------------------------------------------------------ helpers
 
-__device__ inline double dev_frexp(double x, int *e) { return frexp(x, e); }
-__device__ inline float dev_frexp(float x, int *e) { return frexpf(x, e); }
-__device__ inline double dev_ldexp(double x, int e) { return ldexp(x, e); }
-__device__ inline float dev_ldexp(float x, int e) { return ldexpf(x, e); }
-
-template <typename T>
-__device__ inline T tip_value(const T *P, const T *AT, int pm, int namb, int code, int j)
+// contribution of a tip with code `code` to parent state j: P_{j,code}, or the
+// ambiguity row sum
+__device__ inline double tip_value(const double *P, const double *AT, int pm, int namb, int code,
+                                   int j)
 {
-    return code < 61 ? P[code * 64 + j] : AT[((size_t)pm * namb + (code - 61)) * 64 + j];
+    if (code < 61) return P[code * 64 + j] + (code == j ? 1.0 : 0.0);
+    return AT[((size_t)pm * namb + (code - 61)) * 64 + j];
 }
 
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-    typedef d4 V;
-    static __device__ inline V op(double a, double b, V c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // 16 consecutive doubles from a 128-byte aligned address
+__device__ inline uint32_t mix32(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t x = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ c * 0xC2B2AE3Du;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// storage of conditional likelihoods
+template <typename T> struct Store;
+template <> struct Store<double> {
+    static __device__ inline double get(const double *p) { return *p; }
+    static __device__ inline void put(double *p, double v, uint32_t) { *p = v; }
+    // 16 consecutive values from a 128-byte aligned address
     static __device__ inline void load16(const double *p, double *a)
     {
         const d2 *q = reinterpret_cast<const d2 *>(p);
@@ -124,13 +140,18 @@ template <> struct Mfma<double> {
         }
     }
 };
-template <> struct Mfma<float> {
-    typedef f4 V;
-    static __device__ inline V op(float a, float b, V c)
+template <> struct Store<float> {
+    static __device__ inline double get(const float *p) { return (double)*p; }
+    // stochastic rounding of v >= 0: add 29 random bits below the float
+    // mantissa of the double, then truncate (exact conversion afterwards)
+    static __device__ inline void put(float *p, double v, uint32_t r)
     {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+        unsigned long long u = (unsigned long long)__double_as_longlong(v);
+        u += (unsigned long long)(r & 0x1FFFFFFFu);
+        u &= ~0x1FFFFFFFull;
+        *p = (float)__longlong_as_double((long long)u);
     }
-    static __device__ inline void load16(const float *p, float *a)
+    static __device__ inline void load16(const float *p, double *a)
     {
         const f4 *q = reinterpret_cast<const f4 *>(p);
 #pragma unroll
@@ -144,116 +165,148 @@ template <> struct Mfma<float> {
     }
 };
 
+__device__ inline d4 mfma64(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 // ------------------------------------------------------------------ pruning (MFMA)
 
-// Block = 4 waves = one tile of 64 patterns x 64 parent states of one
-// (node, class) job; wave w owns patterns 16w..16w+15.  Interior children are
-// a (16 x 64) x (64 x 64) product per wave as 16 k-steps x 4 column blocks of
-// the 16x16x4 MFMA.  The k order inside the product is free, so lane group
-// g = lane/16 takes child states 16g..16g+15 at steps 0..15: each lane then
-// reads 16 consecutive values of its pattern row straight from HBM.
-// Accumulator layout: lane l, element i of column block jb holds
-// pattern 16w + 4(l/16) + i, parent state 16 jb + l%16.
-template <typename T>
+// Block = 4 waves, one (node, class) job, a tile of 64*RG patterns x 64 parent
+// states; wave w owns 16-pattern row groups 64 rg + 16 w (rg < RG), so the P
+// tile staged in LDS for a child serves 64*RG patterns.  Interior children
+// are (16 x 64) x (64 x 64) products per row group: 16 k-steps x 4 column
+// blocks of the FP64 16x16x4 MFMA with B = P - I from LDS; the child's own
+// values (the identity part) are added after the accumulation.  The k order
+// is free, so lane group g = lane/16 takes child states 16g..16g+15 at steps
+// 0..15 and each lane reads 16 consecutive values of its pattern row straight
+// from HBM.  Accumulator element i of column block b of lane l is pattern
+// (row group base) + l/16 + 4i, parent state 16b + l%16.
+template <typename T, int RG>
 __global__ __launch_bounds__(256) void k_prune_mfma(const PruneJob *__restrict__ jobs, int npad,
                                                     const unsigned char *__restrict__ z,
-                                                    const T *__restrict__ PT,
-                                                    const T *__restrict__ AT, int namb,
+                                                    const double *__restrict__ PT,
+                                                    const double *__restrict__ AT, int namb,
                                                     T *__restrict__ L, int *__restrict__ E)
 {
-    typedef typename Mfma<T>::V V;
-    __shared__ T sP[64 * 64];
+    __shared__ double sP[64 * 64];
     const PruneJob &jb = jobs[blockIdx.y];
-    const int h0 = blockIdx.x * kTileP;
+    const int h0 = blockIdx.x * (64 * RG);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
-    const int hrow = h0 + 16 * w + 4 * g;        // first of this lane's 4 output patterns
-    V acc[4];
-    int esum[4] = {0, 0, 0, 0};
+    d4 acc[RG][4];
+    int esum[RG][4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[b] = V{1, 1, 1, 1};
+    for (int q = 0; q < RG; q++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            acc[q][b] = d4{1, 1, 1, 1};
+            esum[q][b] = 0;
+        }
     const int nch = jb.nch;
     for (int c = 0; c < nch; c++) {
         const int pm = jb.ch_pmat[c];
-        const T *P = PT + (size_t)pm * 4096;
+        const double *P = PT + (size_t)pm * 4096;
         const int tip = jb.ch_tip[c];
         if (tip >= 0) {
-            const unsigned char *zt = z + (size_t)tip * npad + hrow;
-            int code[4];
+            const unsigned char *zt = z + (size_t)tip * npad;
 #pragma unroll
-            for (int i = 0; i < 4; i++) code[i] = zt[i];
+            for (int q = 0; q < RG; q++) {
+                const int hb = h0 + 64 * q + 16 * w + g;
+                int code[4];
 #pragma unroll
-            for (int b = 0; b < 4; b++)
+                for (int i = 0; i < 4; i++) code[i] = zt[hb + 4 * i];
 #pragma unroll
-                for (int i = 0; i < 4; i++)
-                    acc[b][i] *= tip_value(P, AT, pm, namb, code[i], 16 * b + r);
+                for (int b = 0; b < 4; b++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        acc[q][b][i] *= tip_value(P, AT, pm, namb, code[i], 16 * b + r);
+            }
         } else {
             const int slot = jb.ch_slot[c];
+            const T *Lc = L + (size_t)slot * npad * 64;
+            const int *Ec = E + (size_t)slot * npad;
             __syncthreads();
             for (int q = tid; q < 4096; q += 256) sP[q] = P[q];
             __syncthreads();
-            T a[16];
-            Mfma<T>::load16(L + ((size_t)slot * npad + h0 + 16 * w + r) * 64 + 16 * g, a);
-            V tmp[4];
 #pragma unroll
-            for (int b = 0; b < 4; b++) tmp[b] = V{0, 0, 0, 0};
+            for (int q = 0; q < RG; q++) {
+                const int hw = h0 + 64 * q + 16 * w;
+                const int hb = hw + g;
+                double a[16];
+                Store<T>::load16(Lc + (size_t)(hw + r) * 64 + 16 * g, a);
+                d4 tmp[4];
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) {
-                const T *prow = sP + (16 * g + kk) * 64 + r;
+                for (int b = 0; b < 4; b++) tmp[b] = d4{0, 0, 0, 0};
 #pragma unroll
-                for (int b = 0; b < 4; b++) tmp[b] = Mfma<T>::op(a[kk], prow[16 * b], tmp[b]);
+                for (int kk = 0; kk < 16; kk++) {
+                    const double *prow = sP + (16 * g + kk) * 64 + r;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) tmp[b] = mfma64(a[kk], prow[16 * b], tmp[b]);
+                }
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        acc[q][b][i] *=
+                            tmp[b][i] + Store<T>::get(Lc + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
+#pragma unroll
+                for (int i = 0; i < 4; i++) esum[q][i] += Ec[hb + 4 * i];
             }
-#pragma unroll
-            for (int b = 0; b < 4; b++) acc[b] *= tmp[b];
-            const int *Ec = E + (size_t)slot * npad + hrow;
-#pragma unroll
-            for (int i = 0; i < 4; i++) esum[i] += Ec[i];
         }
     }
-    // per-pattern power-of-two rescaling: max over the 64 states of a row lives
-    // in the 4 column blocks of the 16 lanes of one lane group
-    T *Lo = L + ((size_t)jb.out_slot * npad + hrow) * 64;
+    // per-pattern power-of-two rescaling: the 64 states of a row live in the
+    // 4 column blocks of the 16 lanes of one lane group
+    T *Lo = L + (size_t)jb.out_slot * npad * 64;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        T m = acc[0][i];
+    for (int q = 0; q < RG; q++) {
+        const int hb = h0 + 64 * q + 16 * w + g;
 #pragma unroll
-        for (int b = 1; b < 4; b++) m = acc[b][i] > m ? acc[b][i] : m;
+        for (int i = 0; i < 4; i++) {
+            double m = acc[q][0][i];
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            T o = __shfl_xor(m, off, 64);
-            m = o > m ? o : m;
+            for (int b = 1; b < 4; b++) m = acc[q][b][i] > m ? acc[q][b][i] : m;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                double o = __shfl_xor(m, off, 64);
+                m = o > m ? o : m;
+            }
+            int e = 0;
+            if (m > 0) frexp(m, &e);
+            const int h = hb + 4 * i;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                Store<T>::put(Lo + (size_t)h * 64 + 16 * b + r, ldexp(acc[q][b][i], -e),
+                              mix32((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)(16 * b + r)));
+            if (r == 0) E[(size_t)jb.out_slot * npad + h] = esum[q][i] + e;
         }
-        int e = 0;
-        if (m > 0) dev_frexp(m, &e);
-#pragma unroll
-        for (int b = 0; b < 4; b++) Lo[i * 64 + 16 * b + r] = dev_ldexp(acc[b][i], -e);
-        if (r == 0) E[(size_t)jb.out_slot * npad + hrow + i] = esum[i] + e;
     }
 }
 
 // ------------------------------------------------------------------ pruning (VALU)
 
-// Same job on vector FMA: thread = parent state j (lane) x 16 patterns (wave).
+// Same job on vector FMA, tile of 64 patterns: thread = parent state j (lane)
+// x 16 patterns (wave); identity part added after the product as above.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__ jobs, int npad,
                                                     const unsigned char *__restrict__ z,
-                                                    const T *__restrict__ PT,
-                                                    const T *__restrict__ AT, int namb,
+                                                    const double *__restrict__ PT,
+                                                    const double *__restrict__ AT, int namb,
                                                     T *__restrict__ L, int *__restrict__ E)
 {
-    __shared__ T sP[64 * 64];
-    __shared__ T sA[64 * 64];
+    __shared__ double sP[64 * 64];
+    __shared__ double sA[64 * 64];
     const PruneJob &jb = jobs[blockIdx.y];
-    const int h0 = blockIdx.x * kTileP;
+    const int h0 = blockIdx.x * 64;
     const int tid = threadIdx.x, j = tid & 63, pg = tid >> 6;
     const int hb = h0 + 16 * pg;
-    T acc[16];
+    double acc[16];
     int esum[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) { acc[i] = 1; esum[i] = 0; }
     const int nch = jb.nch;
     for (int c = 0; c < nch; c++) {
         const int pm = jb.ch_pmat[c];
-        const T *P = PT + (size_t)pm * 4096;
+        const double *P = PT + (size_t)pm * 4096;
         const int tip = jb.ch_tip[c];
         if (tip >= 0) {
             const unsigned char *zt = z + (size_t)tip * npad + hb;
@@ -263,41 +316,48 @@ __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__
             const int slot = jb.ch_slot[c];
             const T *A = L + ((size_t)slot * npad + h0) * 64;
             __syncthreads();
-            for (int q = tid; q < 4096; q += 256) { sP[q] = P[q]; sA[q] = A[q]; }
+            for (int q = tid; q < 4096; q += 256) { sP[q] = P[q]; sA[q] = Store<T>::get(A + q); }
             __syncthreads();
-            T tmp[16];
+            double tmp[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) tmp[i] = 0;
             for (int k = 0; k < 64; k++) {
-                T p = sP[k * 64 + j];
+                double p = sP[k * 64 + j];
 #pragma unroll
                 for (int i = 0; i < 16; i++) tmp[i] = fma(sA[(16 * pg + i) * 64 + k], p, tmp[i]);
             }
             const int *Ec = E + (size_t)slot * npad + hb;
 #pragma unroll
-            for (int i = 0; i < 16; i++) { acc[i] *= tmp[i]; esum[i] += Ec[i]; }
+            for (int i = 0; i < 16; i++) {
+                acc[i] *= tmp[i] + sA[(16 * pg + i) * 64 + j];
+                esum[i] += Ec[i];
+            }
         }
     }
-    T *Lo = L + ((size_t)jb.out_slot * npad + hb) * 64;
+    T *Lo = L + (size_t)jb.out_slot * npad * 64;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        T m = acc[i];
+        double m = acc[i];
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            T o = __shfl_xor(m, off, 64);
+            double o = __shfl_xor(m, off, 64);
             m = o > m ? o : m;
         }
         int e = 0;
-        if (m > 0) dev_frexp(m, &e);
-        Lo[i * 64 + j] = dev_ldexp(acc[i], -e);
-        if (j == 0) E[(size_t)jb.out_slot * npad + hb + i] = esum[i] + e;
+        if (m > 0) frexp(m, &e);
+        const int h = hb + i;
+        Store<T>::put(Lo + (size_t)h * 64 + j, ldexp(acc[i], -e),
+                      mix32((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)j));
+        if (j == 0) E[(size_t)jb.out_slot * npad + h] = esum[i] + e;
     }
 }
 
 // ------------------------------------------------------------------ root
 
 // One wave per pattern at a time, lane = state.  Per class: f = sum_i pi_i L_i
-// (<= 0 -> 1e-300, treesub.c:7522-7533), l_c = log f + E ln 2; site value
+// (<= 0 -> 1e-300, treesub.c:7522-7533; a floored class gets log(1e-300)
+// itself: the reference would add its own node-scaling logs here, which
+// depend on where it scales), l_c = log f + E ln 2; site value
 // m + log sum_c p_c exp(l_c - m) (treesub.c:7364-7393); the fpatt-weighted
 // site sum is compensated (Neumaier) per wave, per block and across blocks.
 __device__ inline double wave_sum(double v)
@@ -327,16 +387,16 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double pil = lane < 61 ? pi[lane] : 0.0;
     double s = 0, comp = 0;
-    for (int q = 0; q < 16; q++) {
-        const int h = blockIdx.x * 64 + w * 16 + q;
+    for (int q = 0; q < kRootPerWave; q++) {
+        const int h = blockIdx.x * (4 * kRootPerWave) + w * kRootPerWave + q;
         if (h >= a.npatt) break;
         double l[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const size_t row = (size_t)a.root_slot[c] * a.npad + h;
-            double v = wave_sum(pil * (double)L[row * 64 + lane]);
-            if (v <= 0) v = 1e-300;
-            l[c] = log(v) + (double)E[row] * kLn2;
+            double v = wave_sum(pil * Store<T>::get(L + row * 64 + lane));
+            // a class that cannot produce the pattern gets f = 1e-300
+            l[c] = v > 0 ? log(v) + (double)E[row] * kLn2 : kLogFloor;
         }
         double m = l[0];
 #pragma unroll
@@ -362,12 +422,30 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
     }
 }
 
-__global__ void k_final(const double *__restrict__ partial, int nblk, double *__restrict__ out)
+// One wave: lane l sums blocks l, l+64, ... (compensated), then a fixed
+// butterfly combines the 64 (sum, compensation) pairs; the order never
+// depends on timing, so the result is bitwise repeatable.
+__global__ __launch_bounds__(64) void k_final(const double *__restrict__ partial, int nblk,
+                                              double *__restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int lane = threadIdx.x;
     double S = 0, Cc = 0;
-    for (int i = 0; i < nblk; i++) { neumaier(S, Cc, partial[2 * i]); Cc += partial[2 * i + 1]; }
-    out[0] = S + Cc;
+    for (int i = lane; i < nblk; i += 64) {
+        neumaier(S, Cc, partial[2 * i]);
+        Cc += partial[2 * i + 1];
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double So = __shfl_xor(S, off, 64), Co = __shfl_xor(Cc, off, 64);
+        // combine in lane-index order so both partners compute the same value
+        double lo = (lane & off) ? So : S, hi = (lane & off) ? S : So;
+        double lc = (lane & off) ? Co : Cc, hc = (lane & off) ? Cc : Co;
+        double s2 = lo, c2 = lc;
+        neumaier(s2, c2, hi);
+        S = s2;
+        Cc = c2 + hc;
+    }
+    if (lane == 0) out[0] = S + Cc;
 }
 
 // ------------------------------------------------------------------ probe
@@ -391,27 +469,31 @@ __global__ void k_mfma_probe(const double *A, const double *B, double *D, const 
 
 // ------------------------------------------------------------------ launchers
 
-template <typename T>
-void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, T *PT,
-                 const unsigned char *mask_amb, int namb, T *AT)
+void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
+                 const unsigned char *mask_amb, int namb, double *AT)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_pmat<T>, dim3(njobs), dim3(256), 0, s, uvr, jobs, PT);
+    hipLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), 0, s, uvr, jobs, PT);
     if (namb > 0)
-        hipLaunchKernelGGL(k_amb<T>, dim3(njobs), dim3(256), 0, s, jobs, (const T *)PT, mask_amb,
+        hipLaunchKernelGGL(k_amb, dim3(njobs), dim3(256), 0, s, jobs, (const double *)PT, mask_amb,
                            namb, AT);
 }
 
 template <typename T>
-void launch_prune(hipStream_t s, int kernel, const PruneJob *jobs, int njobs, int npad,
-                  const unsigned char *z, const T *PT, const T *AT, int namb, T *L, int *E)
+void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
+                  const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
+                  int *E)
 {
     if (njobs <= 0) return;
-    dim3 grid(npad / kTileP, njobs);
-    if (kernel == 0)
-        hipLaunchKernelGGL(k_prune_mfma<T>, grid, dim3(256), 0, s, jobs, npad, z, PT, AT, namb, L, E);
+    if (kernel == 0 && tile == 128)
+        hipLaunchKernelGGL((k_prune_mfma<T, 2>), dim3(npad / 128, njobs), dim3(256), 0, s, jobs,
+                           npad, z, PT, AT, namb, L, E);
+    else if (kernel == 0)
+        hipLaunchKernelGGL((k_prune_mfma<T, 1>), dim3(npad / 64, njobs), dim3(256), 0, s, jobs,
+                           npad, z, PT, AT, namb, L, E);
     else
-        hipLaunchKernelGGL(k_prune_valu<T>, grid, dim3(256), 0, s, jobs, npad, z, PT, AT, namb, L, E);
+        hipLaunchKernelGGL(k_prune_valu<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z,
+                           PT, AT, namb, L, E);
 }
 
 template <typename T>
@@ -429,15 +511,11 @@ void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, A, B, D64, Af, Bf, D32);
 }
 
-template void launch_pmat<double>(hipStream_t, const double *, const PmatJob *, int, double *,
-                                  const unsigned char *, int, double *);
-template void launch_pmat<float>(hipStream_t, const double *, const PmatJob *, int, float *,
-                                 const unsigned char *, int, float *);
-template void launch_prune<double>(hipStream_t, int, const PruneJob *, int, int,
+template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, int, int,
                                    const unsigned char *, const double *, const double *, int,
                                    double *, int *);
-template void launch_prune<float>(hipStream_t, int, const PruneJob *, int, int,
-                                  const unsigned char *, const float *, const float *, int,
+template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, int, int,
+                                  const unsigned char *, const double *, const double *, int,
                                   float *, int *);
 template void launch_root<double>(hipStream_t, const RootArgs &, const double *, const int *,
                                   const double *, const double *, double *, double *, double *,

@@ -37,7 +37,7 @@ class _Problem(C.Structure):
                 ("z", C.POINTER(C.c_ubyte)), ("ncodes", C.c_int),
                 ("code_mask", C.POINTER(C.c_ubyte)), ("fpatt", C.POINTER(C.c_double)),
                 ("pi", C.POINTER(C.c_double)), ("precision", C.c_int), ("kernel", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("tile", C.c_int)]
 
 
 class _Opts(C.Structure):
@@ -94,7 +94,7 @@ def x_from_params(nbranch_t, kappa, p0, p1, w0, w2=None, transformed=True):
 
 class BranchSiteEngine:
     def __init__(self, tree, data, precision="fp64", kernel="mfma", device=0,
-                 fix_omega=False, omega_fix=1.0, p_transformed=True, pi=None):
+                 fix_omega=False, omega_fix=1.0, p_transformed=True, pi=None, tile=0):
         self.tree, self.data = tree, data
         self._keep = []
         ns, nn = tree.ns, tree.nnode
@@ -116,7 +116,7 @@ class BranchSiteEngine:
                         code_mask=mask.ctypes.data_as(C.POINTER(C.c_ubyte)), fpatt=_dptr(fp),
                         pi=_dptr(pv), precision=BSA_FP32 if precision == "fp32" else BSA_FP64,
                         kernel=BSA_KERNEL_VALU if kernel == "valu" else BSA_KERNEL_MFMA,
-                        device=device)
+                        device=device, tile=tile)
         h = C.c_void_p()
         _check(lib().bsa_create(C.byref(prob), C.byref(h)))
         self._h = h

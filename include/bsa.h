@@ -37,8 +37,9 @@ extern "C" {
 #define BSA_ENUMERIC      4   /* eigen solver failed / non-finite value  */
 
 /* precision of the conditional likelihoods */
-#define BSA_FP64          0   /* plain FP64 partials                       */
-#define BSA_FP32          1   /* FP32 partials, power-of-two site scaling  */
+#define BSA_FP64          0   /* FP64 partials                             */
+#define BSA_FP32          1   /* FP32 partials (FP64 arithmetic, stochastic
+                                 rounding on store), power-of-two scaling  */
 
 /* pruning kernel variant */
 #define BSA_KERNEL_MFMA   0   /* 16x16x4 matrix-core contraction (default) */
@@ -70,6 +71,8 @@ typedef struct {
     int precision;            /* BSA_FP64 or BSA_FP32                       */
     int kernel;               /* BSA_KERNEL_MFMA or BSA_KERNEL_VALU         */
     int device;               /* HIP device ordinal                         */
+    int tile;                 /* site patterns per pruning block: 64, 128,
+                                 or 0 for the default                       */
 } bsa_problem;
 
 /* Parameter vector x in codeml's optimisation variables (codeml.c:1980-1981,

@@ -45,7 +45,8 @@ def _problem(tree, data, **over):
               z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
               code_mask=m.ctypes.data_as(C.POINTER(C.c_ubyte)),
               fpatt=fp.ctypes.data_as(C.POINTER(C.c_double)),
-              pi=pi.ctypes.data_as(C.POINTER(C.c_double)), precision=0, kernel=0, device=0)
+              pi=pi.ctypes.data_as(C.POINTER(C.c_double)), precision=0, kernel=0, device=0,
+              tile=0)
     for k, v in over.items():
         kw[k] = ia(v) if isinstance(v, np.ndarray) else v
     return engine._Problem(**kw), keep
@@ -67,6 +68,7 @@ def small():
     (dict(precision=7), "precision"),
     (dict(kernel=5), "kernel"),
     (dict(ncodes=300), "ncodes"),
+    (dict(tile=96), "tile"),
 ])
 def test_create_rejects_bad_problems(small, change, msg):
     tree, data = small

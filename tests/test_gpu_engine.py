@@ -27,21 +27,23 @@ def x_fixed(tree, fx):
 
 
 def test_mfma_lane_layout():
-    """One 16x16x4 MFMA in FP64 and FP32: lane l, element i holds
-    D[4*(l//16)+i][l%16] for A[l%16][l//16], B[l//16][l%16] operands."""
+    """One 16x16x4 MFMA in FP64 and FP32 with A[l%16][l//16], B[l//16][l%16]
+    operands: element i of lane l holds D[l//16 + 4i][l%16] in FP64 and
+    D[4(l//16) + i][l%16] in FP32, the layouts the pruning kernel assumes."""
     say("case: mfma probe")
     rng = np.random.default_rng(0)
     A = rng.integers(-4, 5, size=(16, 4)).astype(np.float64)
     B = rng.integers(-4, 5, size=(4, 16)).astype(np.float64)
     D = A @ B
     d64, d32 = engine.mfma_probe(A, B)
-    exp = np.array([[D[4 * (l // 16) + i, l % 16] for i in range(4)] for l in range(64)])
-    if not np.array_equal(d64, exp):
+    e64 = np.array([[D[l // 16 + 4 * i, l % 16] for i in range(4)] for l in range(64)])
+    e32 = np.array([[D[4 * (l // 16) + i, l % 16] for i in range(4)] for l in range(64)])
+    if not np.array_equal(d64, e64):
         say("raw f64 lanes:", d64.tolist())
-    if not np.array_equal(d32, exp):
+    if not np.array_equal(d32, e32):
         say("raw f32 lanes:", d32.tolist())
-    assert np.array_equal(d64, exp)
-    assert np.array_equal(d32, exp)
+    assert np.array_equal(d64, e64)
+    assert np.array_equal(d32, e32)
 
 
 @pytest.mark.parametrize("kernel", ["mfma", "valu"])
@@ -60,24 +62,32 @@ def test_fp64_matches_reference_and_oracle(name, kernel):
     Lo, fho, siteo = oracle.lnl(tree, data, *fixtures.params(fx))
     assert abs(L - Lo) / abs(Lo) < 1e-11
     assert np.all(np.abs(site - siteo) <= 1e-10 * np.abs(siteo) + 1e-12)
-    assert np.all(np.abs(fh - fho) <= 1e-10 * np.abs(fho) + 1e-12)
+    # per-class values of improbable classes rest on tiny P(t) entries whose
+    # eigen-route rounding differs between the two eigen solvers: 1e-8 relative
+    assert np.all(np.abs(fh - fho) <= 1e-8 * np.abs(fho) + 1e-12)
     assert abs(np.dot(site, data.fpatt) - L) < 1e-12 * abs(L)
     pr = eng.last_params()
     fk, qf, _, _, _ = oracle.model(*fixtures.params(fx), data.pi)
     assert np.allclose(pr[5:7], qf, rtol=1e-13) and np.allclose(pr[7:11], fk, rtol=1e-14)
 
 
+@pytest.mark.parametrize("kernel", ["mfma", "valu"])
 @pytest.mark.parametrize("name", fixtures.NAMES)
-def test_fp32_scaled_error(name):
-    say(f"case: fp32 {name}")
+def test_fp32_scaled_error(name, kernel):
+    say(f"case: fp32 {kernel} {name}")
     fx, tree, data = fixtures.load(name)
-    e32 = make_engine(tree, data, precision="fp32")
+    e32 = make_engine(tree, data, precision="fp32", kernel=kernel)
     L32 = -e32.lnl(x_fixed(tree, fx))
     Lo, _, siteo = oracle.lnl(tree, data, *fixtures.params(fx))
     _, site32 = e32.fhK()
-    say(f"  |dlnL| fp32 vs oracle = {abs(L32 - Lo):.3e} (npatt {data.npatt})")
-    assert abs(L32 - Lo) < 1e-4
-    assert np.max(np.abs(site32 - siteo)) < 1e-4
+    d = site32 - siteo
+    say(f"  fp32 {kernel}: dlnL = {L32 - Lo:.3e} (npatt {data.npatt}); per-site mean "
+        f"{np.mean(d):.3e} rms {np.sqrt(np.mean(d * d)):.3e} max {np.max(np.abs(d)):.3e}")
+    # FP32 storage, FP64 arithmetic, stochastic rounding: unbiased per-site
+    # errors of ~1e-7 that average out over patterns
+    assert abs(L32 - Lo) < 2e-5
+    assert abs(np.mean(d)) < 3e-8
+    assert np.max(np.abs(d)) < 5e-6
 
 
 def test_repeatable_and_cache_consistent():
@@ -186,7 +196,8 @@ def test_edge_cases_against_oracle():
 
 def test_full_size_properties():
     """128 taxa x 10,000 codons: the oracle is too slow here, so check
-    MFMA = VALU (FP64), sum of sites = lnL, FP32 within 1e-4... of FP64."""
+    MFMA = VALU (FP64), sum of sites = lnL, and the FP32-storage path within
+    1e-4 of FP64 in lnL."""
     say("case: full size 128 x 10000")
     tree, data, p = _random_case(2024, 128, 10000, mean_bl=0.05)
     x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"], p["w2"])
@@ -195,9 +206,11 @@ def test_full_size_properties():
     a, b = em.lnl(x), ev.lnl(x)
     say(f"  npatt {data.npatt} lnL {-a:.6f} mfma-valu {a - b:.3e} timing {em.timing()}")
     assert abs(a - b) < 1e-11 * abs(a)
+    e128 = make_engine(tree, data, tile=128)
+    assert abs(e128.lnl(x) - a) < 1e-11 * abs(a)
     _, site = em.fhK()
     assert abs(np.dot(site, data.fpatt) + a) < 1e-11 * abs(a)
     e32 = make_engine(tree, data, precision="fp32")
     c = e32.lnl(x)
     say(f"  fp32 - fp64 = {c - a:.3e}")
-    assert abs(c - a) < 1e-2
+    assert abs(c - a) < 1e-4
