@@ -3,15 +3,18 @@
 
 Metric: lnL+gradient evaluations per second at 128 taxa x 10,000 codons.
 One step = one evaluation of -lnL and its gradient in codeml's parameter
-vector (np = 2*128-3 + 5 = 258 parameters).  The path timed here is the plain
-FP64 one with ming2's forward-difference gradient (gradientB: np+1 = 259
-likelihood evaluations per step), every likelihood evaluation running the
-full HIP pipeline (P(t) batch, level-batched MFMA pruning, root reduction).
+vector (np = 2*128-3 + 5 = 258 parameters).  The timed path is the one the
+engine ships for optimisation: FP32-stored conditional likelihoods (FP64
+arithmetic, stochastic rounding) with the adjoint gradient
+(bsa_lnl_grad: forward pruning + one preorder adjoint sweep).  Beside it, in
+the same run, the plain FP64 path with ming2's forward-difference gradient
+(gradientB: np+1 = 259 likelihood evaluations per step) is timed on the same
+GPU, and the reference's own CPU codeml (built from its sources into
+oracle/_ref/codeml) on a bounded sample of the same workload.
 
-Inputs are synthetic (seeded simulation under model A, config-2 parameter
-draws) and resident in HBM before timing starts.  The CPU baseline is the
-reference's own CPU codeml (built from its sources into oracle/_ref/codeml),
-timed on this host on a bounded sample of the same workload.
+Inputs are synthetic (seeded simulation under model A, parameter draws of
+the 128-taxon gradient-check configuration) and resident in HBM before
+timing starts.
 """
 import argparse
 import json
@@ -108,17 +111,31 @@ def cpu_baseline(tree, seqs, p, ncodon_full, npatt_full, np_full, sample_codons)
                        f"evaluations per forward-difference gradient")}
 
 
+def roofline(work_bytes, work_flops, ms_sum, nev, launches, kernel, hbm=True):
+    sec = ms_sum / 1e3 / nev
+    gbs = work_bytes / sec / 1e9
+    tf = work_flops / sec / 1e12
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+            "bytes_per_eval": work_bytes, "flops_per_eval": work_flops,
+            "avg_launch_us": ms_sum * 1e3 / max(launches, 1), "launches_per_eval": launches / nev,
+            "fp64_tflops": tf, "fp64_frac": tf / FP64_PEAK_TFLOPS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--codons", type=int, default=10000)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
+    ap.add_argument("--gradient", choices=["adjoint", "fd"], default="adjoint")
     ap.add_argument("--kernel", choices=["mfma", "valu"], default="mfma")
     ap.add_argument("--tile", type=int, default=0, help="patterns per pruning block (64/128)")
+    ap.add_argument("--fd-steps", type=int, default=1,
+                    help="steps of the FP64 forward-difference path timed beside (0: skip)")
     ap.add_argument("--cpu-sample-codons", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -128,32 +145,36 @@ def main():
         sys.exit(2)
 
     tree, data, p, seqs = make_workload(args.taxa, args.codons, args.seed)
-    eng = engine.BranchSiteEngine(tree, data, precision=args.precision, kernel=args.kernel,
-                                  tile=args.tile)
     x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"],
                              p["w2"])
     npar = len(x)
+    eng = engine.BranchSiteEngine(tree, data, precision=args.precision, kernel=args.kernel,
+                                  tile=args.tile)
+    step = eng.lnl_grad if args.gradient == "adjoint" else eng.gradientB
     for _ in range(args.warmup):
-        eng.gradientB(x)
+        step(x)
     eng.timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        f, g = eng.gradientB(x)
+        f, g = step(x)
     elapsed = time.perf_counter() - t0
     tm = eng.timing(reset=True)
     work = eng.work()
     nev = tm["n_eval"]
-    prune_ms = tm["sum_prune"]
-    prune_s_per_eval = prune_ms / 1e3 / nev
-    bytes_per_eval = work[1]
-    achieved = bytes_per_eval / prune_s_per_eval / 1e9
-    flops_ach = work[0] / prune_s_per_eval / 1e12
-    launches = tm["n_launch"]
-
-    w = 8 if args.precision == "fp64" else 4
+    sto = "double" if args.precision == "fp64" else "float"
+    prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
+                        f"k_prune_{args.kernel}<{sto}> (FP64 arithmetic, {sto} storage)")
+    rl = prune_rl
+    adj_rl = None
+    if args.gradient == "adjoint":
+        adj_rl = roofline(work[7], work[6], tm["sum_adjoint"], nev, tm["n_adj_launch"],
+                          f"k_adjoint<{sto}> (FP64 arithmetic, {sto} storage)")
+        if tm["sum_adjoint"] > tm["sum_prune"]:
+            rl = adj_rl
+    value = args.steps / elapsed
     out = {
         "metric": "lnL+gradient evals/sec at 128 taxa x 10k codons (branch-site model A)",
-        "value": args.steps / elapsed,
+        "value": value,
         "unit": "lnL+gradient evals/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -162,26 +183,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "dtype": "f64" if args.precision == "fp64" else "f32-storage/f64-arith",
         "data": "synthetic: seeded model-A simulation (Yule tree, F3x4 Dirichlet(1))",
-        "config": {"workload": f"model A, {args.taxa} taxa x {args.codons} codons "
+        "config": {"workload": f"branch-site model A, {args.taxa} taxa x {args.codons} codons "
                                f"({data.npatt} patterns), {args.precision} partials, "
-                               f"forward-difference gradient ({npar + 1} lnL per step)",
+                               f"{args.gradient} gradient of {npar} parameters",
                    "taxa": args.taxa, "codons": args.codons, "patterns": data.npatt,
-                   "np": npar, "gradient": "fd-forward", "kernel": args.kernel,
+                   "np": npar, "gradient": args.gradient, "kernel": args.kernel,
                    "tile": args.tile or 64},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_prune_{args.kernel}<{'double' if w == 8 else 'float'}>"
-                               " (FP64 arithmetic, storage as named)",
-                     "bytes_per_lnl": bytes_per_eval, "flops_per_lnl": work[0],
-                     "launches_per_lnl": work[4],
-                     "avg_launch_us": prune_ms * 1e3 / max(launches, 1),
-                     "fp64_tflops": flops_ach, "fp64_frac": flops_ach / FP64_PEAK_TFLOPS},
-        "lnl_ms": {"wall": tm["sum_wall"] / nev, "pmat": tm["sum_pmat"] / nev,
-                   "prune": prune_ms / nev, "root": tm["sum_root"] / nev},
+        "roofline": rl,
+        "roofline_pruning": prune_rl,
+        "roofline_adjoint": adj_rl,
+        "eval_ms": {k: tm["sum_" + k] / nev for k in
+                    ("pmat", "prune", "root", "adjoint", "finish", "wall")},
         "lnL": -f,
     }
+    if args.fd_steps > 0:
+        e64 = engine.BranchSiteEngine(tree, data, precision="fp64", kernel=args.kernel)
+        e64.gradientB(x)
+        t1 = time.perf_counter()
+        for _ in range(args.fd_steps):
+            f64, _ = e64.gradientB(x)
+        fd_el = (time.perf_counter() - t1) / args.fd_steps
+        out["fd64_gpu"] = {"value": 1.0 / fd_el, "unit": "lnL+gradient evals/s",
+                           "ms_per_step": fd_el * 1e3,
+                           "path": f"FP64 partials, forward differences ({npar + 1} lnL/step)"}
+        out["speedup_vs_fd64_gpu"] = value * fd_el
+        out["lnL_error_vs_fp64"] = abs(f - f64)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tree, seqs, p, args.codons, data.npatt, npar,
                                            args.cpu_sample_codons)

@@ -63,7 +63,7 @@ struct bsa_engine {
 
     // device
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *d_uvr = nullptr;
     PmatJob *d_pjobs = nullptr;
     PruneJob *d_prune = nullptr;
@@ -84,11 +84,32 @@ struct bsa_engine {
     bool have_eigen = false;
     double cache_kappa = NAN, cache_w[3] = {NAN, NAN, NAN};
     double last_lnl = NAN;
-    double timing[5] = {0, 0, 0, 0, 0};
-    double timing_sum[5] = {0, 0, 0, 0, 0};
-    double n_eval = 0, n_launch = 0;
+    double timing[6] = {0, 0, 0, 0, 0, 0};      // pmat, prune, root, adjoint, finish, wall
+    double timing_sum[6] = {0, 0, 0, 0, 0, 0};
+    double n_eval = 0, n_launch = 0, n_adj_launch = 0;
     bool evaluated = false;
-    double work[6] = {0, 0, 0, 0, 0, 0};
+    double work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // adjoint (allocated on the first gradient call)
+    bool adj_ready = false;
+    std::vector<AdjJob> adj;              // grouped by depth of the child node
+    std::vector<int> adj_off;             // depth d: adj[adj_off[d-1] .. adj_off[d])
+    std::vector<EigJob> ejobs;
+    std::vector<int> ejob_node;           // node of each EigJob
+    std::vector<int> set_off, set_list;
+    int nchunk = 0, nmrow = 0;
+    int root_aslot[4];
+    AdjJob *d_adj = nullptr;
+    EigJob *d_ejobs = nullptr;
+    EigJob *h_ejobs = nullptr;
+    int *d_set_off = nullptr, *d_set_list = nullptr;
+    void *d_A = nullptr;
+    int *d_EA = nullptr;
+    double *d_Mpart = nullptr, *d_M = nullptr, *d_G = nullptr, *d_dtau = nullptr, *d_Z = nullptr;
+    double *d_mask64 = nullptr;
+    double *h_adjout = nullptr;           // dtau[nej] then Z[3*4096]
+    std::vector<unsigned char> code_mask; // ncodes x 61
+    std::vector<std::vector<int>> slot;   // inside slot per node and class
 };
 
 static size_t tsize(const bsa_engine *e) { return e->precision == BSA_FP32 ? 4 : 8; }
@@ -98,12 +119,16 @@ static void free_all(bsa_engine *e)
     if (e->stream) hipSetDevice(e->dev);
     void *dev_ptrs[] = {e->d_uvr, e->d_pjobs, e->d_prune, e->d_PT, e->d_AT, e->d_L, e->d_E,
                         e->d_z, e->d_mask_amb, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
-                        e->d_partial, e->d_out};
+                        e->d_partial, e->d_out, e->d_adj, e->d_ejobs, e->d_set_off,
+                        e->d_set_list, e->d_A, e->d_EA, e->d_Mpart, e->d_M, e->d_G, e->d_dtau,
+                        e->d_Z, e->d_mask64};
     for (void *p : dev_ptrs)
         if (p) hipFree(p);
     if (e->h_uvr) hipHostFree(e->h_uvr);
     if (e->h_out) hipHostFree(e->h_out);
     if (e->h_pjobs) hipHostFree(e->h_pjobs);
+    if (e->h_ejobs) hipHostFree(e->h_ejobs);
+    if (e->h_adjout) hipHostFree(e->h_adjout);
     for (auto &v : e->ev)
         if (v) hipEventDestroy(v);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -162,6 +187,9 @@ static int build_schedule(bsa_engine *e)
     }
     e->nslot = ns_;
     for (int c = 0; c < 4; c++) e->root_slot[c] = slot[e->root][c];
+    e->slot.assign(nn, std::vector<int>(4, -1));
+    for (int v = 0; v < nn; v++)
+        for (int c = 0; c < 4; c++) e->slot[v][c] = slot[v][c];
     // pruning jobs grouped by height
     int hmax = e->height[e->root];
     e->prune.clear();
@@ -246,8 +274,8 @@ static int upload_static(bsa_engine *e, const bsa_problem *p)
     HIPCHK(hipMalloc(&e->d_fpatt, sizeof(double) * np));
     HIPCHK(hipMalloc(&e->d_fh, sizeof(double) * 4 * (size_t)np));
     HIPCHK(hipMalloc(&e->d_site, sizeof(double) * np));
-    e->nroot_blk = (e->npatt + 15) / 16;   // k_root: 16 patterns per block
-    HIPCHK(hipMalloc(&e->d_partial, sizeof(double) * 2 * e->nroot_blk));
+    e->nroot_blk = e->npad / 16;   // k_root: 16 patterns per block, padding included
+    HIPCHK(hipMalloc(&e->d_partial, sizeof(double) * 2 * kRootSums * e->nroot_blk));
     HIPCHK(hipMalloc(&e->d_out, sizeof(double) * 8));
     HIPCHK(hipHostMalloc((void **)&e->h_uvr, sizeof(double) * 3 * kUVRSet, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&e->h_out, sizeof(double) * 8, hipHostMallocDefault));
@@ -351,6 +379,10 @@ extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
         if (p->z[i] >= p->ncodes) { delete e; return fail(BSA_EINVAL, "code %d >= ncodes", p->z[i]); }
     e->pi.assign(p->pi, p->pi + kS);
     e->fpatt.assign(p->fpatt, p->fpatt + p->npatt);
+    e->code_mask.assign((size_t)p->ncodes * kS, 0);
+    for (int c = 0; c < kS; c++) e->code_mask[(size_t)c * kS + c] = 1;
+    for (size_t i = (size_t)kS * kS; i < (size_t)p->ncodes * kS; i++)
+        e->code_mask[i] = p->code_mask[i] ? 1 : 0;
     int rc = build_schedule(e);
     if (rc) { delete e; return rc; }
     rc = upload_static(e, p);
@@ -402,12 +434,15 @@ static void natural_params(const bsa_engine *e, const double *x, double *kappa, 
 }
 
 template <typename T>
-static int run_device(bsa_engine *e)
+static int run_device(bsa_engine *e, bool adjoint)
 {
     hipStream_t s = e->stream;
     HIPCHK(hipMemcpyAsync(e->d_uvr, e->h_uvr, sizeof(double) * 3 * kUVRSet, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(e->d_pjobs, e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
                           hipMemcpyHostToDevice, s));
+    if (adjoint)
+        HIPCHK(hipMemcpyAsync(e->d_ejobs, e->h_ejobs, sizeof(EigJob) * e->ejobs.size(),
+                              hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(e->ev[0], s));
     launch_pmat(s, e->d_uvr, e->d_pjobs, (int)e->pjobs.size(), (double *)e->d_PT, e->d_mask_amb,
                 e->namb, (double *)e->d_AT);
@@ -427,12 +462,32 @@ static int run_device(bsa_engine *e)
     for (int c = 0; c < 4; c++) {
         ra.root_slot[c] = e->root_slot[c];
         ra.freqK[c] = e->model.freqK[c];
+        ra.seed_aslot[c] = adjoint ? e->root_aslot[c] : -1;
     }
     launch_root<T>(s, ra, (const T *)e->d_L, e->d_E, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
-                   e->d_partial, e->nroot_blk, e->d_out);
+                   e->d_partial, e->nroot_blk, e->d_out, (T *)e->d_A, e->d_EA);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev[3], s));
-    HIPCHK(hipMemcpyAsync(e->h_out, e->d_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (adjoint) {
+        for (size_t d = 0; d + 1 < e->adj_off.size(); d++) {
+            int a = e->adj_off[d], b = e->adj_off[d + 1];
+            launch_adjoint_level<T>(s, e->d_adj + a, b - a, e->npad, e->nchunk, e->d_z,
+                                    (const double *)e->d_PT, (const double *)e->d_AT, e->namb,
+                                    e->d_mask64, (const T *)e->d_L, e->d_E, (T *)e->d_A, e->d_EA,
+                                    e->d_Mpart);
+        }
+        HIPCHK(hipEventRecord(e->ev[4], s));
+        launch_adjoint_finish(s, e->d_Mpart, e->nmrow, e->nchunk, e->d_M, e->d_ejobs,
+                              (int)e->ejobs.size(), e->d_set_off, e->d_set_list, e->d_uvr, e->d_G,
+                              e->d_dtau, e->d_Z);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e->ev[5], s));
+        HIPCHK(hipMemcpyAsync(e->h_adjout, e->d_dtau, sizeof(double) * e->ejobs.size(),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(e->h_adjout + e->ejobs.size(), e->d_Z, sizeof(double) * 3 * 4096,
+                              hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * kRootSums, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     float ms;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
@@ -441,28 +496,35 @@ static int run_device(bsa_engine *e)
     e->timing[1] = ms;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[2], e->ev[3]));
     e->timing[2] = ms;
-    e->timing[3] = 0;
-    for (int i = 0; i < 3; i++) e->timing_sum[i] += e->timing[i];
+    e->timing[3] = e->timing[4] = 0;
+    if (adjoint) {
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[3], e->ev[4]));
+        e->timing[3] = ms;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[4], e->ev[5]));
+        e->timing[4] = ms;
+        e->n_adj_launch += (double)(e->adj_off.size() - 1);
+    }
+    for (int i = 0; i < 5; i++) e->timing_sum[i] += e->timing[i];
     e->n_launch += (double)(e->level_off.size() - 1);
     return BSA_OK;
 }
 
-static int evaluate_inner(bsa_engine *e, const double *x, double *lnl);
+static int evaluate_inner(bsa_engine *e, const double *x, double *lnl, bool adjoint);
 
-static int evaluate(bsa_engine *e, const double *x, double *lnl)
+static int evaluate(bsa_engine *e, const double *x, double *lnl, bool adjoint = false)
 {
     auto t0 = std::chrono::steady_clock::now();
-    int rc = evaluate_inner(e, x, lnl);
+    int rc = evaluate_inner(e, x, lnl, adjoint);
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc == BSA_OK) {
-        e->timing[4] = ms;
-        e->timing_sum[4] += ms;
+        e->timing[5] = ms;
+        e->timing_sum[5] += ms;
         e->n_eval += 1;
     }
     return rc;
 }
 
-static int evaluate_inner(bsa_engine *e, const double *x, double *lnl)
+static int evaluate_inner(bsa_engine *e, const double *x, double *lnl, bool adjoint)
 {
     double kappa, p0, p1, w0, w2;
     natural_params(e, x, &kappa, &p0, &p1, &w0, &w2);
@@ -505,7 +567,14 @@ static int evaluate_inner(bsa_engine *e, const double *x, double *lnl)
         p.t = x[e->ibranch[p.node]] * e->model.Qfactor[e->label[p.node]];
         e->h_pjobs[i] = p;
     }
-    int rc = e->precision == BSA_FP32 ? run_device<float>(e) : run_device<double>(e);
+    if (adjoint)
+        for (size_t i = 0; i < e->ejobs.size(); i++) {
+            EigJob j = e->ejobs[i];
+            const int v = e->ejob_node[i];
+            j.tau = x[e->ibranch[v]] * e->model.Qfactor[e->label[v]];
+            e->h_ejobs[i] = j;
+        }
+    int rc = e->precision == BSA_FP32 ? run_device<float>(e, adjoint) : run_device<double>(e, adjoint);
     if (rc) return rc;
     *lnl = e->h_out[0];
     e->last_lnl = *lnl;
@@ -521,6 +590,162 @@ extern "C" int bsa_lnl(bsa_engine *e, const double *x, int np, double *neg_lnl)
     double lnl;
     int rc = evaluate(e, x, &lnl);
     if (rc) return rc;
+    *neg_lnl = -lnl;
+    return BSA_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+static int setup_adjoint(bsa_engine *e)
+{
+    if (e->adj_ready) return BSA_OK;
+    const int ns = e->ns, nn = e->nnode;
+    std::vector<int> depth(nn, 0), order{e->root};
+    for (size_t i = 0; i < order.size(); i++)
+        for (int c : e->sons[order[i]]) {
+            depth[c] = depth[order[i]] + 1;
+            order.push_back(c);
+        }
+    int maxd = 0;
+    for (int v = 0; v < nn; v++) maxd = std::max(maxd, depth[v]);
+    auto aslot = [&](int v, int c) { return (v - ns) * 4 + c; };
+    for (int c = 0; c < 4; c++) e->root_aslot[c] = aslot(e->root, c);
+    e->adj.clear();
+    e->adj_off.assign(1, 0);
+    for (int d = 1; d <= maxd; d++) {
+        for (int v = 0; v < nn; v++) {
+            if (depth[v] != d) continue;
+            const int a = e->father[v];
+            for (int c = 0; c < 4; c++) {
+                AdjJob j{};
+                j.v_tip = v < ns ? v : -1;
+                j.v_slot = v < ns ? -1 : e->slot[v][c];
+                j.v_pmat = v * 3 + uvr_set(c, e->label[v]);
+                j.v_aslot = v < ns ? -1 : aslot(v, c);
+                j.a_aslot = aslot(a, c);
+                j.m_index = v * 4 + c;
+                int n = 0;
+                for (int s2 : e->sons[a]) {
+                    if (s2 == v) continue;
+                    j.sib_tip[n] = s2 < ns ? s2 : -1;
+                    j.sib_slot[n] = s2 < ns ? -1 : e->slot[s2][c];
+                    j.sib_pmat[n] = s2 * 3 + uvr_set(c, e->label[s2]);
+                    n++;
+                }
+                j.nsib = n;
+                e->adj.push_back(j);
+            }
+        }
+        e->adj_off.push_back((int)e->adj.size());
+    }
+    e->ejobs.clear();
+    e->ejob_node.clear();
+    std::vector<int> lists[3];
+    for (int v = 0; v < nn; v++) {
+        if (v == e->root) continue;
+        for (int c = 0; c < 4; c++) {
+            EigJob j{};
+            j.m_index = v * 4 + c;
+            j.set = uvr_set(c, e->label[v]);
+            j.tau = 0;
+            lists[j.set].push_back((int)e->ejobs.size());
+            e->ejobs.push_back(j);
+            e->ejob_node.push_back(v);
+        }
+    }
+    e->set_off = {0};
+    e->set_list.clear();
+    for (int s = 0; s < 3; s++) {
+        e->set_list.insert(e->set_list.end(), lists[s].begin(), lists[s].end());
+        e->set_off.push_back((int)e->set_list.size());
+    }
+    e->nmrow = nn * 4;
+    const int ntile = e->npad / 64;
+    e->nchunk = (ntile + kAdjTilesPerChunk - 1) / kAdjTilesPerChunk;
+    const size_t ts = tsize(e), nej = e->ejobs.size();
+    const size_t nasl = (size_t)(nn - ns) * 4;
+    HIPCHK(hipSetDevice(e->dev));
+    HIPCHK(hipMalloc(&e->d_adj, sizeof(AdjJob) * e->adj.size()));
+    HIPCHK(hipMemcpy(e->d_adj, e->adj.data(), sizeof(AdjJob) * e->adj.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&e->d_ejobs, sizeof(EigJob) * nej));
+    HIPCHK(hipHostMalloc((void **)&e->h_ejobs, sizeof(EigJob) * nej, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&e->d_set_off, sizeof(int) * 4));
+    HIPCHK(hipMemcpy(e->d_set_off, e->set_off.data(), sizeof(int) * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&e->d_set_list, sizeof(int) * std::max<size_t>(1, e->set_list.size())));
+    HIPCHK(hipMemcpy(e->d_set_list, e->set_list.data(), sizeof(int) * e->set_list.size(),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&e->d_A, ts * 64 * (size_t)e->npad * nasl));
+    HIPCHK(hipMalloc(&e->d_EA, sizeof(int) * (size_t)e->npad * nasl));
+    HIPCHK(hipMalloc(&e->d_Mpart, sizeof(double) * 4096 * (size_t)e->nmrow * e->nchunk));
+    HIPCHK(hipMemset(e->d_Mpart, 0, sizeof(double) * 4096 * (size_t)e->nmrow * e->nchunk));
+    HIPCHK(hipMalloc(&e->d_M, sizeof(double) * 4096 * (size_t)e->nmrow));
+    HIPCHK(hipMalloc(&e->d_G, sizeof(double) * 4096 * nej));
+    HIPCHK(hipMalloc(&e->d_dtau, sizeof(double) * nej));
+    HIPCHK(hipMalloc(&e->d_Z, sizeof(double) * 3 * 4096));
+    std::vector<double> m64((size_t)e->ncodes * 64, 0.0);
+    for (int c = 0; c < e->ncodes; c++)
+        for (int k = 0; k < kS; k++) m64[(size_t)c * 64 + k] = e->code_mask[(size_t)c * kS + k];
+    HIPCHK(hipMalloc(&e->d_mask64, sizeof(double) * m64.size()));
+    HIPCHK(hipMemcpy(e->d_mask64, m64.data(), sizeof(double) * m64.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipHostMalloc((void **)&e->h_adjout, sizeof(double) * (nej + 3 * 4096), hipHostMallocDefault));
+    // work of one adjoint sweep on 61 states: per job, one contraction per
+    // interior sibling, one for the child's adjoint (interior child) and the
+    // branch-matrix product; bytes: parent adjoint, interior siblings' and
+    // the child's partials read, the child's adjoint written, exponents
+    {
+        const double w = (double)ts, np_ = e->npatt;
+        double fl = 0, by = 0;
+        for (const AdjJob &j : e->adj) {
+            int nint = 0;
+            for (int q = 0; q < j.nsib; q++) nint += j.sib_tip[q] < 0;
+            const bool vint = j.v_tip < 0;
+            fl += 2.0 * kS * kS * np_ * (nint + (vint ? 2 : 1));
+            by += np_ * (kS * w * (1 + nint + (vint ? 2 : 0)) + 4.0 * (1 + nint + (vint ? 2 : 0)) +
+                         (j.nsib - nint) + (vint ? 0 : 1));
+        }
+        e->work[6] = fl;
+        e->work[7] = by;
+    }
+    e->adj_ready = true;
+    return BSA_OK;
+}
+
+// -lnL and d(-lnL)/dx by one forward pass and one adjoint (preorder) sweep
+extern "C" int bsa_lnl_grad(bsa_engine *e, const double *x, int np, double *neg_lnl, double *grad)
+{
+    if (!e || !x || !neg_lnl || !grad) return fail(BSA_EINVAL, "null argument");
+    if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    HIPCHK(hipSetDevice(e->dev));
+    int rc = setup_adjoint(e);
+    if (rc) return rc;
+    double lnl;
+    if ((rc = evaluate(e, x, &lnl, true))) return rc;
+    const size_t nej = e->ejobs.size();
+    const double *dtau = e->h_adjout, *Z = e->h_adjout + nej;
+    std::vector<double> dt_node(e->nnode, 0.0);
+    for (size_t i = 0; i < nej; i++) dt_node[e->ejob_node[i]] += dtau[i];
+    double dQf[2] = {0, 0};
+    const int nb = e->nbranch;
+    for (int v = 0; v < e->nnode; v++) {
+        if (v == e->root) continue;
+        const int b = e->ibranch[v], l = e->label[v];
+        dQf[l] += x[b] * dt_node[v];
+        grad[b] = -e->model.Qfactor[l] * dt_node[v];
+    }
+    const double Sf[4] = {e->h_out[1], e->h_out[2], e->h_out[3], e->h_out[4]};
+    ModelGrad mg;
+    model_a_param_grad(e->model, e->pi.data(), Z, dQf, Sf, &mg);
+    grad[nb] = -mg.kappa;
+    if (e->opts.p_transformed) {
+        const double p0 = e->model.p0, p1 = e->model.p1;
+        grad[nb + 1] = -(mg.p0 * p0 * (1 - p0) - mg.p1 * p0 * p1);
+        grad[nb + 2] = -(-mg.p0 * p0 * p1 + mg.p1 * p1 * (1 - p1));
+    } else {
+        grad[nb + 1] = -mg.p0;
+        grad[nb + 2] = -mg.p1;
+    }
+    grad[nb + 3] = -mg.w0;
+    if (!e->opts.fix_omega) grad[nb + 4] = -mg.w2;
     *neg_lnl = -lnl;
     return BSA_OK;
 }
@@ -590,13 +815,15 @@ extern "C" int bsa_timing(bsa_engine *e, double *o, int reset)
 {
     if (!e || !o) return fail(BSA_EINVAL, "null argument");
     std::memcpy(o, e->timing, sizeof e->timing);
-    std::memcpy(o + 5, e->timing_sum, sizeof e->timing_sum);
-    o[10] = e->n_eval;
-    o[11] = e->n_launch;
+    std::memcpy(o + 6, e->timing_sum, sizeof e->timing_sum);
+    o[12] = e->n_eval;
+    o[13] = e->n_launch;
+    o[14] = e->n_adj_launch;
     if (reset) {
         for (double &v : e->timing_sum) v = 0;
         e->n_eval = 0;
         e->n_launch = 0;
+        e->n_adj_launch = 0;
     }
     return BSA_OK;
 }
@@ -620,9 +847,11 @@ extern "C" int bsa_model_a_setup(double kappa, double p0, double p1, double w0, 
     return BSA_OK;
 }
 
-extern "C" int bsa_work(const bsa_engine *e, double *o)
+extern "C" int bsa_work(bsa_engine *e, double *o)
 {
     if (!e || !o) return fail(BSA_EINVAL, "null argument");
+    int rc = setup_adjoint(e);
+    if (rc) return rc;
     std::memcpy(o, e->work, sizeof e->work);
     return BSA_OK;
 }

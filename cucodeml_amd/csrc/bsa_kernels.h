@@ -28,7 +28,54 @@ struct RootArgs {
     int npatt, npad;
     int root_slot[4];           // root partial slot per class
     double freqK[4];
+    int seed_aslot[4];          // adjoint slot of the root per class, or -1:
+                                // no adjoint seeds are written
 };
+
+// quantities reduced over patterns by the root pass: lnL, then per class
+// Sf_c = sum_h fpatt_h f_hc / f_h (= d lnL / d p_c)
+constexpr int kRootSums = 5;
+
+// One job of the adjoint (preorder) sweep: branch above node v, site class c.
+// X = (adjoint of the parent) x (contributions of v's siblings) is the
+// derivative of lnL with respect to v's contribution to its parent; from it
+// the job writes v's adjoint (if v is interior) and the branch matrix
+// M = sum_h X_h^T I_h over its chunk of patterns.
+struct AdjJob {
+    int v_tip;                  // tip index of v, or -1
+    int v_slot;                 // inside partial slot of v (interior)
+    int v_pmat;                 // P(t) matrix index of v's branch for class c
+    int v_aslot;                // adjoint slot written for v (interior) or -1
+    int a_aslot;                // adjoint slot of the parent
+    int m_index;                // row of the branch-matrix buffer
+    int nsib;
+    int sib_tip[kMaxCh];
+    int sib_slot[kMaxCh];
+    int sib_pmat[kMaxCh];
+};
+
+// per (branch, class) eigen-basis job: M -> (d lnL / d tau, A o X)
+struct EigJob {
+    int m_index;                // branch-matrix row
+    int set;                    // eigen set (0: w0, 1: omega = 1, 2: w2)
+    double tau;                 // t * Qfactor
+};
+
+constexpr int kAdjTilesPerChunk = 16;   // 64-pattern tiles per adjoint block
+
+template <typename T>
+void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+                          const unsigned char *z, const double *PT, const double *AT, int namb,
+                          const double *mask64, const T *L, const int *E, T *A, int *EA,
+                          double *Mpart);
+
+// sums the chunk partials of nm branch matrices, then for every EigJob
+// forms A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set
+// W = sum G and Z = V^T W U^T.  Outputs dtau[nm] and Z[3][64*64].
+void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchunk, double *M,
+                           const EigJob *ejobs, int nej, const int *set_off,
+                           const int *set_list, const double *uvr, double *G, double *dtau,
+                           double *Z);
 
 // launchers; T = storage type of the conditional likelihoods (double or float)
 void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
@@ -40,10 +87,12 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int
                   const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
                   int *E);
 
+// per-pattern class values, mixture, reduced sums (out[kRootSums]); with
+// a.seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA)
 template <typename T>
 void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,
                  const double *fpatt, double *fh, double *site, double *partial, int nblk,
-                 double *out);
+                 double *out, T *A, int *EA);
 
 // raw MFMA lane-layout probe: one wave, writes each lane's accumulator
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,

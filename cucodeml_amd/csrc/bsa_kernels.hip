@@ -381,22 +381,38 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
                                               const double *__restrict__ pi,
                                               const double *__restrict__ fpatt,
                                               double *__restrict__ fh, double *__restrict__ site,
-                                              double *__restrict__ partial)
+                                              double *__restrict__ partial, T *__restrict__ A,
+                                              int *__restrict__ EA)
 {
-    __shared__ double ss[4], sc[4];
+    __shared__ double ss[4][kRootSums], sc[4][kRootSums];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double pil = lane < 61 ? pi[lane] : 0.0;
-    double s = 0, comp = 0;
+    const bool seed = a.seed_aslot[0] >= 0;
+    double s[kRootSums], comp[kRootSums];
+#pragma unroll
+    for (int q = 0; q < kRootSums; q++) s[q] = comp[q] = 0;
     for (int q = 0; q < kRootPerWave; q++) {
         const int h = blockIdx.x * (4 * kRootPerWave) + w * kRootPerWave + q;
-        if (h >= a.npatt) break;
-        double l[4];
+        if (h >= a.npad) break;
+        if (h >= a.npatt) {   // padding: zero adjoint seeds
+            if (seed)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const size_t row = (size_t)a.seed_aslot[c] * a.npad + h;
+                    Store<T>::put(A + row * 64 + lane, 0.0, 0);
+                    if (lane == 0) EA[row] = 0;
+                }
+            continue;
+        }
+        double l[4], v[4];
+        int e[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const size_t row = (size_t)a.root_slot[c] * a.npad + h;
-            double v = wave_sum(pil * Store<T>::get(L + row * 64 + lane));
+            v[c] = wave_sum(pil * Store<T>::get(L + row * 64 + lane));
+            e[c] = E[row];
             // a class that cannot produce the pattern gets f = 1e-300
-            l[c] = v > 0 ? log(v) + (double)E[row] * kLn2 : kLogFloor;
+            l[c] = v[c] > 0 ? log(v[c]) + (double)e[c] * kLn2 : kLogFloor;
         }
         double m = l[0];
 #pragma unroll
@@ -405,34 +421,54 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
 #pragma unroll
         for (int c = 0; c < 4; c++) f += a.freqK[c] * exp(l[c] - m);
         const double st = m + log(f);
+        const double wgt = fpatt[h] > 0 ? fpatt[h] : 0.0;
+        if (seed) {
+            // d lnL / d L_root^c(h, j) = w pi_j p_c / f_h; stored scaled by
+            // 2^E_root: w pi_j post_c / ftilde_c, exponent -E_root
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const size_t row = (size_t)a.seed_aslot[c] * a.npad + h;
+                const double post = a.freqK[c] * exp(l[c] - st);
+                const double val = v[c] > 0 ? wgt * pil * post / v[c] : 0.0;
+                Store<T>::put(A + row * 64 + lane, val, (uint32_t)(h * 64 + lane) ^ 0x5bd1e995u);
+                if (lane == 0) EA[row] = -e[c];
+            }
+        }
         if (lane == 0) {
 #pragma unroll
             for (int c = 0; c < 4; c++) fh[(size_t)c * a.npad + h] = l[c];
             site[h] = st;
-            if (fpatt[h] > 0) neumaier(s, comp, fpatt[h] * st);
+            if (wgt > 0) {
+                neumaier(s[0], comp[0], wgt * st);
+#pragma unroll
+                for (int c = 0; c < 4; c++) neumaier(s[1 + c], comp[1 + c], wgt * exp(l[c] - st));
+            }
         }
     }
-    if (lane == 0) { ss[w] = s; sc[w] = comp; }
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < kRootSums; q++) { ss[w][q] = s[q]; sc[w][q] = comp[q]; }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kRootSums) {
+        const int q = threadIdx.x;
         double S = 0, Cc = 0;
-        for (int i = 0; i < 4; i++) { neumaier(S, Cc, ss[i]); Cc += sc[i]; }
-        partial[2 * blockIdx.x] = S;
-        partial[2 * blockIdx.x + 1] = Cc;
+        for (int i = 0; i < 4; i++) { neumaier(S, Cc, ss[i][q]); Cc += sc[i][q]; }
+        partial[(size_t)blockIdx.x * 2 * kRootSums + 2 * q] = S;
+        partial[(size_t)blockIdx.x * 2 * kRootSums + 2 * q + 1] = Cc;
     }
 }
 
-// One wave: lane l sums blocks l, l+64, ... (compensated), then a fixed
-// butterfly combines the 64 (sum, compensation) pairs; the order never
-// depends on timing, so the result is bitwise repeatable.
+// One wave per reduced quantity: lane l sums blocks l, l+64, ...
+// (compensated), then a fixed butterfly combines the 64 (sum, compensation)
+// pairs; the order never depends on timing, so results are bitwise repeatable.
 __global__ __launch_bounds__(64) void k_final(const double *__restrict__ partial, int nblk,
                                               double *__restrict__ out)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, q = blockIdx.x;
     double S = 0, Cc = 0;
     for (int i = lane; i < nblk; i += 64) {
-        neumaier(S, Cc, partial[2 * i]);
-        Cc += partial[2 * i + 1];
+        neumaier(S, Cc, partial[(size_t)i * 2 * kRootSums + 2 * q]);
+        Cc += partial[(size_t)i * 2 * kRootSums + 2 * q + 1];
     }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -445,7 +481,337 @@ __global__ __launch_bounds__(64) void k_final(const double *__restrict__ partial
         S = s2;
         Cc = c2 + hc;
     }
-    if (lane == 0) out[0] = S + Cc;
+    if (lane == 0) out[q] = S + Cc;
+}
+
+// ------------------------------------------------------------------ adjoint
+
+constexpr int kLdsStride = 65;   // doubles per LDS row: row and column reads both conflict-free
+
+__device__ inline double pow2i(int d)
+{
+    d = d < -1000 ? -1000 : (d > 1000 ? 1000 : d);
+    return ldexp(1.0, d);
+}
+
+// Stage P - I of matrix pm into LDS with row stride 65 (row k = child state).
+__device__ inline void stage_p(double *sP, const double *P, int tid)
+{
+    for (int q = tid; q < 4096; q += 256) sP[(q >> 6) * kLdsStride + (q & 63)] = P[q];
+}
+
+// Contribution of an interior child s to its parent for 16 patterns of this
+// wave: C(h, j) = sum_k P_{j,k} I_s(h, k) in the accumulator layout
+// (element i of column block b: pattern hw + g + 4i, state 16b + r).
+// sP must hold child s's P - I.
+template <typename T>
+__device__ inline void child_contrib(const double *sP, const T *Ls, int hw, int g, int r, d4 *C)
+{
+    double a[16];
+    Store<T>::load16(Ls + (size_t)(hw + r) * 64 + 16 * g, a);
+#pragma unroll
+    for (int b = 0; b < 4; b++) C[b] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+        const double *prow = sP + (16 * g + kk) * kLdsStride + r;
+#pragma unroll
+        for (int b = 0; b < 4; b++) C[b] = mfma64(a[kk], prow[16 * b], C[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) C[b][i] += Store<T>::get(Ls + (size_t)(hw + g + 4 * i) * 64 + 16 * b + r);
+}
+
+// One (branch v, class c, chunk of 64-pattern tiles) job of the preorder
+// adjoint sweep.  Per tile:
+//   X(h, j) = Abar_parent(h, j) * prod_{siblings s} C_s(h, j)   (scaled, exponent ex)
+//   Abar_v(h, k) = sum_j X(h, j) P_{j,k}                         (interior v)
+//   M_v += sum_h X(h, :)^T 2^{ex + E_v} I_v(h, :)                (tips: I = code mask)
+// M is split over the 4 waves by rows j (16 each) and accumulated over the
+// chunk in registers; each chunk writes its own partial (reduced later in a
+// fixed order).
+template <typename T>
+__global__ __launch_bounds__(256) void k_adjoint(const AdjJob *__restrict__ jobs, int npad,
+                                                 const unsigned char *__restrict__ z,
+                                                 const double *__restrict__ PT,
+                                                 const double *__restrict__ AT, int namb,
+                                                 const double *__restrict__ mask64,
+                                                 const T *__restrict__ L, const int *__restrict__ E,
+                                                 T *__restrict__ A, int *__restrict__ EA,
+                                                 double *__restrict__ Mpart, int nchunk)
+{
+    __shared__ double sP[64 * kLdsStride];
+    __shared__ double sX[64 * kLdsStride];
+    __shared__ double sI[64 * 64];
+    __shared__ int sd[64];
+    const AdjJob &jb = jobs[blockIdx.y];
+    const int chunk = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    const int ntile = npad / 64;
+    d4 Macc[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) Macc[b] = d4{0, 0, 0, 0};
+
+    for (int t = 0; t < kAdjTilesPerChunk; t++) {
+        const int tile = chunk * kAdjTilesPerChunk + t;
+        if (tile >= ntile) break;
+        const int h0 = tile * 64, hw = h0 + 16 * w, hb = hw + g;
+        // parent adjoint
+        d4 X[4];
+        int ex[4];
+        {
+            const T *Ap = A + (size_t)jb.a_aslot * npad * 64;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) X[b][i] = Store<T>::get(Ap + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
+#pragma unroll
+            for (int i = 0; i < 4; i++) ex[i] = EA[(size_t)jb.a_aslot * npad + hb + 4 * i];
+        }
+        // siblings
+        for (int q = 0; q < jb.nsib; q++) {
+            const int pm = jb.sib_pmat[q];
+            const int tip = jb.sib_tip[q];
+            if (tip >= 0) {
+                const unsigned char *zt = z + (size_t)tip * npad;
+                const double *P = PT + (size_t)pm * 4096;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int code = zt[hb + 4 * i];
+#pragma unroll
+                    for (int b = 0; b < 4; b++) X[b][i] *= tip_value(P, AT, pm, namb, code, 16 * b + r);
+                }
+            } else {
+                const int slot = jb.sib_slot[q];
+                __syncthreads();
+                stage_p(sP, PT + (size_t)pm * 4096, tid);
+                __syncthreads();
+                d4 C[4];
+                child_contrib<T>(sP, L + (size_t)slot * npad * 64, hw, g, r, C);
+#pragma unroll
+                for (int b = 0; b < 4; b++) X[b] *= C[b];
+#pragma unroll
+                for (int i = 0; i < 4; i++) ex[i] += E[(size_t)slot * npad + hb + 4 * i];
+            }
+        }
+        // X into LDS (row = pattern in tile)
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) sX[(16 * w + g + 4 * i) * kLdsStride + 16 * b + r] = X[b][i];
+        // exponent of each pattern's M contribution: ex + E_v
+        if (r == 0)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int hl = 16 * w + g + 4 * i;
+                sd[hl] = ex[i] + (jb.v_tip >= 0 ? 0 : E[(size_t)jb.v_slot * npad + h0 + hl]);
+            }
+        if (jb.v_tip < 0) {
+            // adjoint of the interior child: Abar_v(h, k) = sum_j X(h, j) P_{j,k}
+            stage_p(sP, PT + (size_t)jb.v_pmat * 4096, tid);
+            __syncthreads();
+            double a[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) a[kk] = sX[(16 * w + r) * kLdsStride + 16 * g + kk];
+            d4 Y[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) Y[b] = d4{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    Y[b] = mfma64(a[kk], sP[(16 * b + r) * kLdsStride + 16 * g + kk], Y[b]);
+            T *Av = A + (size_t)jb.v_aslot * npad * 64;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                double m = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    Y[b][i] += X[b][i];      // identity part of P
+                    m = Y[b][i] > m ? Y[b][i] : m;
+                }
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1) {
+                    double o = __shfl_xor(m, off, 64);
+                    m = o > m ? o : m;
+                }
+                int e = 0;
+                if (m > 0) frexp(m, &e);
+                const int h = hb + 4 * i;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    Store<T>::put(Av + (size_t)h * 64 + 16 * b + r, ldexp(Y[b][i], -e),
+                                  mix32((uint32_t)jb.v_aslot ^ 0xA5A5A5A5u, (uint32_t)h,
+                                        (uint32_t)(16 * b + r)));
+                if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = ex[i] + e;
+            }
+        }
+        __syncthreads();
+        // I_v scaled by 2^d per pattern row
+        for (int q = tid; q < 4096; q += 256) {
+            const int hl = q >> 6, k = q & 63;
+            double v;
+            if (jb.v_tip >= 0) {
+                const int code = z[(size_t)jb.v_tip * npad + h0 + hl];
+                v = mask64[code * 64 + k];
+            } else {
+                v = Store<T>::get(L + ((size_t)jb.v_slot * npad + h0 + hl) * 64 + k);
+            }
+            sI[q] = v * pow2i(sd[hl]);
+        }
+        __syncthreads();
+        // M rows 16w..16w+15: A = X^T (row j, k-dim = pattern), B = I (pattern x k)
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const double av = sX[(4 * kk + g) * kLdsStride + 16 * w + r];
+#pragma unroll
+            for (int b = 0; b < 4; b++) Macc[b] = mfma64(av, sI[(4 * kk + g) * 64 + 16 * b + r], Macc[b]);
+        }
+    }
+    double *Mo = Mpart + ((size_t)jb.m_index * nchunk + chunk) * 4096;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) Mo[(16 * w + g + 4 * i) * 64 + 16 * b + r] = Macc[b][i];
+}
+
+// Deterministic sum of the chunk partials: M[m] = sum_c Mpart[m][c].
+__global__ __launch_bounds__(256) void k_mreduce(const double *__restrict__ Mpart, int nchunk,
+                                                 double *__restrict__ M)
+{
+    const int m = blockIdx.x;
+    for (int q = threadIdx.x; q < 4096; q += 256) {
+        double s = 0;
+        for (int c = 0; c < nchunk; c++) s += Mpart[((size_t)m * nchunk + c) * 4096 + q];
+        M[(size_t)m * 4096 + q] = s;
+    }
+}
+
+// Per (branch, class): A = U^T M V^T in the eigenbasis of its set;
+// d lnL / d tau = sum_a A_aa lambda_a e^{lambda_a tau} and
+// G = A o X(tau), X_ab = (e^{lambda_a tau} - e^{lambda_b tau}) / (lambda_a - lambda_b)
+// (tau e^{lambda tau} on ties), so that for any rate-matrix parameter theta
+// d lnL / d theta = <G, V dQ/dtheta U> (derivative of U e^{Lambda tau} V).
+__global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jobs,
+                                                   const double *__restrict__ M,
+                                                   const double *__restrict__ uvr,
+                                                   double *__restrict__ G, double *__restrict__ dtau)
+{
+    __shared__ double sM[64 * kLdsStride];
+    __shared__ double sU[64 * kLdsStride];
+    __shared__ double sE[64];
+    __shared__ double sred[4];
+    const EigJob jb = jobs[blockIdx.x];
+    const double *U = uvr + (size_t)jb.set * (64 * 64 * 2 + 64);
+    const double *V = U + 4096;
+    const double *R = U + 8192;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < 4096; q += 256) {
+        sM[(q >> 6) * kLdsStride + (q & 63)] = M[(size_t)jb.m_index * 4096 + q];
+        sU[(q >> 6) * kLdsStride + (q & 63)] = U[q];
+    }
+    if (tid < 64) sE[tid] = exp(R[tid] * jb.tau);
+    __syncthreads();
+    // Y = U^T M: Y[a][k] = sum_j U[j][a] M[j][k]; thread: column k = tid%64,
+    // rows a = tid/64 + 4q
+    const int col = tid & 63, r0 = tid >> 6;
+    double Y[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) Y[q] = 0;
+    for (int j = 0; j < 64; j++) {
+        const double mjk = sM[j * kLdsStride + col];
+#pragma unroll
+        for (int q = 0; q < 16; q++) Y[q] = fma(sU[j * kLdsStride + r0 + 4 * q], mjk, Y[q]);
+    }
+    __syncthreads();
+    // Y back into sM, V into sU
+#pragma unroll
+    for (int q = 0; q < 16; q++) sM[(r0 + 4 * q) * kLdsStride + col] = Y[q];
+    for (int q = tid; q < 4096; q += 256) sU[(q >> 6) * kLdsStride + (q & 63)] = V[q];
+    __syncthreads();
+    // A[a][b] = sum_k Y[a][k] V[b][k]; thread: column b = tid%64, rows a = tid/64 + 4q
+    double Aab[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) Aab[q] = 0;
+    for (int k = 0; k < 64; k++) {
+        const double vbk = sU[col * kLdsStride + k];
+#pragma unroll
+        for (int q = 0; q < 16; q++) Aab[q] = fma(sM[(r0 + 4 * q) * kLdsStride + k], vbk, Aab[q]);
+    }
+    const double lb = R[col], eb = sE[col];
+    double dsum = 0;
+    double *Go = G + (size_t)blockIdx.x * 4096;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int a = r0 + 4 * q;
+        const double la = R[a], ea = sE[a];
+        double x;
+        const double dl = la - lb;
+        if (fabs(dl * jb.tau) < 1e-6) {
+            // e^{lb tau} (e^{dl tau} - 1)/dl = e^{lb tau} tau (1 + dl tau/2 + ...)
+            const double u = dl * jb.tau;
+            x = eb * jb.tau * (1.0 + u * (0.5 + u / 6.0));
+        } else {
+            x = (ea - eb) / dl;
+        }
+        Go[a * 64 + col] = Aab[q] * x;
+        if (a == col) dsum += Aab[q] * la * ea;
+    }
+    // reduce the diagonal terms
+    for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
+    if ((tid & 63) == 0) sred[tid >> 6] = dsum;
+    __syncthreads();
+    if (tid == 0) dtau[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
+}
+
+// Per eigen set: W = sum over its (branch, class) jobs of G (fixed order),
+// then Z = V^T W U^T (so that <W, V dQ U> = <Z, dQ>).
+__global__ __launch_bounds__(256) void k_setreduce(const double *__restrict__ G,
+                                                   const int *__restrict__ set_off,
+                                                   const int *__restrict__ set_list,
+                                                   const double *__restrict__ uvr,
+                                                   double *__restrict__ Z)
+{
+    __shared__ double sW[64 * kLdsStride];
+    __shared__ double sB[64 * kLdsStride];
+    const int set = blockIdx.x, tid = threadIdx.x;
+    const double *U = uvr + (size_t)set * (64 * 64 * 2 + 64);
+    const double *V = U + 4096;
+    for (int q = tid; q < 4096; q += 256) {
+        double s = 0;
+        for (int i = set_off[set]; i < set_off[set + 1]; i++) s += G[(size_t)set_list[i] * 4096 + q];
+        sW[(q >> 6) * kLdsStride + (q & 63)] = s;
+        sB[(q >> 6) * kLdsStride + (q & 63)] = V[q];
+    }
+    __syncthreads();
+    const int col = tid & 63, r0 = tid >> 6;
+    // T1 = V^T W: T1[m][b] = sum_a V[a][m] W[a][b]
+    double T1[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) T1[q] = 0;
+    for (int a = 0; a < 64; a++) {
+        const double wab = sW[a * kLdsStride + col];
+#pragma unroll
+        for (int q = 0; q < 16; q++) T1[q] = fma(sB[a * kLdsStride + r0 + 4 * q], wab, T1[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) sW[(r0 + 4 * q) * kLdsStride + col] = T1[q];
+    for (int q = tid; q < 4096; q += 256) sB[(q >> 6) * kLdsStride + (q & 63)] = U[q];
+    __syncthreads();
+    // Z[m][n] = sum_b T1[m][b] U[n][b]
+    double Zv[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) Zv[q] = 0;
+    for (int b = 0; b < 64; b++) {
+        const double unb = sB[col * kLdsStride + b];
+#pragma unroll
+        for (int q = 0; q < 16; q++) Zv[q] = fma(sW[(r0 + 4 * q) * kLdsStride + b], unb, Zv[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) Z[(size_t)set * 4096 + (r0 + 4 * q) * 64 + col] = Zv[q];
 }
 
 // ------------------------------------------------------------------ probe
@@ -499,11 +865,44 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int
 template <typename T>
 void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,
                  const double *fpatt, double *fh, double *site, double *partial, int nblk,
-                 double *out)
+                 double *out, T *A, int *EA)
 {
-    hipLaunchKernelGGL(k_root<T>, dim3(nblk), dim3(256), 0, s, a, L, E, pi, fpatt, fh, site, partial);
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, s, (const double *)partial, nblk, out);
+    hipLaunchKernelGGL(k_root<T>, dim3(nblk), dim3(256), 0, s, a, L, E, pi, fpatt, fh, site, partial,
+                       A, EA);
+    hipLaunchKernelGGL(k_final, dim3(kRootSums), dim3(64), 0, s, (const double *)partial, nblk, out);
 }
+
+template <typename T>
+void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+                          const unsigned char *z, const double *PT, const double *AT, int namb,
+                          const double *mask64, const T *L, const int *E, T *A, int *EA,
+                          double *Mpart)
+{
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_adjoint<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
+                       namb, mask64, L, E, A, EA, Mpart, nchunk);
+}
+
+void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchunk, double *M,
+                           const EigJob *ejobs, int nej, const int *set_off,
+                           const int *set_list, const double *uvr, double *G, double *dtau,
+                           double *Z)
+{
+    hipLaunchKernelGGL(k_mreduce, dim3(nm), dim3(256), 0, s, Mpart, nchunk, M);
+    hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, (const double *)M, uvr, G,
+                       dtau);
+    hipLaunchKernelGGL(k_setreduce, dim3(3), dim3(256), 0, s, (const double *)G, set_off, set_list,
+                       uvr, Z);
+}
+
+template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int, int,
+                                           const unsigned char *, const double *, const double *,
+                                           int, const double *, const double *, const int *,
+                                           double *, int *, double *);
+template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, int, int, int,
+                                          const unsigned char *, const double *, const double *,
+                                          int, const double *, const float *, const int *, float *,
+                                          int *, double *);
 
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,
                        const float *Af, const float *Bf, float *D32)
@@ -519,9 +918,9 @@ template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, int, 
                                   float *, int *);
 template void launch_root<double>(hipStream_t, const RootArgs &, const double *, const int *,
                                   const double *, const double *, double *, double *, double *,
-                                  int, double *);
+                                  int, double *, double *, int *);
 template void launch_root<float>(hipStream_t, const RootArgs &, const float *, const int *,
                                  const double *, const double *, double *, double *, double *,
-                                 int, double *);
+                                 int, double *, float *, int *);
 
 }  // namespace bsa

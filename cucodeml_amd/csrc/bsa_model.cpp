@@ -310,4 +310,76 @@ bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, con
     return true;
 }
 
+// <Z, dQ/dkappa> and <Z, dQ/domega> at Q(kappa, omega); Z has row stride 64.
+// Off-diagonal dQ_ij/dtheta contributes dQ_ij (Z_ij - Z_ii) because the
+// diagonal is minus the row sum.  Codons with pi = 0 are held fixed by the
+// eigen split (identity rows), so their rows and columns are skipped.
+static void q_param_grads(double kappa, double omega, const double *pi, const double *Z,
+                          double *gk, double *gw)
+{
+    const CodonTables &T = tables();
+    double sk = 0, sw = 0;
+    for (int i = 0; i < kS; i++) {
+        if (!(pi[i] > 1e-100)) continue;
+        for (int j = 0; j < kS; j++) {
+            unsigned char kd = T.kind[i][j];
+            if (!kd || !(pi[j] > 1e-100)) continue;
+            const double dz = Z[i * 64 + j] - Z[i * 64 + i];
+            const double w = (kd & 2) ? omega : 1.0, k = (kd & 1) ? kappa : 1.0;
+            if (kd & 1) sk += dz * w * pi[j];
+            if (kd & 2) sw += dz * k * pi[j];
+        }
+    }
+    *gk = sk;
+    *gw = sw;
+}
+
+// d mr / d kappa and d mr / d omega of the mean rate sum_i pi_i sum_j Q_ij
+static void mean_rate_grads(double kappa, double omega, const double *pi, double *mk, double *mw)
+{
+    const CodonTables &T = tables();
+    double a = 0, b = 0;
+    for (int i = 0; i < kS; i++)
+        for (int j = 0; j < kS; j++) {
+            unsigned char kd = T.kind[i][j];
+            if (!kd) continue;
+            const double pp = pi[i] * pi[j];
+            if (kd & 1) a += pp * ((kd & 2) ? omega : 1.0);
+            if (kd & 2) b += pp * ((kd & 1) ? kappa : 1.0);
+        }
+    *mk = a;
+    *mw = b;
+}
+
+void model_a_param_grad(const ModelA &m, const double *pi, const double *Z, const double *dQf,
+                        const double *Sf, ModelGrad *g)
+{
+    const double w[3] = {m.w0, 1.0, m.w2};
+    double gk = 0, gw[3] = {0, 0, 0};
+    for (int s = 0; s < 3; s++) {
+        double k, ww;
+        q_param_grads(m.kappa, w[s], pi, Z + (size_t)s * 4096, &k, &ww);
+        gk += k;
+        gw[s] = ww;
+    }
+    const double p0 = m.p0, p1 = m.p1, t = p0 + p1;
+    // Qfactor_l = 1 / mr(kappa, wbar_l)
+    double mk[2], mw[2];
+    for (int l = 0; l < 2; l++) mean_rate_grads(m.kappa, m.wbar[l], pi, &mk[l], &mw[l]);
+    const double cQ[2] = {-dQf[0] * m.Qfactor[0] * m.Qfactor[0],
+                          -dQf[1] * m.Qfactor[1] * m.Qfactor[1]};   // d lnL / d mr_l
+    // wbar_bg = (p0 w0 + p1) / t ; wbar_fg = p0 w0 + p1 + (1 - t) w2
+    const double db_dw0 = p0 / t, db_dp0 = p1 * (m.w0 - 1) / (t * t), db_dp1 = p0 * (1 - m.w0) / (t * t);
+    const double df_dw0 = p0, df_dw2 = 1 - t, df_dp0 = m.w0 - m.w2, df_dp1 = 1 - m.w2;
+    g->kappa = gk + cQ[0] * mk[0] + cQ[1] * mk[1];
+    g->w0 = gw[0] + cQ[0] * mw[0] * db_dw0 + cQ[1] * mw[1] * df_dw0;
+    g->w2 = gw[2] + cQ[1] * mw[1] * df_dw2;
+    // class proportions (p0, p1, (1-t) p0 / t, (1-t) p1 / t)
+    const double t2 = t * t;
+    g->p0 = Sf[0] + Sf[2] * (p1 / t2 - 1) - Sf[3] * (p1 / t2) + cQ[0] * mw[0] * db_dp0 +
+            cQ[1] * mw[1] * df_dp0;
+    g->p1 = Sf[1] - Sf[2] * (p0 / t2) + Sf[3] * (p0 / t2 - 1) + cQ[0] * mw[0] * db_dp1 +
+            cQ[1] * mw[1] * df_dp1;
+}
+
 }  // namespace bsa

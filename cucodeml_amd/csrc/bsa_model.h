@@ -31,6 +31,18 @@ bool eigen_rev(const double *Q, const double *pi, double *root, double *U, doubl
 bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, const double *pi,
                    ModelA *m, bool need_eigen);
 
+// Derivatives of lnL with respect to the natural model parameters, given
+// the adjoint outputs: Z[3][64*64] (per eigen set, Z = V^T W U^T, so that
+// d lnL / d theta|_Q = <Z, dQ/dtheta>), dQf[2] = d lnL / d Qfactor for
+// background / foreground branches, Sf[4] = d lnL / d p_c (class
+// proportions).  Qfactor and the proportions are differentiated through
+// p0, p1, w0, w2 and kappa here.
+struct ModelGrad {
+    double kappa, p0, p1, w0, w2;
+};
+void model_a_param_grad(const ModelA &m, const double *pi, const double *Z, const double *dQf,
+                        const double *Sf, ModelGrad *g);
+
 // site class -> eigen set for a branch of type `label` (Set_UVR_BranchSite,
 // codeml.c:2555-2558): background w0,1,w0,1; foreground w0,1,w2,w2
 inline int uvr_set(int iclass, int label)

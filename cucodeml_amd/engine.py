@@ -21,7 +21,7 @@ BSA_FP64, BSA_FP32 = 0, 1
 BSA_KERNEL_MFMA, BSA_KERNEL_VALU = 0, 1
 
 EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
-            "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
+            "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
             "bsa_last_error", "bsa_version", "bsa_debug_mfma_probe", "bsa_work",
             "bsa_model_a_setup"]
 
@@ -59,6 +59,7 @@ def lib():
         L.bsa_set_model.argtypes = [C.c_void_p, C.POINTER(_Opts)]
         L.bsa_num_params.argtypes = [C.c_void_p]
         L.bsa_lnl.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.bsa_lnl_grad.argtypes = [C.c_void_p, dp, C.c_int, dp, dp]
         L.bsa_lnl_grad_fd.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, C.c_double, dp, dp]
         L.bsa_class_lnl.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.bsa_last_params.argtypes = [C.c_void_p, dp]
@@ -147,6 +148,16 @@ class BranchSiteEngine:
 
     lfundG = lnl
 
+    def lnl_grad(self, x):
+        """(-lnL, d(-lnL)/dx) by the adjoint sweep (ming2's dfun)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros(len(x))
+        out = C.c_double()
+        _check(lib().bsa_lnl_grad(self._h, _dptr(x), len(x), C.byref(out), _dptr(g)))
+        return out.value, g
+
+    dfun = lnl_grad
+
     def gradientB(self, x, central=False, small=1e-6):
         x = np.ascontiguousarray(x, dtype=np.float64)
         g = np.zeros(len(x))
@@ -168,18 +179,18 @@ class BranchSiteEngine:
         return o
 
     def work(self):
-        o = np.zeros(6)
+        o = np.zeros(8)
         _check(lib().bsa_work(self._h, _dptr(o)))
         return o
 
     def timing(self, reset=False):
         """dict of last / summed device times (ms) and counts"""
-        o = np.zeros(12)
+        o = np.zeros(15)
         _check(lib().bsa_timing(self._h, _dptr(o), int(reset)))
-        keys = ["pmat", "prune", "root", "adjoint", "wall"]
-        d = {f"last_{k}": o[i] for i, k in enumerate(keys)}
-        d.update({f"sum_{k}": o[5 + i] for i, k in enumerate(keys)})
-        d["n_eval"], d["n_launch"] = o[10], o[11]
+        keys = ["pmat", "prune", "root", "adjoint", "finish", "wall"]
+        d = {f"last_{k}": float(o[i]) for i, k in enumerate(keys)}
+        d.update({f"sum_{k}": float(o[6 + i]) for i, k in enumerate(keys)})
+        d["n_eval"], d["n_launch"], d["n_adj_launch"] = float(o[12]), float(o[13]), float(o[14])
         return d
 
 

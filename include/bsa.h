@@ -96,6 +96,15 @@ int  bsa_num_params(const bsa_engine *eng);          /* np, or -1          */
 /* -lnL at x (np values), as codeml's lfundG returns it */
 int  bsa_lnl(bsa_engine *eng, const double *x, int np, double *neg_lnl);
 
+/* -lnL and its gradient d(-lnL)/dx in the same variables (ming2's dfun
+ * contract, tools.c:6361, 6408, 6461-6462), from one forward pass and one
+ * adjoint (preorder) sweep: branch-length derivatives through
+ * dP/dt = Qfactor U diag(lambda e^{lambda t}) V, rate-matrix parameters through
+ * the eigen derivative of P(t), and the chain rule through Qfactor, the class
+ * proportions and the logistic p0, p1.  Replaces the finite-difference
+ * gradient the reference uses because it passes dfun = NULL (codeml.c:688). */
+int  bsa_lnl_grad(bsa_engine *eng, const double *x, int np, double *neg_lnl, double *grad);
+
 /* -lnL and its finite-difference gradient d(-lnL)/dx in the same variables,
  * as gradientB (tools.c:6326-6350): step small*(|x_i|+1) (small <= 0 means
  * 1e-6, Small_Diff), forward differences (np+1 evaluations) or, with
@@ -103,7 +112,7 @@ int  bsa_lnl(bsa_engine *eng, const double *x, int np, double *neg_lnl);
 int  bsa_lnl_grad_fd(bsa_engine *eng, const double *x, int np, int central, double small,
                      double *neg_lnl, double *grad);
 
-/* after bsa_lnl / bsa_lnl_grad: per-class log f(x_h | class) [4*npatt]
+/* after bsa_lnl / bsa_lnl_grad / bsa_lnl_grad_fd: per-class log f(x_h | class) [4*npatt]
  * (class-major, as com.fhK with node scaling) and the mixture log f_h
  * [npatt]; either pointer may be NULL */
 int  bsa_class_lnl(bsa_engine *eng, double *fhK, double *site_lnf);
@@ -115,11 +124,13 @@ int  bsa_last_params(const bsa_engine *eng, double *out11);
 
 /* timing in ms, from HIP events on the engine's stream: out[0] P(t) and tip
  * tables, out[1] pruning (all levels), out[2] root reduction, out[3]
- * adjoint (0 if none), out[4] host wall time of the whole evaluation, all
- * for the last evaluation; out[5..9] the same summed since creation or the
- * last reset; out[10] evaluations and out[11] pruning launches in that sum.
+ * adjoint sweep (all levels; 0 without gradient), out[4] adjoint finish
+ * (branch-matrix reduction and eigen-basis kernels), out[5] host wall time
+ * of the whole evaluation, all for the last evaluation; out[6..11] the same
+ * summed since creation or the last reset; out[12] evaluations, out[13]
+ * pruning launches and out[14] adjoint-sweep launches in that sum.
  * reset != 0 clears the sums after reading them. */
-int  bsa_timing(bsa_engine *eng, double *out12, int reset);
+int  bsa_timing(bsa_engine *eng, double *out15, int reset);
 
 /* host only (no GPU): model A at natural parameters, as SetParametersNSsites
  * (codeml.c:2470-2520) with eigenQcodon/eigenQREV (codeml.c:3103-3272,
@@ -130,13 +141,15 @@ int  bsa_model_a_setup(double kappa, double p0, double p1, double w0, double w2,
                        const double *pi61, double *freqK4, double *Qfactor2, double *Root,
                        double *U, double *V);
 
-/* work of one evaluation's pruning pass, counted on 61 states:
- * out[0] flops executed (site classes 2/3 share partials with 0/1 off the
- * foreground path), out[1] algorithmic HBM bytes executed (partials read
- * and written, tip codes, scale exponents), out[2] flops and out[3] bytes of
- * the reference algorithm (4 classes over the whole tree), out[4] pruning
- * launches (tree levels), out[5] (node, class) jobs */
-int  bsa_work(const bsa_engine *eng, double *out6);
+/* work of one evaluation, counted on 61 states:
+ * out[0] pruning flops executed (site classes 2/3 share partials with 0/1
+ * off the foreground path), out[1] pruning algorithmic HBM bytes executed
+ * (partials read and written, tip codes, scale exponents), out[2] flops and
+ * out[3] bytes of the reference algorithm's pruning (4 classes over the
+ * whole tree), out[4] pruning launches (tree levels), out[5] pruning
+ * (node, class) jobs, out[6] adjoint-sweep flops and out[7] adjoint-sweep
+ * algorithmic bytes per gradient */
+int  bsa_work(bsa_engine *eng, double *out8);
 
 const char *bsa_last_error(void);
 
