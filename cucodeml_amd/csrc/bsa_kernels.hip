@@ -766,13 +766,23 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     if (tid == 0) dtau[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
 }
 
-// Per eigen set: W = sum over its (branch, class) jobs of G (fixed order),
-// then Z = V^T W U^T (so that <W, V dQ U> = <Z, dQ>).
-__global__ __launch_bounds__(256) void k_setreduce(const double *__restrict__ G,
-                                                   const int *__restrict__ set_off,
-                                                   const int *__restrict__ set_list,
-                                                   const double *__restrict__ uvr,
-                                                   double *__restrict__ Z)
+// Per eigen set: W = sum over its (branch, class) jobs of G, in a fixed
+// order; one thread per matrix element, 16 blocks per set.
+__global__ __launch_bounds__(256) void k_wreduce(const double *__restrict__ G,
+                                                 const int *__restrict__ set_off,
+                                                 const int *__restrict__ set_list,
+                                                 double *__restrict__ W)
+{
+    const int set = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
+    double s = 0;
+    for (int i = set_off[set]; i < set_off[set + 1]; i++) s += G[(size_t)set_list[i] * 4096 + q];
+    W[(size_t)set * 4096 + q] = s;
+}
+
+// Per eigen set: Z = V^T W U^T (so that <W, V dQ U> = <Z, dQ>).
+__global__ __launch_bounds__(256) void k_zform(const double *__restrict__ W,
+                                               const double *__restrict__ uvr,
+                                               double *__restrict__ Z)
 {
     __shared__ double sW[64 * kLdsStride];
     __shared__ double sB[64 * kLdsStride];
@@ -780,9 +790,7 @@ __global__ __launch_bounds__(256) void k_setreduce(const double *__restrict__ G,
     const double *U = uvr + (size_t)set * (64 * 64 * 2 + 64);
     const double *V = U + 4096;
     for (int q = tid; q < 4096; q += 256) {
-        double s = 0;
-        for (int i = set_off[set]; i < set_off[set + 1]; i++) s += G[(size_t)set_list[i] * 4096 + q];
-        sW[(q >> 6) * kLdsStride + (q & 63)] = s;
+        sW[(q >> 6) * kLdsStride + (q & 63)] = W[(size_t)set * 4096 + q];
         sB[(q >> 6) * kLdsStride + (q & 63)] = V[q];
     }
     __syncthreads();
@@ -891,8 +899,10 @@ void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchun
     hipLaunchKernelGGL(k_mreduce, dim3(nm), dim3(256), 0, s, Mpart, nchunk, M);
     hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, (const double *)M, uvr, G,
                        dtau);
-    hipLaunchKernelGGL(k_setreduce, dim3(3), dim3(256), 0, s, (const double *)G, set_off, set_list,
-                       uvr, Z);
+    // W reuses the first three matrices of M (no longer needed)
+    hipLaunchKernelGGL(k_wreduce, dim3(3, 16), dim3(256), 0, s, (const double *)G, set_off, set_list,
+                       M);
+    hipLaunchKernelGGL(k_zform, dim3(3), dim3(256), 0, s, (const double *)M, uvr, Z);
 }
 
 template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int, int,

@@ -36,3 +36,49 @@ def site_to_pattern(fx, data):
 def params(fx):
     p = fx["fixed_params"]
     return p["kappa"], p["p0"], p["p1"], p["w0"], p["w2"]
+
+
+# control file of the reference codeml runs (model A; {fix_omega}, {omega})
+CTL = """seqfile = seq.phy
+treefile = tree.nwk
+outfile = out
+noisy = 0
+verbose = 0
+runmode = 0
+seqtype = 1
+CodonFreq = 2
+clock = 0
+aaDist = 0
+model = 2
+NSsites = 2
+icode = 0
+Mgene = 0
+fix_kappa = 0
+kappa = 2
+fix_omega = {fix_omega}
+omega = {omega}
+fix_alpha = 1
+alpha = 0
+Malpha = 0
+getSE = 0
+RateAncestor = 0
+Small_Diff = 1e-6
+cleandata = 0
+fix_blength = 0
+method = 0
+"""
+
+
+def parse_out(text):
+    """(lnL, np, x) from a codeml `out` file (codeml.c:738-751)."""
+    import re
+    m = re.search(r"lnL\(ntime:\s*(\d+)\s+np:\s*(\d+)\):\s+(-?\d+\.\d+)", text)
+    lnl, npar = float(m.group(3)), int(m.group(2))
+    for line in text[m.end():].splitlines()[1:6]:
+        vals = line.split()
+        if len(vals) == npar:
+            try:
+                return lnl, npar, [float(v) for v in vals]
+            except ValueError:
+                pass
+    return lnl, npar, None
