@@ -610,48 +610,80 @@ static int setup_adjoint(bsa_engine *e)
     for (int v = 0; v < nn; v++) maxd = std::max(maxd, depth[v]);
     auto aslot = [&](int v, int c) { return (v - ns) * 4 + c; };
     for (int c = 0; c < 4; c++) e->root_aslot[c] = aslot(e->root, c);
+    // Nodes off the foreground path with a background branch hold one
+    // adjoint per merged class pair (0+2, 1+3): their partials and P(t) are
+    // the same for both classes, so the two adjoints can be summed.
+    auto merged = [&](int v) {
+        return v != e->root && !e->onpath[v] && e->label[v] == 0;
+    };
     e->adj.clear();
     e->adj_off.assign(1, 0);
+    e->ejobs.clear();
+    e->ejob_node.clear();
+    std::vector<int> lists[3];
+    auto fill_sibs = [&](AdjJob &j, int term, int v, int a, int c) {
+        int n = 0;
+        for (int s2 : e->sons[a]) {
+            if (s2 == v) continue;
+            j.sib_tip[n] = s2 < ns ? s2 : -1;
+            j.sib_slot[term][n] = s2 < ns ? -1 : e->slot[s2][c];
+            j.sib_pmat[term][n] = s2 * 3 + uvr_set(c, e->label[s2]);
+            n++;
+        }
+        j.nsib = n;
+    };
+    auto add_eig = [&](int m_index, int set, int v) {
+        EigJob ej{};
+        ej.m_index = m_index;
+        ej.set = set;
+        ej.tau = 0;
+        lists[set].push_back((int)e->ejobs.size());
+        e->ejobs.push_back(ej);
+        e->ejob_node.push_back(v);
+    };
     for (int d = 1; d <= maxd; d++) {
         for (int v = 0; v < nn; v++) {
             if (depth[v] != d) continue;
             const int a = e->father[v];
-            for (int c = 0; c < 4; c++) {
-                AdjJob j{};
-                j.v_tip = v < ns ? v : -1;
-                j.v_slot = v < ns ? -1 : e->slot[v][c];
-                j.v_pmat = v * 3 + uvr_set(c, e->label[v]);
-                j.v_aslot = v < ns ? -1 : aslot(v, c);
-                j.a_aslot = aslot(a, c);
-                j.m_index = v * 4 + c;
-                int n = 0;
-                for (int s2 : e->sons[a]) {
-                    if (s2 == v) continue;
-                    j.sib_tip[n] = s2 < ns ? s2 : -1;
-                    j.sib_slot[n] = s2 < ns ? -1 : e->slot[s2][c];
-                    j.sib_pmat[n] = s2 * 3 + uvr_set(c, e->label[s2]);
-                    n++;
+            if (merged(v)) {
+                for (int m = 0; m < 2; m++) {
+                    AdjJob j{};
+                    j.v_tip = v < ns ? v : -1;
+                    j.v_slot = v < ns ? -1 : e->slot[v][m];
+                    j.v_pmat = v * 3 + uvr_set(m, 0);
+                    j.v_aslot = v < ns ? -1 : aslot(v, m);
+                    j.m_index = v * 4 + m;
+                    if (merged(a)) {
+                        j.nterm = 1;
+                        j.a_aslot[0] = aslot(a, m);
+                        fill_sibs(j, 0, v, a, m);
+                    } else {
+                        j.nterm = 2;
+                        for (int t = 0; t < 2; t++) {
+                            j.a_aslot[t] = aslot(a, m + 2 * t);
+                            fill_sibs(j, t, v, a, m + 2 * t);
+                        }
+                    }
+                    e->adj.push_back(j);
+                    add_eig(j.m_index, uvr_set(m, 0), v);
                 }
-                j.nsib = n;
-                e->adj.push_back(j);
+            } else {
+                for (int c = 0; c < 4; c++) {
+                    AdjJob j{};
+                    j.v_tip = v < ns ? v : -1;
+                    j.v_slot = v < ns ? -1 : e->slot[v][c];
+                    j.v_pmat = v * 3 + uvr_set(c, e->label[v]);
+                    j.v_aslot = v < ns ? -1 : aslot(v, c);
+                    j.m_index = v * 4 + c;
+                    j.nterm = 1;
+                    j.a_aslot[0] = aslot(a, c);   // the parent of an unmerged node is unmerged
+                    fill_sibs(j, 0, v, a, c);
+                    e->adj.push_back(j);
+                    add_eig(j.m_index, uvr_set(c, e->label[v]), v);
+                }
             }
         }
         e->adj_off.push_back((int)e->adj.size());
-    }
-    e->ejobs.clear();
-    e->ejob_node.clear();
-    std::vector<int> lists[3];
-    for (int v = 0; v < nn; v++) {
-        if (v == e->root) continue;
-        for (int c = 0; c < 4; c++) {
-            EigJob j{};
-            j.m_index = v * 4 + c;
-            j.set = uvr_set(c, e->label[v]);
-            j.tau = 0;
-            lists[j.set].push_back((int)e->ejobs.size());
-            e->ejobs.push_back(j);
-            e->ejob_node.push_back(v);
-        }
     }
     e->set_off = {0};
     e->set_list.clear();
@@ -699,9 +731,11 @@ static int setup_adjoint(bsa_engine *e)
             int nint = 0;
             for (int q = 0; q < j.nsib; q++) nint += j.sib_tip[q] < 0;
             const bool vint = j.v_tip < 0;
-            fl += 2.0 * kS * kS * np_ * (nint + (vint ? 2 : 1));
-            by += np_ * (kS * w * (1 + nint + (vint ? 2 : 0)) + 4.0 * (1 + nint + (vint ? 2 : 0)) +
-                         (j.nsib - nint) + (vint ? 0 : 1));
+            const int nt = j.nterm;
+            fl += 2.0 * kS * kS * np_ * (nt * nint + (vint ? 2 : 1));
+            by += np_ * (kS * w * (nt * (1 + nint) + (vint ? 2 : 0)) +
+                         4.0 * (nt * (1 + nint) + (vint ? 2 : 0)) + nt * (j.nsib - nint) +
+                         (vint ? 0 : 1));
         }
         e->work[6] = fl;
         e->work[7] = by;

@@ -36,22 +36,26 @@ struct RootArgs {
 // Sf_c = sum_h fpatt_h f_hc / f_h (= d lnL / d p_c)
 constexpr int kRootSums = 5;
 
-// One job of the adjoint (preorder) sweep: branch above node v, site class c.
-// X = (adjoint of the parent) x (contributions of v's siblings) is the
-// derivative of lnL with respect to v's contribution to its parent; from it
-// the job writes v's adjoint (if v is interior) and the branch matrix
-// M = sum_h X_h^T I_h over its chunk of patterns.
+// One job of the adjoint (preorder) sweep: the branch above node v for one
+// site class, or for a merged pair of classes (m, m+2) where v's partials and
+// P(t) are shared by both.  Each term t (one or two) gives
+// X_t = (adjoint of the parent, class of term t) x (contributions of v's
+// siblings for that class); X = sum_t X_t is the derivative of lnL with
+// respect to v's contribution to its parent.  From it the job writes v's
+// adjoint (if v is interior) and the branch matrix M = sum_h X_h^T I_h over
+// its chunk of patterns.
 struct AdjJob {
     int v_tip;                  // tip index of v, or -1
     int v_slot;                 // inside partial slot of v (interior)
-    int v_pmat;                 // P(t) matrix index of v's branch for class c
+    int v_pmat;                 // P(t) matrix index of v's branch
     int v_aslot;                // adjoint slot written for v (interior) or -1
-    int a_aslot;                // adjoint slot of the parent
     int m_index;                // row of the branch-matrix buffer
+    int nterm;                  // 1 or 2
+    int a_aslot[2];             // adjoint slot of the parent, per term
     int nsib;
     int sib_tip[kMaxCh];
-    int sib_slot[kMaxCh];
-    int sib_pmat[kMaxCh];
+    int sib_slot[2][kMaxCh];
+    int sib_pmat[2][kMaxCh];
 };
 
 // per (branch, class) eigen-basis job: M -> (d lnL / d tau, A o X)

@@ -523,16 +523,19 @@ __device__ inline void child_contrib(const double *sP, const T *Ls, int hw, int 
         for (int i = 0; i < 4; i++) C[b][i] += Store<T>::get(Ls + (size_t)(hw + g + 4 * i) * 64 + 16 * b + r);
 }
 
-// One (branch v, class c, chunk of 64-pattern tiles) job of the preorder
-// adjoint sweep.  Per tile:
-//   X(h, j) = Abar_parent(h, j) * prod_{siblings s} C_s(h, j)   (scaled, exponent ex)
-//   Abar_v(h, k) = sum_j X(h, j) P_{j,k}                         (interior v)
-//   M_v += sum_h X(h, :)^T 2^{ex + E_v} I_v(h, :)                (tips: I = code mask)
-// M is split over the 4 waves by rows j (16 each) and accumulated over the
-// chunk in registers; each chunk writes its own partial (reduced later in a
-// fixed order).
+// One (branch v, class or merged class pair, chunk of 64-pattern tiles) job
+// of the preorder adjoint sweep.  Per tile:
+//   X_t(h, j) = Abar_parent,t(h, j) * prod_{siblings s} C_s,t(h, j)   per term t
+//   X = sum_t X_t (exponents aligned), scaled by 2^ex per pattern
+//   Abar_v(h, k) = sum_j X(h, j) P_{j,k}                          (interior v)
+//   M_v[j][k] += sum_h X(h, j) 2^{ex + E_v} I_v(h, k)              (tips: code mask)
+// The M product is split over the 4 waves by its columns k (16 each): the A
+// operand X^T comes from LDS, the B operand I is read once per wave from
+// HBM/L2.  M accumulates over the chunk in registers and each chunk writes its
+// own partial (summed later in a fixed order).  LDS: 2 x 64 x 65 doubles, so
+// two blocks share a CU.
 template <typename T>
-__global__ __launch_bounds__(256) void k_adjoint(const AdjJob *__restrict__ jobs, int npad,
+__global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ jobs, int npad,
                                                  const unsigned char *__restrict__ z,
                                                  const double *__restrict__ PT,
                                                  const double *__restrict__ AT, int namb,
@@ -543,12 +546,12 @@ __global__ __launch_bounds__(256) void k_adjoint(const AdjJob *__restrict__ jobs
 {
     __shared__ double sP[64 * kLdsStride];
     __shared__ double sX[64 * kLdsStride];
-    __shared__ double sI[64 * 64];
     __shared__ int sd[64];
     const AdjJob &jb = jobs[blockIdx.y];
     const int chunk = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
     const int ntile = npad / 64;
+    const bool vtip = jb.v_tip >= 0;
     d4 Macc[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) Macc[b] = d4{0, 0, 0, 0};
@@ -557,42 +560,60 @@ __global__ __launch_bounds__(256) void k_adjoint(const AdjJob *__restrict__ jobs
         const int tile = chunk * kAdjTilesPerChunk + t;
         if (tile >= ntile) break;
         const int h0 = tile * 64, hw = h0 + 16 * w, hb = hw + g;
-        // parent adjoint
         d4 X[4];
         int ex[4];
-        {
-            const T *Ap = A + (size_t)jb.a_aslot * npad * 64;
+        for (int term = 0; term < jb.nterm; term++) {
+            d4 Xt[4];
+            int et[4];
+            const int asl = jb.a_aslot[term];
+            const T *Ap = A + (size_t)asl * npad * 64;
 #pragma unroll
             for (int b = 0; b < 4; b++)
 #pragma unroll
-                for (int i = 0; i < 4; i++) X[b][i] = Store<T>::get(Ap + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
+                for (int i = 0; i < 4; i++)
+                    Xt[b][i] = Store<T>::get(Ap + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
 #pragma unroll
-            for (int i = 0; i < 4; i++) ex[i] = EA[(size_t)jb.a_aslot * npad + hb + 4 * i];
-        }
-        // siblings
-        for (int q = 0; q < jb.nsib; q++) {
-            const int pm = jb.sib_pmat[q];
-            const int tip = jb.sib_tip[q];
-            if (tip >= 0) {
-                const unsigned char *zt = z + (size_t)tip * npad;
-                const double *P = PT + (size_t)pm * 4096;
+            for (int i = 0; i < 4; i++) et[i] = EA[(size_t)asl * npad + hb + 4 * i];
+            for (int q = 0; q < jb.nsib; q++) {
+                const int pm = jb.sib_pmat[term][q];
+                const int tip = jb.sib_tip[q];
+                if (tip >= 0) {
+                    const unsigned char *zt = z + (size_t)tip * npad;
+                    const double *P = PT + (size_t)pm * 4096;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int code = zt[hb + 4 * i];
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            Xt[b][i] *= tip_value(P, AT, pm, namb, code, 16 * b + r);
+                    }
+                } else {
+                    const int slot = jb.sib_slot[term][q];
+                    __syncthreads();
+                    stage_p(sP, PT + (size_t)pm * 4096, tid);
+                    __syncthreads();
+                    d4 C[4];
+                    child_contrib<T>(sP, L + (size_t)slot * npad * 64, hw, g, r, C);
+#pragma unroll
+                    for (int b = 0; b < 4; b++) Xt[b] *= C[b];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) et[i] += E[(size_t)slot * npad + hb + 4 * i];
+                }
+            }
+            if (term == 0) {
+#pragma unroll
+                for (int b = 0; b < 4; b++) X[b] = Xt[b];
+#pragma unroll
+                for (int i = 0; i < 4; i++) ex[i] = et[i];
+            } else {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const int code = zt[hb + 4 * i];
+                    const int em = ex[i] > et[i] ? ex[i] : et[i];
+                    const double f0 = pow2i(ex[i] - em), f1 = pow2i(et[i] - em);
 #pragma unroll
-                    for (int b = 0; b < 4; b++) X[b][i] *= tip_value(P, AT, pm, namb, code, 16 * b + r);
+                    for (int b = 0; b < 4; b++) X[b][i] = X[b][i] * f0 + Xt[b][i] * f1;
+                    ex[i] = em;
                 }
-            } else {
-                const int slot = jb.sib_slot[q];
-                __syncthreads();
-                stage_p(sP, PT + (size_t)pm * 4096, tid);
-                __syncthreads();
-                d4 C[4];
-                child_contrib<T>(sP, L + (size_t)slot * npad * 64, hw, g, r, C);
-#pragma unroll
-                for (int b = 0; b < 4; b++) X[b] *= C[b];
-#pragma unroll
-                for (int i = 0; i < 4; i++) ex[i] += E[(size_t)slot * npad + hb + 4 * i];
             }
         }
         // X into LDS (row = pattern in tile)
@@ -606,9 +627,9 @@ __global__ __launch_bounds__(256) void k_adjoint(const AdjJob *__restrict__ jobs
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int hl = 16 * w + g + 4 * i;
-                sd[hl] = ex[i] + (jb.v_tip >= 0 ? 0 : E[(size_t)jb.v_slot * npad + h0 + hl]);
+                sd[hl] = ex[i] + (vtip ? 0 : E[(size_t)jb.v_slot * npad + h0 + hl]);
             }
-        if (jb.v_tip < 0) {
+        if (!vtip) {
             // adjoint of the interior child: Abar_v(h, k) = sum_j X(h, j) P_{j,k}
             stage_p(sP, PT + (size_t)jb.v_pmat * 4096, tid);
             __syncthreads();
@@ -647,34 +668,33 @@ __global__ __launch_bounds__(256) void k_adjoint(const AdjJob *__restrict__ jobs
                                         (uint32_t)(16 * b + r)));
                 if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = ex[i] + e;
             }
+        } else {
+            __syncthreads();
         }
-        __syncthreads();
-        // I_v scaled by 2^d per pattern row
-        for (int q = tid; q < 4096; q += 256) {
-            const int hl = q >> 6, k = q & 63;
-            double v;
-            if (jb.v_tip >= 0) {
-                const int code = z[(size_t)jb.v_tip * npad + h0 + hl];
-                v = mask64[code * 64 + k];
-            } else {
-                v = Store<T>::get(L + ((size_t)jb.v_slot * npad + h0 + hl) * 64 + k);
-            }
-            sI[q] = v * pow2i(sd[hl]);
-        }
-        __syncthreads();
-        // M rows 16w..16w+15: A = X^T (row j, k-dim = pattern), B = I (pattern x k)
-#pragma unroll
+        // M columns 16w..16w+15: A = X^T (row j, k-dim = pattern) from LDS,
+        // B = 2^d I (pattern x column) read from HBM / L2
+        const int col = 16 * w + r;
+#pragma unroll 4
         for (int kk = 0; kk < 16; kk++) {
-            const double av = sX[(4 * kk + g) * kLdsStride + 16 * w + r];
+            const int hl = 4 * kk + g;
+            double bv;
+            if (vtip) {
+                const int code = z[(size_t)jb.v_tip * npad + h0 + hl];
+                bv = code < 61 ? (code == col ? 1.0 : 0.0) : mask64[code * 64 + col];
+            } else {
+                bv = Store<T>::get(L + ((size_t)jb.v_slot * npad + h0 + hl) * 64 + col);
+            }
+            bv *= pow2i(sd[hl]);
 #pragma unroll
-            for (int b = 0; b < 4; b++) Macc[b] = mfma64(av, sI[(4 * kk + g) * 64 + 16 * b + r], Macc[b]);
+            for (int b = 0; b < 4; b++)
+                Macc[b] = mfma64(sX[hl * kLdsStride + 16 * b + r], bv, Macc[b]);
         }
     }
     double *Mo = Mpart + ((size_t)jb.m_index * nchunk + chunk) * 4096;
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) Mo[(16 * w + g + 4 * i) * 64 + 16 * b + r] = Macc[b][i];
+        for (int i = 0; i < 4; i++) Mo[(16 * b + g + 4 * i) * 64 + 16 * w + r] = Macc[b][i];
 }
 
 // Deterministic sum of the chunk partials: M[m] = sum_c Mpart[m][c].
