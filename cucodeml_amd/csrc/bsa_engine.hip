@@ -103,7 +103,7 @@ struct bsa_engine {
     EigJob *d_ejobs = nullptr;
     EigJob *h_ejobs = nullptr;
     int *d_set_off = nullptr, *d_set_list = nullptr;
-    void *d_A = nullptr;
+    void *d_A = nullptr, *d_C = nullptr;
     int *d_EA = nullptr;
     double *d_Mpart = nullptr, *d_M = nullptr, *d_G = nullptr, *d_dtau = nullptr, *d_Z = nullptr;
     double *d_mask64 = nullptr;
@@ -120,8 +120,8 @@ static void free_all(bsa_engine *e)
     void *dev_ptrs[] = {e->d_uvr, e->d_pjobs, e->d_prune, e->d_PT, e->d_AT, e->d_L, e->d_E,
                         e->d_z, e->d_mask_amb, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
                         e->d_partial, e->d_out, e->d_adj, e->d_ejobs, e->d_set_off,
-                        e->d_set_list, e->d_A, e->d_EA, e->d_Mpart, e->d_M, e->d_G, e->d_dtau,
-                        e->d_Z, e->d_mask64};
+                        e->d_set_list, e->d_A, e->d_C, e->d_EA, e->d_Mpart, e->d_M, e->d_G,
+                        e->d_dtau, e->d_Z, e->d_mask64};
     for (void *p : dev_ptrs)
         if (p) hipFree(p);
     if (e->h_uvr) hipHostFree(e->h_uvr);
@@ -207,6 +207,7 @@ static int build_schedule(bsa_engine *e)
                     j.ch_tip[k] = s < ns ? s : -1;
                     j.ch_slot[k] = s < ns ? -1 : slot[s][c];
                     j.ch_pmat[k] = s * 3 + uvr_set(c, e->label[s]);
+                    j.ch_cslot[k] = s < ns ? -1 : (s - ns) * 4 + c;
                 }
                 e->prune.push_back(j);
             }
@@ -452,7 +453,7 @@ static int run_device(bsa_engine *e, bool adjoint)
         int a = e->level_off[l], b = e->level_off[l + 1];
         launch_prune<T>(s, e->kernel, e->tile, e->d_prune + a, b - a, e->npad, e->d_z,
                         (const double *)e->d_PT, (const double *)e->d_AT, e->namb, (T *)e->d_L,
-                        e->d_E);
+                        e->d_E, adjoint ? (T *)e->d_C : nullptr);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev[2], s));
@@ -473,8 +474,8 @@ static int run_device(bsa_engine *e, bool adjoint)
             int a = e->adj_off[d], b = e->adj_off[d + 1];
             launch_adjoint_level<T>(s, e->d_adj + a, b - a, e->npad, e->nchunk, e->d_z,
                                     (const double *)e->d_PT, (const double *)e->d_AT, e->namb,
-                                    e->d_mask64, (const T *)e->d_L, e->d_E, (T *)e->d_A, e->d_EA,
-                                    e->d_Mpart);
+                                    e->d_mask64, (const T *)e->d_L, e->d_E, (const T *)e->d_C,
+                                    (T *)e->d_A, e->d_EA, e->d_Mpart);
         }
         HIPCHK(hipEventRecord(e->ev[4], s));
         launch_adjoint_finish(s, e->d_Mpart, e->nmrow, e->nchunk, e->d_M, e->d_ejobs,
@@ -621,12 +622,16 @@ static int setup_adjoint(bsa_engine *e)
     e->ejobs.clear();
     e->ejob_node.clear();
     std::vector<int> lists[3];
+    // contributions of a's children were kept by the forward job (a, c') with
+    // c' = c when a has its own class-c job (c < 2 or a on the path), else c - 2
     auto fill_sibs = [&](AdjJob &j, int term, int v, int a, int c) {
+        const int cf = (c < 2 || e->onpath[a]) ? c : c - 2;
         int n = 0;
         for (int s2 : e->sons[a]) {
             if (s2 == v) continue;
             j.sib_tip[n] = s2 < ns ? s2 : -1;
             j.sib_slot[term][n] = s2 < ns ? -1 : e->slot[s2][c];
+            j.sib_cslot[term][n] = s2 < ns ? -1 : (s2 - ns) * 4 + cf;
             j.sib_pmat[term][n] = s2 * 3 + uvr_set(c, e->label[s2]);
             n++;
         }
@@ -707,6 +712,7 @@ static int setup_adjoint(bsa_engine *e)
     HIPCHK(hipMemcpy(e->d_set_list, e->set_list.data(), sizeof(int) * e->set_list.size(),
                      hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&e->d_A, ts * 64 * (size_t)e->npad * nasl));
+    HIPCHK(hipMalloc(&e->d_C, ts * 64 * (size_t)e->npad * nasl));
     HIPCHK(hipMalloc(&e->d_EA, sizeof(int) * (size_t)e->npad * nasl));
     HIPCHK(hipMalloc(&e->d_Mpart, sizeof(double) * 4096 * (size_t)e->nmrow * e->nchunk));
     HIPCHK(hipMemset(e->d_Mpart, 0, sizeof(double) * 4096 * (size_t)e->nmrow * e->nchunk));
@@ -732,7 +738,7 @@ static int setup_adjoint(bsa_engine *e)
             for (int q = 0; q < j.nsib; q++) nint += j.sib_tip[q] < 0;
             const bool vint = j.v_tip < 0;
             const int nt = j.nterm;
-            fl += 2.0 * kS * kS * np_ * (nt * nint + (vint ? 2 : 1));
+            fl += 2.0 * kS * kS * np_ * (vint ? 2 : 1);
             by += np_ * (kS * w * (nt * (1 + nint) + (vint ? 2 : 0)) +
                          4.0 * (nt * (1 + nint) + (vint ? 2 : 0)) + nt * (j.nsib - nint) +
                          (vint ? 0 : 1));

@@ -15,6 +15,9 @@ struct PruneJob {
     int ch_tip[kMaxCh];         // tip index, or -1 for an interior child
     int ch_slot[kMaxCh];        // partial slot of an interior child
     int ch_pmat[kMaxCh];        // P(t) matrix index (node * 3 + eigen set)
+    int ch_cslot[kMaxCh];       // slot of the child's contribution P I in the
+                                // contribution buffer (interior children), kept
+                                // for the adjoint sweep when that buffer is given
 };
 
 // One P(t) matrix to build: branch above `node`, eigen set `set`.
@@ -54,8 +57,9 @@ struct AdjJob {
     int a_aslot[2];             // adjoint slot of the parent, per term
     int nsib;
     int sib_tip[kMaxCh];
-    int sib_slot[2][kMaxCh];
-    int sib_pmat[2][kMaxCh];
+    int sib_slot[2][kMaxCh];    // inside slot of an interior sibling (its exponent), per term
+    int sib_cslot[2][kMaxCh];   // contribution slot of an interior sibling, per term
+    int sib_pmat[2][kMaxCh];    // P(t) of a tip sibling, per term
 };
 
 // per (branch, class) eigen-basis job: M -> (d lnL / d tau, A o X)
@@ -70,8 +74,8 @@ constexpr int kAdjTilesPerChunk = 16;   // 64-pattern tiles per adjoint block
 template <typename T>
 void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
                           const unsigned char *z, const double *PT, const double *AT, int namb,
-                          const double *mask64, const T *L, const int *E, T *A, int *EA,
-                          double *Mpart);
+                          const double *mask64, const T *L, const int *E, const T *Cb, T *A,
+                          int *EA, double *Mpart);
 
 // sums the chunk partials of nm branch matrices, then for every EigJob
 // forms A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set
@@ -86,10 +90,11 @@ void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njob
                  const unsigned char *mask_amb, int namb, double *AT);
 
 // kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64)
+// Cb: contribution buffer (interior children's P I per class) or nullptr
 template <typename T>
 void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
                   const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
-                  int *E);
+                  int *E, T *Cb);
 
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with
 // a.seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA)

@@ -187,7 +187,8 @@ __global__ __launch_bounds__(256) void k_prune_mfma(const PruneJob *__restrict__
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
                                                     const double *__restrict__ AT, int namb,
-                                                    T *__restrict__ L, int *__restrict__ E)
+                                                    T *__restrict__ L, int *__restrict__ E,
+                                                    T *__restrict__ Cb)
 {
     __shared__ double sP[64 * 64];
     const PruneJob &jb = jobs[blockIdx.y];
@@ -247,8 +248,20 @@ __global__ __launch_bounds__(256) void k_prune_mfma(const PruneJob *__restrict__
                 for (int b = 0; b < 4; b++)
 #pragma unroll
                     for (int i = 0; i < 4; i++)
-                        acc[q][b][i] *=
-                            tmp[b][i] + Store<T>::get(Lc + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
+                        tmp[b][i] += Store<T>::get(Lc + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
+                if (Cb) {
+                    const int cs = jb.ch_cslot[c];
+                    T *Co = Cb + (size_t)cs * npad * 64;
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            Store<T>::put(Co + (size_t)(hb + 4 * i) * 64 + 16 * b + r, tmp[b][i],
+                                          mix32((uint32_t)cs ^ 0x3C6EF372u, (uint32_t)(hb + 4 * i),
+                                                (uint32_t)(16 * b + r)));
+                }
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[q][b] *= tmp[b];
 #pragma unroll
                 for (int i = 0; i < 4; i++) esum[q][i] += Ec[hb + 4 * i];
             }
@@ -291,7 +304,8 @@ __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
                                                     const double *__restrict__ AT, int namb,
-                                                    T *__restrict__ L, int *__restrict__ E)
+                                                    T *__restrict__ L, int *__restrict__ E,
+                                                    T *__restrict__ Cb)
 {
     __shared__ double sP[64 * 64];
     __shared__ double sA[64 * 64];
@@ -327,9 +341,15 @@ __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__
                 for (int i = 0; i < 16; i++) tmp[i] = fma(sA[(16 * pg + i) * 64 + k], p, tmp[i]);
             }
             const int *Ec = E + (size_t)slot * npad + hb;
+            T *Co = Cb ? Cb + (size_t)jb.ch_cslot[c] * npad * 64 : nullptr;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                acc[i] *= tmp[i] + sA[(16 * pg + i) * 64 + j];
+                const double cv = tmp[i] + sA[(16 * pg + i) * 64 + j];
+                if (Co)
+                    Store<T>::put(Co + (size_t)(hb + i) * 64 + j, cv,
+                                  mix32((uint32_t)jb.ch_cslot[c] ^ 0x3C6EF372u, (uint32_t)(hb + i),
+                                        (uint32_t)j));
+                acc[i] *= cv;
                 esum[i] += Ec[i];
             }
         }
@@ -500,49 +520,30 @@ __device__ inline void stage_p(double *sP, const double *P, int tid)
     for (int q = tid; q < 4096; q += 256) sP[(q >> 6) * kLdsStride + (q & 63)] = P[q];
 }
 
-// Contribution of an interior child s to its parent for 16 patterns of this
-// wave: C(h, j) = sum_k P_{j,k} I_s(h, k) in the accumulator layout
-// (element i of column block b: pattern hw + g + 4i, state 16b + r).
-// sP must hold child s's P - I.
-template <typename T>
-__device__ inline void child_contrib(const double *sP, const T *Ls, int hw, int g, int r, d4 *C)
-{
-    double a[16];
-    Store<T>::load16(Ls + (size_t)(hw + r) * 64 + 16 * g, a);
-#pragma unroll
-    for (int b = 0; b < 4; b++) C[b] = d4{0, 0, 0, 0};
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) {
-        const double *prow = sP + (16 * g + kk) * kLdsStride + r;
-#pragma unroll
-        for (int b = 0; b < 4; b++) C[b] = mfma64(a[kk], prow[16 * b], C[b]);
-    }
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) C[b][i] += Store<T>::get(Ls + (size_t)(hw + g + 4 * i) * 64 + 16 * b + r);
-}
-
 // One (branch v, class or merged class pair, chunk of 64-pattern tiles) job
 // of the preorder adjoint sweep.  Per tile:
 //   X_t(h, j) = Abar_parent,t(h, j) * prod_{siblings s} C_s,t(h, j)   per term t
 //   X = sum_t X_t (exponents aligned), scaled by 2^ex per pattern
 //   Abar_v(h, k) = sum_j X(h, j) P_{j,k}                          (interior v)
 //   M_v[j][k] += sum_h X(h, j) 2^{ex + E_v} I_v(h, k)              (tips: code mask)
-// The M product is split over the 4 waves by its columns k (16 each): the A
-// operand X^T comes from LDS, the B operand I is read once per wave from
-// HBM/L2.  M accumulates over the chunk in registers and each chunk writes its
-// own partial (summed later in a fixed order).  LDS: 2 x 64 x 65 doubles, so
-// two blocks share a CU.
+// Interior siblings' contributions C_s = P_s I_s were kept by the forward
+// pass, tip siblings' are row gathers, so the only products here are the
+// child's adjoint (P_v staged in LDS once per block) and M.  M is split over
+// the 4 waves by its columns k (16 each): the A operand X^T comes from LDS,
+// the B operand I is read once per wave from HBM/L2.  M accumulates over the
+// chunk in registers and each chunk writes its own partial (summed later in
+// a fixed order).  LDS: 2 x 64 x 65 doubles, two blocks per CU.
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ jobs, int npad,
-                                                 const unsigned char *__restrict__ z,
-                                                 const double *__restrict__ PT,
-                                                 const double *__restrict__ AT, int namb,
-                                                 const double *__restrict__ mask64,
-                                                 const T *__restrict__ L, const int *__restrict__ E,
-                                                 T *__restrict__ A, int *__restrict__ EA,
-                                                 double *__restrict__ Mpart, int nchunk)
+                                                    const unsigned char *__restrict__ z,
+                                                    const double *__restrict__ PT,
+                                                    const double *__restrict__ AT, int namb,
+                                                    const double *__restrict__ mask64,
+                                                    const T *__restrict__ L,
+                                                    const int *__restrict__ E,
+                                                    const T *__restrict__ Cb,
+                                                    T *__restrict__ A, int *__restrict__ EA,
+                                                    double *__restrict__ Mpart, int nchunk)
 {
     __shared__ double sP[64 * kLdsStride];
     __shared__ double sX[64 * kLdsStride];
@@ -555,6 +556,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ j
     d4 Macc[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) Macc[b] = d4{0, 0, 0, 0};
+    if (!vtip) stage_p(sP, PT + (size_t)jb.v_pmat * 4096, tid);   // read after the first barrier
 
     for (int t = 0; t < kAdjTilesPerChunk; t++) {
         const int tile = chunk * kAdjTilesPerChunk + t;
@@ -575,9 +577,9 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ j
 #pragma unroll
             for (int i = 0; i < 4; i++) et[i] = EA[(size_t)asl * npad + hb + 4 * i];
             for (int q = 0; q < jb.nsib; q++) {
-                const int pm = jb.sib_pmat[term][q];
                 const int tip = jb.sib_tip[q];
                 if (tip >= 0) {
+                    const int pm = jb.sib_pmat[term][q];
                     const unsigned char *zt = z + (size_t)tip * npad;
                     const double *P = PT + (size_t)pm * 4096;
 #pragma unroll
@@ -588,14 +590,13 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ j
                             Xt[b][i] *= tip_value(P, AT, pm, namb, code, 16 * b + r);
                     }
                 } else {
-                    const int slot = jb.sib_slot[term][q];
-                    __syncthreads();
-                    stage_p(sP, PT + (size_t)pm * 4096, tid);
-                    __syncthreads();
-                    d4 C[4];
-                    child_contrib<T>(sP, L + (size_t)slot * npad * 64, hw, g, r, C);
+                    const T *Cs = Cb + (size_t)jb.sib_cslot[term][q] * npad * 64;
 #pragma unroll
-                    for (int b = 0; b < 4; b++) Xt[b] *= C[b];
+                    for (int b = 0; b < 4; b++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            Xt[b][i] *= Store<T>::get(Cs + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
+                    const int slot = jb.sib_slot[term][q];
 #pragma unroll
                     for (int i = 0; i < 4; i++) et[i] += E[(size_t)slot * npad + hb + 4 * i];
                 }
@@ -629,10 +630,9 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ j
                 const int hl = 16 * w + g + 4 * i;
                 sd[hl] = ex[i] + (vtip ? 0 : E[(size_t)jb.v_slot * npad + h0 + hl]);
             }
+        __syncthreads();
         if (!vtip) {
             // adjoint of the interior child: Abar_v(h, k) = sum_j X(h, j) P_{j,k}
-            stage_p(sP, PT + (size_t)jb.v_pmat * 4096, tid);
-            __syncthreads();
             double a[16];
 #pragma unroll
             for (int kk = 0; kk < 16; kk++) a[kk] = sX[(16 * w + r) * kLdsStride + 16 * g + kk];
@@ -668,8 +668,6 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ j
                                         (uint32_t)(16 * b + r)));
                 if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = ex[i] + e;
             }
-        } else {
-            __syncthreads();
         }
         // M columns 16w..16w+15: A = X^T (row j, k-dim = pattern) from LDS,
         // B = 2^d I (pattern x column) read from HBM / L2
@@ -876,18 +874,18 @@ void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njob
 template <typename T>
 void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
                   const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
-                  int *E)
+                  int *E, T *Cb)
 {
     if (njobs <= 0) return;
     if (kernel == 0 && tile == 128)
         hipLaunchKernelGGL((k_prune_mfma<T, 2>), dim3(npad / 128, njobs), dim3(256), 0, s, jobs,
-                           npad, z, PT, AT, namb, L, E);
+                           npad, z, PT, AT, namb, L, E, Cb);
     else if (kernel == 0)
         hipLaunchKernelGGL((k_prune_mfma<T, 1>), dim3(npad / 64, njobs), dim3(256), 0, s, jobs,
-                           npad, z, PT, AT, namb, L, E);
+                           npad, z, PT, AT, namb, L, E, Cb);
     else
         hipLaunchKernelGGL(k_prune_valu<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z,
-                           PT, AT, namb, L, E);
+                           PT, AT, namb, L, E, Cb);
 }
 
 template <typename T>
@@ -903,12 +901,12 @@ void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, con
 template <typename T>
 void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
                           const unsigned char *z, const double *PT, const double *AT, int namb,
-                          const double *mask64, const T *L, const int *E, T *A, int *EA,
-                          double *Mpart)
+                          const double *mask64, const T *L, const int *E, const T *Cb, T *A,
+                          int *EA, double *Mpart)
 {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(k_adjoint<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
-                       namb, mask64, L, E, A, EA, Mpart, nchunk);
+                       namb, mask64, L, E, Cb, A, EA, Mpart, nchunk);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchunk, double *M,
@@ -928,11 +926,11 @@ void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchun
 template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int, int,
                                            const unsigned char *, const double *, const double *,
                                            int, const double *, const double *, const int *,
-                                           double *, int *, double *);
+                                           const double *, double *, int *, double *);
 template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, int, int, int,
                                           const unsigned char *, const double *, const double *,
-                                          int, const double *, const float *, const int *, float *,
-                                          int *, double *);
+                                          int, const double *, const float *, const int *,
+                                          const float *, float *, int *, double *);
 
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,
                        const float *Af, const float *Bf, float *D32)
@@ -942,10 +940,10 @@ void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *
 
 template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, int, int,
                                    const unsigned char *, const double *, const double *, int,
-                                   double *, int *);
+                                   double *, int *, double *);
 template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, int, int,
                                   const unsigned char *, const double *, const double *, int,
-                                  float *, int *);
+                                  float *, int *, float *);
 template void launch_root<double>(hipStream_t, const RootArgs &, const double *, const int *,
                                   const double *, const double *, double *, double *, double *,
                                   int, double *, double *, int *);
