@@ -94,6 +94,7 @@ struct bsa_engine {
     bool adj_ready = false;
     std::vector<AdjJob> adj;              // grouped by depth of the child node
     std::vector<int> adj_off;             // depth d: adj[adj_off[d-1] .. adj_off[d])
+    std::vector<int> adj_nint;            // depth d: the first adj_nint[d-1] jobs have an interior child
     std::vector<EigJob> ejobs;
     std::vector<int> ejob_node;           // node of each EigJob
     std::vector<int> set_off, set_list;
@@ -470,13 +471,14 @@ static int run_device(bsa_engine *e, bool adjoint)
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev[3], s));
     if (adjoint) {
-        for (size_t d = 0; d + 1 < e->adj_off.size(); d++) {
-            int a = e->adj_off[d], b = e->adj_off[d + 1];
-            launch_adjoint_level<T>(s, e->d_adj + a, b - a, e->npad, e->nchunk, e->d_z,
+        for (size_t d = 0; d + 1 < e->adj_off.size(); d++)
+            launch_adjoint_level<T>(s, e->d_adj + e->adj_off[d], e->adj_nint[d], e->npad, e->d_z,
                                     (const double *)e->d_PT, (const double *)e->d_AT, e->namb,
-                                    e->d_mask64, (const T *)e->d_L, e->d_E, (const T *)e->d_C,
-                                    (T *)e->d_A, e->d_EA, e->d_Mpart);
-        }
+                                    e->d_E, (const T *)e->d_C, (T *)e->d_A, e->d_EA);
+        launch_adjoint_m<T>(s, e->d_adj, (int)e->adj.size(), e->npad, e->nchunk, e->d_z,
+                            (const double *)e->d_PT, (const double *)e->d_AT, e->namb, e->d_mask64,
+                            (const T *)e->d_L, e->d_E, (const T *)e->d_C, (const T *)e->d_A,
+                            e->d_EA, e->d_Mpart);
         HIPCHK(hipEventRecord(e->ev[4], s));
         launch_adjoint_finish(s, e->d_Mpart, e->nmrow, e->nchunk, e->d_M, e->d_ejobs,
                               (int)e->ejobs.size(), e->d_set_off, e->d_set_list, e->d_uvr, e->d_G,
@@ -503,7 +505,7 @@ static int run_device(bsa_engine *e, bool adjoint)
         e->timing[3] = ms;
         HIPCHK(hipEventElapsedTime(&ms, e->ev[4], e->ev[5]));
         e->timing[4] = ms;
-        e->n_adj_launch += (double)(e->adj_off.size() - 1);
+        e->n_adj_launch += (double)e->adj_off.size();
     }
     for (int i = 0; i < 5; i++) e->timing_sum[i] += e->timing[i];
     e->n_launch += (double)(e->level_off.size() - 1);
@@ -619,6 +621,7 @@ static int setup_adjoint(bsa_engine *e)
     };
     e->adj.clear();
     e->adj_off.assign(1, 0);
+    e->adj_nint.clear();
     e->ejobs.clear();
     e->ejob_node.clear();
     std::vector<int> lists[3];
@@ -647,8 +650,15 @@ static int setup_adjoint(bsa_engine *e)
         e->ejob_node.push_back(v);
     };
     for (int d = 1; d <= maxd; d++) {
-        for (int v = 0; v < nn; v++) {
-            if (depth[v] != d) continue;
+        // interior children first, so one launch per depth covers exactly them
+        std::vector<int> at_d;
+        for (int v = ns; v < nn; v++)
+            if (depth[v] == d) at_d.push_back(v);
+        for (int v = 0; v < ns; v++)
+            if (depth[v] == d) at_d.push_back(v);
+        size_t nint_jobs = 0;
+        for (int v : at_d) {
+            if (v >= ns) nint_jobs += merged(v) ? 2 : 4;
             const int a = e->father[v];
             if (merged(v)) {
                 for (int m = 0; m < 2; m++) {
@@ -689,6 +699,7 @@ static int setup_adjoint(bsa_engine *e)
             }
         }
         e->adj_off.push_back((int)e->adj.size());
+        e->adj_nint.push_back((int)nint_jobs);
     }
     e->set_off = {0};
     e->set_list.clear();

@@ -71,11 +71,18 @@ struct EigJob {
 
 constexpr int kAdjTilesPerChunk = 16;   // 64-pattern tiles per adjoint block
 
+// adjoints of one depth's interior children (jobs with v_tip < 0 only)
 template <typename T>
-void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad,
                           const unsigned char *z, const double *PT, const double *AT, int namb,
-                          const double *mask64, const T *L, const int *E, const T *Cb, T *A,
-                          int *EA, double *Mpart);
+                          const int *E, const T *Cb, T *A, int *EA);
+
+// branch-matrix chunk partials of all jobs (after the sweep)
+template <typename T>
+void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+                      const unsigned char *z, const double *PT, const double *AT, int namb,
+                      const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
+                      const int *EA, double *Mpart);
 
 // sums the chunk partials of nm branch matrices, then for every EigJob
 // forms A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set

@@ -514,185 +514,181 @@ __device__ inline double pow2i(int d)
     return ldexp(1.0, d);
 }
 
-// Stage P - I of matrix pm into LDS with row stride 65 (row k = child state).
-__device__ inline void stage_p(double *sP, const double *P, int tid)
+// Value of X_t(h, j) = Abar_parent,t(h, j) * prod_{siblings} C_s,t(h, j) for 16
+// consecutive states j0..j0+15 of pattern h (one lane), and its exponent.
+template <typename T>
+__device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npad,
+                              const unsigned char *z, const double *PT, const double *AT, int namb,
+                              const int *E, const T *Cb, const T *A, const int *EA, double *x)
 {
-    for (int q = tid; q < 4096; q += 256) sP[(q >> 6) * kLdsStride + (q & 63)] = P[q];
+    const int asl = jb.a_aslot[term];
+    Store<T>::load16(A + ((size_t)asl * npad + h) * 64 + j0, x);
+    int ex = EA[(size_t)asl * npad + h];
+    for (int q = 0; q < jb.nsib; q++) {
+        const int tip = jb.sib_tip[q];
+        if (tip >= 0) {
+            const int pm = jb.sib_pmat[term][q];
+            const int code = z[(size_t)tip * npad + h];
+            const double *P = PT + (size_t)pm * 4096;
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) x[kk] *= tip_value(P, AT, pm, namb, code, j0 + kk);
+        } else {
+            double c[16];
+            Store<T>::load16(Cb + ((size_t)jb.sib_cslot[term][q] * npad + h) * 64 + j0, c);
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) x[kk] *= c[kk];
+            ex += E[(size_t)jb.sib_slot[term][q] * npad + h];
+        }
+    }
+    return ex;
 }
 
-// One (branch v, class or merged class pair, chunk of 64-pattern tiles) job
-// of the preorder adjoint sweep.  Per tile:
-//   X_t(h, j) = Abar_parent,t(h, j) * prod_{siblings s} C_s,t(h, j)   per term t
-//   X = sum_t X_t (exponents aligned), scaled by 2^ex per pattern
-//   Abar_v(h, k) = sum_j X(h, j) P_{j,k}                          (interior v)
-//   M_v[j][k] += sum_h X(h, j) 2^{ex + E_v} I_v(h, k)              (tips: code mask)
-// Interior siblings' contributions C_s = P_s I_s were kept by the forward
-// pass, tip siblings' are row gathers, so the only products here are the
-// child's adjoint (P_v staged in LDS once per block) and M.  M is split over
-// the 4 waves by its columns k (16 each): the A operand X^T comes from LDS,
-// the B operand I is read once per wave from HBM/L2.  M accumulates over the
-// chunk in registers and each chunk writes its own partial (summed later in
-// a fixed order).  LDS: 2 x 64 x 65 doubles, two blocks per CU.
+// X(h, j0..j0+15) = sum over the job's terms of X_t, exponents aligned to the
+// largest; returns the exponent.
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_adjoint(const AdjJob *__restrict__ jobs, int npad,
-                                                    const unsigned char *__restrict__ z,
-                                                    const double *__restrict__ PT,
-                                                    const double *__restrict__ AT, int namb,
-                                                    const double *__restrict__ mask64,
-                                                    const T *__restrict__ L,
-                                                    const int *__restrict__ E,
-                                                    const T *__restrict__ Cb,
-                                                    T *__restrict__ A, int *__restrict__ EA,
-                                                    double *__restrict__ Mpart, int nchunk)
+__device__ inline int x_sum16(const AdjJob &jb, int h, int j0, int npad, const unsigned char *z,
+                              const double *PT, const double *AT, int namb, const int *E,
+                              const T *Cb, const T *A, const int *EA, double *x)
+{
+    int ex = x_row16<T>(jb, 0, h, j0, npad, z, PT, AT, namb, E, Cb, A, EA, x);
+    for (int term = 1; term < jb.nterm; term++) {
+        double t[16];
+        const int et = x_row16<T>(jb, term, h, j0, npad, z, PT, AT, namb, E, Cb, A, EA, t);
+        const int em = ex > et ? ex : et;
+        const double f0 = pow2i(ex - em), f1 = pow2i(et - em);
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) x[kk] = x[kk] * f0 + t[kk] * f1;
+        ex = em;
+    }
+    return ex;
+}
+
+// Adjoint of the interior children of one tree depth, per (job, 64-pattern
+// tile): Abar_v(h, k) = sum_j X(h, j) P_{j,k}.  Shaped like the pruning
+// kernel: wave w owns patterns 16w..16w+15, each lane forms the 16 X values it
+// feeds to the MFMA (pattern 16w + r, states 16g..16g+15) straight from
+// HBM/L2, and P_v (identity included) is staged in LDS once.
+template <typename T>
+__global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ jobs, int npad,
+                                                  const unsigned char *__restrict__ z,
+                                                  const double *__restrict__ PT,
+                                                  const double *__restrict__ AT, int namb,
+                                                  const int *__restrict__ E,
+                                                  const T *__restrict__ Cb, T *__restrict__ A,
+                                                  int *__restrict__ EA)
 {
     __shared__ double sP[64 * kLdsStride];
+    const AdjJob &jb = jobs[blockIdx.y];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    const int hw = blockIdx.x * 64 + 16 * w;
+    {
+        const double *P = PT + (size_t)jb.v_pmat * 4096;
+        for (int q = tid; q < 4096; q += 256) {
+            const int k = q >> 6, j = q & 63;
+            sP[k * kLdsStride + j] = P[q] + (k == j ? 1.0 : 0.0);
+        }
+    }
+    double x[16];
+    const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, PT, AT, namb, E, Cb, A, EA, x);
+    __syncthreads();
+    d4 Y[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) Y[b] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            Y[b] = mfma64(x[kk], sP[(16 * b + r) * kLdsStride + 16 * g + kk], Y[b]);
+    T *Av = A + (size_t)jb.v_aslot * npad * 64;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        // output row g + 4i of the wave; its X exponent sits in lane g + 4i
+        const int exr = __shfl(ex, g + 4 * i, 64);
+        double m = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) m = Y[b][i] > m ? Y[b][i] : m;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            double o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        int e = 0;
+        if (m > 0) frexp(m, &e);
+        const int h = hw + g + 4 * i;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            Store<T>::put(Av + (size_t)h * 64 + 16 * b + r, ldexp(Y[b][i], -e),
+                          mix32((uint32_t)jb.v_aslot ^ 0xA5A5A5A5u, (uint32_t)h,
+                                (uint32_t)(16 * b + r)));
+        if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = exr + e;
+    }
+}
+
+// Branch matrices of every adjoint job, one launch after the sweep:
+// M_v[j][k] += sum_h X(h, j) 2^{ex(h) + E_v(h)} I_v(h, k) over a chunk of tiles.
+// Per tile the block forms the scaled X tile in LDS (each lane one pattern and
+// 16 states, as in the child-adjoint kernel); wave w then owns columns
+// 16w..16w+15 of M and loads its B operand (I_v) straight from HBM/L2.  The
+// waves' column strips are disjoint, so each chunk writes its partial with no
+// further reduction (chunk partials are summed later in a fixed order).
+template <typename T>
+__global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs, int npad,
+                                                  const unsigned char *__restrict__ z,
+                                                  const double *__restrict__ PT,
+                                                  const double *__restrict__ AT, int namb,
+                                                  const double *__restrict__ mask64,
+                                                  const T *__restrict__ L,
+                                                  const int *__restrict__ E,
+                                                  const T *__restrict__ Cb,
+                                                  const T *__restrict__ A,
+                                                  const int *__restrict__ EA,
+                                                  double *__restrict__ Mpart, int nchunk)
+{
     __shared__ double sX[64 * kLdsStride];
-    __shared__ int sd[64];
     const AdjJob &jb = jobs[blockIdx.y];
     const int chunk = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
     const int ntile = npad / 64;
     const bool vtip = jb.v_tip >= 0;
+    const int col = 16 * w + r;
     d4 Macc[4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) Macc[b] = d4{0, 0, 0, 0};
-    if (!vtip) stage_p(sP, PT + (size_t)jb.v_pmat * 4096, tid);   // read after the first barrier
-
+    for (int a = 0; a < 4; a++) Macc[a] = d4{0, 0, 0, 0};
     for (int t = 0; t < kAdjTilesPerChunk; t++) {
         const int tile = chunk * kAdjTilesPerChunk + t;
         if (tile >= ntile) break;
-        const int h0 = tile * 64, hw = h0 + 16 * w, hb = hw + g;
-        d4 X[4];
-        int ex[4];
-        for (int term = 0; term < jb.nterm; term++) {
-            d4 Xt[4];
-            int et[4];
-            const int asl = jb.a_aslot[term];
-            const T *Ap = A + (size_t)asl * npad * 64;
+        const int h0 = tile * 64, hx = h0 + 16 * w + r;
+        double x[16];
+        int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, PT, AT, namb, E, Cb, A, EA, x);
+        if (!vtip) ex += E[(size_t)jb.v_slot * npad + hx];
+        const double sc = pow2i(ex);
+        __syncthreads();   // the previous tile's reads of sX are done
 #pragma unroll
-            for (int b = 0; b < 4; b++)
+        for (int kk = 0; kk < 16; kk++) sX[(16 * w + r) * kLdsStride + 16 * g + kk] = x[kk] * sc;
+        double bv[16];
+        if (vtip) {
 #pragma unroll
-                for (int i = 0; i < 4; i++)
-                    Xt[b][i] = Store<T>::get(Ap + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
-#pragma unroll
-            for (int i = 0; i < 4; i++) et[i] = EA[(size_t)asl * npad + hb + 4 * i];
-            for (int q = 0; q < jb.nsib; q++) {
-                const int tip = jb.sib_tip[q];
-                if (tip >= 0) {
-                    const int pm = jb.sib_pmat[term][q];
-                    const unsigned char *zt = z + (size_t)tip * npad;
-                    const double *P = PT + (size_t)pm * 4096;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int code = zt[hb + 4 * i];
-#pragma unroll
-                        for (int b = 0; b < 4; b++)
-                            Xt[b][i] *= tip_value(P, AT, pm, namb, code, 16 * b + r);
-                    }
-                } else {
-                    const T *Cs = Cb + (size_t)jb.sib_cslot[term][q] * npad * 64;
-#pragma unroll
-                    for (int b = 0; b < 4; b++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++)
-                            Xt[b][i] *= Store<T>::get(Cs + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
-                    const int slot = jb.sib_slot[term][q];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) et[i] += E[(size_t)slot * npad + hb + 4 * i];
-                }
+            for (int kk = 0; kk < 16; kk++) {
+                const int code = z[(size_t)jb.v_tip * npad + h0 + 4 * kk + g];
+                bv[kk] = code < 61 ? (code == col ? 1.0 : 0.0) : mask64[code * 64 + col];
             }
-            if (term == 0) {
+        } else {
+            const T *Lv = L + ((size_t)jb.v_slot * npad + h0 + g) * 64 + col;
 #pragma unroll
-                for (int b = 0; b < 4; b++) X[b] = Xt[b];
-#pragma unroll
-                for (int i = 0; i < 4; i++) ex[i] = et[i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int em = ex[i] > et[i] ? ex[i] : et[i];
-                    const double f0 = pow2i(ex[i] - em), f1 = pow2i(et[i] - em);
-#pragma unroll
-                    for (int b = 0; b < 4; b++) X[b][i] = X[b][i] * f0 + Xt[b][i] * f1;
-                    ex[i] = em;
-                }
-            }
+            for (int kk = 0; kk < 16; kk++) bv[kk] = Store<T>::get(Lv + (size_t)(4 * kk) * 64);
         }
-        // X into LDS (row = pattern in tile)
         __syncthreads();
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+        for (int kk = 0; kk < 16; kk++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) sX[(16 * w + g + 4 * i) * kLdsStride + 16 * b + r] = X[b][i];
-        // exponent of each pattern's M contribution: ex + E_v
-        if (r == 0)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int hl = 16 * w + g + 4 * i;
-                sd[hl] = ex[i] + (vtip ? 0 : E[(size_t)jb.v_slot * npad + h0 + hl]);
-            }
-        __syncthreads();
-        if (!vtip) {
-            // adjoint of the interior child: Abar_v(h, k) = sum_j X(h, j) P_{j,k}
-            double a[16];
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) a[kk] = sX[(16 * w + r) * kLdsStride + 16 * g + kk];
-            d4 Y[4];
-#pragma unroll
-            for (int b = 0; b < 4; b++) Y[b] = d4{0, 0, 0, 0};
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++)
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    Y[b] = mfma64(a[kk], sP[(16 * b + r) * kLdsStride + 16 * g + kk], Y[b]);
-            T *Av = A + (size_t)jb.v_aslot * npad * 64;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                double m = 0;
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    Y[b][i] += X[b][i];      // identity part of P
-                    m = Y[b][i] > m ? Y[b][i] : m;
-                }
-#pragma unroll
-                for (int off = 1; off < 16; off <<= 1) {
-                    double o = __shfl_xor(m, off, 64);
-                    m = o > m ? o : m;
-                }
-                int e = 0;
-                if (m > 0) frexp(m, &e);
-                const int h = hb + 4 * i;
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    Store<T>::put(Av + (size_t)h * 64 + 16 * b + r, ldexp(Y[b][i], -e),
-                                  mix32((uint32_t)jb.v_aslot ^ 0xA5A5A5A5u, (uint32_t)h,
-                                        (uint32_t)(16 * b + r)));
-                if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = ex[i] + e;
-            }
-        }
-        // M columns 16w..16w+15: A = X^T (row j, k-dim = pattern) from LDS,
-        // B = 2^d I (pattern x column) read from HBM / L2
-        const int col = 16 * w + r;
-#pragma unroll 4
-        for (int kk = 0; kk < 16; kk++) {
-            const int hl = 4 * kk + g;
-            double bv;
-            if (vtip) {
-                const int code = z[(size_t)jb.v_tip * npad + h0 + hl];
-                bv = code < 61 ? (code == col ? 1.0 : 0.0) : mask64[code * 64 + col];
-            } else {
-                bv = Store<T>::get(L + ((size_t)jb.v_slot * npad + h0 + hl) * 64 + col);
-            }
-            bv *= pow2i(sd[hl]);
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-                Macc[b] = mfma64(sX[hl * kLdsStride + 16 * b + r], bv, Macc[b]);
-        }
+            for (int a = 0; a < 4; a++)
+                Macc[a] = mfma64(sX[(4 * kk + g) * kLdsStride + 16 * a + r], bv[kk], Macc[a]);
     }
     double *Mo = Mpart + ((size_t)jb.m_index * nchunk + chunk) * 4096;
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) Mo[(16 * b + g + 4 * i) * 64 + 16 * w + r] = Macc[b][i];
+        for (int i = 0; i < 4; i++) Mo[(16 * a + g + 4 * i) * 64 + col] = Macc[a][i];
 }
 
 // Deterministic sum of the chunk partials: M[m] = sum_c Mpart[m][c].
@@ -899,13 +895,23 @@ void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, con
 }
 
 template <typename T>
-void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad,
                           const unsigned char *z, const double *PT, const double *AT, int namb,
-                          const double *mask64, const T *L, const int *E, const T *Cb, T *A,
-                          int *EA, double *Mpart)
+                          const int *E, const T *Cb, T *A, int *EA)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_adjoint<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
+    hipLaunchKernelGGL(k_adj_a<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
+                       namb, E, Cb, A, EA);
+}
+
+template <typename T>
+void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+                      const unsigned char *z, const double *PT, const double *AT, int namb,
+                      const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
+                      const int *EA, double *Mpart)
+{
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
                        namb, mask64, L, E, Cb, A, EA, Mpart, nchunk);
 }
 
@@ -923,14 +929,20 @@ void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchun
     hipLaunchKernelGGL(k_zform, dim3(3), dim3(256), 0, s, (const double *)M, uvr, Z);
 }
 
-template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int, int,
+template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int,
                                            const unsigned char *, const double *, const double *,
-                                           int, const double *, const double *, const int *,
-                                           const double *, double *, int *, double *);
-template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, int, int, int,
+                                           int, const int *, const double *, double *, int *);
+template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, int, int,
                                           const unsigned char *, const double *, const double *,
-                                          int, const double *, const float *, const int *,
-                                          const float *, float *, int *, double *);
+                                          int, const int *, const float *, float *, int *);
+template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, int, int, int,
+                                       const unsigned char *, const double *, const double *, int,
+                                       const double *, const double *, const int *, const double *,
+                                       const double *, const int *, double *);
+template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, int, int, int,
+                                      const unsigned char *, const double *, const double *, int,
+                                      const double *, const float *, const int *, const float *,
+                                      const float *, const int *, double *);
 
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,
                        const float *Af, const float *Bf, float *D32)
