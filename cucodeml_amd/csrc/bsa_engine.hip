@@ -106,7 +106,7 @@ struct bsa_engine {
     int *d_set_off = nullptr, *d_set_list = nullptr;
     void *d_A = nullptr, *d_C = nullptr;
     int *d_EA = nullptr;
-    double *d_Mpart = nullptr, *d_M = nullptr, *d_G = nullptr, *d_dtau = nullptr, *d_Z = nullptr;
+    double *d_Mpart = nullptr, *d_W = nullptr, *d_G = nullptr, *d_dtau = nullptr, *d_Z = nullptr;
     double *d_mask64 = nullptr;
     double *h_adjout = nullptr;           // dtau[nej] then Z[3*4096]
     std::vector<unsigned char> code_mask; // ncodes x 61
@@ -121,7 +121,7 @@ static void free_all(bsa_engine *e)
     void *dev_ptrs[] = {e->d_uvr, e->d_pjobs, e->d_prune, e->d_PT, e->d_AT, e->d_L, e->d_E,
                         e->d_z, e->d_mask_amb, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
                         e->d_partial, e->d_out, e->d_adj, e->d_ejobs, e->d_set_off,
-                        e->d_set_list, e->d_A, e->d_C, e->d_EA, e->d_Mpart, e->d_M, e->d_G,
+                        e->d_set_list, e->d_A, e->d_C, e->d_EA, e->d_Mpart, e->d_W, e->d_G,
                         e->d_dtau, e->d_Z, e->d_mask64};
     for (void *p : dev_ptrs)
         if (p) hipFree(p);
@@ -480,9 +480,9 @@ static int run_device(bsa_engine *e, bool adjoint)
                             (const T *)e->d_L, e->d_E, (const T *)e->d_C, (const T *)e->d_A,
                             e->d_EA, e->d_Mpart);
         HIPCHK(hipEventRecord(e->ev[4], s));
-        launch_adjoint_finish(s, e->d_Mpart, e->nmrow, e->nchunk, e->d_M, e->d_ejobs,
-                              (int)e->ejobs.size(), e->d_set_off, e->d_set_list, e->d_uvr, e->d_G,
-                              e->d_dtau, e->d_Z);
+        launch_adjoint_finish(s, e->d_Mpart, e->nchunk, e->d_ejobs, (int)e->ejobs.size(),
+                              e->d_set_off, e->d_set_list, e->d_uvr, e->d_G, e->d_dtau, e->d_W,
+                              e->d_Z);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(e->ev[5], s));
         HIPCHK(hipMemcpyAsync(e->h_adjout, e->d_dtau, sizeof(double) * e->ejobs.size(),
@@ -667,7 +667,7 @@ static int setup_adjoint(bsa_engine *e)
                     j.v_slot = v < ns ? -1 : e->slot[v][m];
                     j.v_pmat = v * 3 + uvr_set(m, 0);
                     j.v_aslot = v < ns ? -1 : aslot(v, m);
-                    j.m_index = v * 4 + m;
+                    j.m_index = (int)e->adj.size();
                     if (merged(a)) {
                         j.nterm = 1;
                         j.a_aslot[0] = aslot(a, m);
@@ -689,7 +689,7 @@ static int setup_adjoint(bsa_engine *e)
                     j.v_slot = v < ns ? -1 : e->slot[v][c];
                     j.v_pmat = v * 3 + uvr_set(c, e->label[v]);
                     j.v_aslot = v < ns ? -1 : aslot(v, c);
-                    j.m_index = v * 4 + c;
+                    j.m_index = (int)e->adj.size();
                     j.nterm = 1;
                     j.a_aslot[0] = aslot(a, c);   // the parent of an unmerged node is unmerged
                     fill_sibs(j, 0, v, a, c);
@@ -707,7 +707,7 @@ static int setup_adjoint(bsa_engine *e)
         e->set_list.insert(e->set_list.end(), lists[s].begin(), lists[s].end());
         e->set_off.push_back((int)e->set_list.size());
     }
-    e->nmrow = nn * 4;
+    e->nmrow = (int)e->adj.size();   // one branch matrix per job
     const int ntile = e->npad / 64;
     e->nchunk = (ntile + kAdjTilesPerChunk - 1) / kAdjTilesPerChunk;
     const size_t ts = tsize(e), nej = e->ejobs.size();
@@ -727,7 +727,7 @@ static int setup_adjoint(bsa_engine *e)
     HIPCHK(hipMalloc(&e->d_EA, sizeof(int) * (size_t)e->npad * nasl));
     HIPCHK(hipMalloc(&e->d_Mpart, sizeof(double) * 4096 * (size_t)e->nmrow * e->nchunk));
     HIPCHK(hipMemset(e->d_Mpart, 0, sizeof(double) * 4096 * (size_t)e->nmrow * e->nchunk));
-    HIPCHK(hipMalloc(&e->d_M, sizeof(double) * 4096 * (size_t)e->nmrow));
+    HIPCHK(hipMalloc(&e->d_W, sizeof(double) * 4096 * 3 * kWParts));
     HIPCHK(hipMalloc(&e->d_G, sizeof(double) * 4096 * nej));
     HIPCHK(hipMalloc(&e->d_dtau, sizeof(double) * nej));
     HIPCHK(hipMalloc(&e->d_Z, sizeof(double) * 3 * 4096));

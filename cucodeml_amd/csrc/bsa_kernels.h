@@ -70,6 +70,7 @@ struct EigJob {
 };
 
 constexpr int kAdjTilesPerChunk = 16;   // 64-pattern tiles per adjoint block
+constexpr int kWParts = 8;              // partial sums per eigen set in the W reduction
 
 // adjoints of one depth's interior children (jobs with v_tip < 0 only)
 template <typename T>
@@ -84,13 +85,13 @@ void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, in
                       const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
                       const int *EA, double *Mpart);
 
-// sums the chunk partials of nm branch matrices, then for every EigJob
-// forms A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set
-// W = sum G and Z = V^T W U^T.  Outputs dtau[nm] and Z[3][64*64].
-void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchunk, double *M,
-                           const EigJob *ejobs, int nej, const int *set_off,
-                           const int *set_list, const double *uvr, double *G, double *dtau,
-                           double *Z);
+// for every EigJob sums the chunk partials of its branch matrix M, forms
+// A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set
+// W = sum G (kWParts partials in Wpart[kWParts][3][64*64]) and
+// Z = V^T W U^T.  Outputs dtau[nej] and Z[3][64*64].
+void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
+                           int nej, const int *set_off, const int *set_list, const double *uvr,
+                           double *G, double *dtau, double *Wpart, double *Z);
 
 // launchers; T = storage type of the conditional likelihoods (double or float)
 void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,

@@ -176,14 +176,14 @@ __device__ inline d4 mfma64(double a, double b, d4 c)
 // states; wave w owns 16-pattern row groups 64 rg + 16 w (rg < RG), so the P
 // tile staged in LDS for a child serves 64*RG patterns.  Interior children
 // are (16 x 64) x (64 x 64) products per row group: 16 k-steps x 4 column
-// blocks of the FP64 16x16x4 MFMA with B = P - I from LDS; the child's own
-// values (the identity part) are added after the accumulation.  The k order
+// blocks of the FP64 16x16x4 MFMA with B = P from LDS (P - I as computed,
+// identity added on staging).  The k order
 // is free, so lane group g = lane/16 takes child states 16g..16g+15 at steps
 // 0..15 and each lane reads 16 consecutive values of its pattern row straight
 // from HBM.  Accumulator element i of column block b of lane l is pattern
 // (row group base) + l/16 + 4i, parent state 16b + l%16.
 template <typename T, int RG>
-__global__ __launch_bounds__(256) void k_prune_mfma(const PruneJob *__restrict__ jobs, int npad,
+__global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const PruneJob *__restrict__ jobs, int npad,
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
                                                     const double *__restrict__ AT, int namb,
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_prune_mfma(const PruneJob *__restrict__
             const T *Lc = L + (size_t)slot * npad * 64;
             const int *Ec = E + (size_t)slot * npad;
             __syncthreads();
-            for (int q = tid; q < 4096; q += 256) sP[q] = P[q];
+            for (int q = tid; q < 4096; q += 256) sP[q] = P[q] + ((q >> 6) == (q & 63) ? 1.0 : 0.0);
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < RG; q++) {
@@ -244,11 +244,6 @@ __global__ __launch_bounds__(256) void k_prune_mfma(const PruneJob *__restrict__
 #pragma unroll
                     for (int b = 0; b < 4; b++) tmp[b] = mfma64(a[kk], prow[16 * b], tmp[b]);
                 }
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        tmp[b][i] += Store<T>::get(Lc + (size_t)(hb + 4 * i) * 64 + 16 * b + r);
                 if (Cb) {
                     const int cs = jb.ch_cslot[c];
                     T *Co = Cb + (size_t)cs * npad * 64;
@@ -691,25 +686,14 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
         for (int i = 0; i < 4; i++) Mo[(16 * a + g + 4 * i) * 64 + col] = Macc[a][i];
 }
 
-// Deterministic sum of the chunk partials: M[m] = sum_c Mpart[m][c].
-__global__ __launch_bounds__(256) void k_mreduce(const double *__restrict__ Mpart, int nchunk,
-                                                 double *__restrict__ M)
-{
-    const int m = blockIdx.x;
-    for (int q = threadIdx.x; q < 4096; q += 256) {
-        double s = 0;
-        for (int c = 0; c < nchunk; c++) s += Mpart[((size_t)m * nchunk + c) * 4096 + q];
-        M[(size_t)m * 4096 + q] = s;
-    }
-}
-
-// Per (branch, class): A = U^T M V^T in the eigenbasis of its set;
+// Per (branch, class): M = sum of its chunk partials (fixed order), then
+// A = U^T M V^T in the eigenbasis of its set;
 // d lnL / d tau = sum_a A_aa lambda_a e^{lambda_a tau} and
 // G = A o X(tau), X_ab = (e^{lambda_a tau} - e^{lambda_b tau}) / (lambda_a - lambda_b)
 // (tau e^{lambda tau} on ties), so that for any rate-matrix parameter theta
 // d lnL / d theta = <G, V dQ/dtheta U> (derivative of U e^{Lambda tau} V).
 __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jobs,
-                                                   const double *__restrict__ M,
+                                                   const double *__restrict__ Mpart, int nchunk,
                                                    const double *__restrict__ uvr,
                                                    double *__restrict__ G, double *__restrict__ dtau)
 {
@@ -722,8 +706,11 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     const double *V = U + 4096;
     const double *R = U + 8192;
     const int tid = threadIdx.x;
+    const double *Mp = Mpart + (size_t)jb.m_index * nchunk * 4096;
     for (int q = tid; q < 4096; q += 256) {
-        sM[(q >> 6) * kLdsStride + (q & 63)] = M[(size_t)jb.m_index * 4096 + q];
+        double m = 0;
+        for (int c = 0; c < nchunk; c++) m += Mp[(size_t)c * 4096 + q];
+        sM[(q >> 6) * kLdsStride + (q & 63)] = m;
         sU[(q >> 6) * kLdsStride + (q & 63)] = U[q];
     }
     if (tid < 64) sE[tid] = exp(R[tid] * jb.tau);
@@ -780,21 +767,23 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     if (tid == 0) dtau[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
 }
 
-// Per eigen set: W = sum over its (branch, class) jobs of G, in a fixed
-// order; one thread per matrix element, 16 blocks per set.
+// Per eigen set: partial sums of G over its (branch, class) jobs; part p
+// takes list entries p, p + kWParts, ... in order (one thread per element).
 __global__ __launch_bounds__(256) void k_wreduce(const double *__restrict__ G,
                                                  const int *__restrict__ set_off,
                                                  const int *__restrict__ set_list,
-                                                 double *__restrict__ W)
+                                                 double *__restrict__ Wpart)
 {
-    const int set = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x;
+    const int set = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x, p = blockIdx.z;
     double s = 0;
-    for (int i = set_off[set]; i < set_off[set + 1]; i++) s += G[(size_t)set_list[i] * 4096 + q];
-    W[(size_t)set * 4096 + q] = s;
+    for (int i = set_off[set] + p; i < set_off[set + 1]; i += kWParts)
+        s += G[(size_t)set_list[i] * 4096 + q];
+    Wpart[((size_t)p * 3 + set) * 4096 + q] = s;
 }
 
-// Per eigen set: Z = V^T W U^T (so that <W, V dQ U> = <Z, dQ>).
-__global__ __launch_bounds__(256) void k_zform(const double *__restrict__ W,
+// Per eigen set: W = sum of its parts (fixed order), Z = V^T W U^T (so that
+// <W, V dQ U> = <Z, dQ>).
+__global__ __launch_bounds__(256) void k_zform(const double *__restrict__ Wpart,
                                                const double *__restrict__ uvr,
                                                double *__restrict__ Z)
 {
@@ -804,7 +793,9 @@ __global__ __launch_bounds__(256) void k_zform(const double *__restrict__ W,
     const double *U = uvr + (size_t)set * (64 * 64 * 2 + 64);
     const double *V = U + 4096;
     for (int q = tid; q < 4096; q += 256) {
-        sW[(q >> 6) * kLdsStride + (q & 63)] = W[(size_t)set * 4096 + q];
+        double w = 0;
+        for (int p = 0; p < kWParts; p++) w += Wpart[((size_t)p * 3 + set) * 4096 + q];
+        sW[(q >> 6) * kLdsStride + (q & 63)] = w;
         sB[(q >> 6) * kLdsStride + (q & 63)] = V[q];
     }
     __syncthreads();
@@ -915,18 +906,14 @@ void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, in
                        namb, mask64, L, E, Cb, A, EA, Mpart, nchunk);
 }
 
-void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nm, int nchunk, double *M,
-                           const EigJob *ejobs, int nej, const int *set_off,
-                           const int *set_list, const double *uvr, double *G, double *dtau,
-                           double *Z)
+void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
+                           int nej, const int *set_off, const int *set_list, const double *uvr,
+                           double *G, double *dtau, double *Wpart, double *Z)
 {
-    hipLaunchKernelGGL(k_mreduce, dim3(nm), dim3(256), 0, s, Mpart, nchunk, M);
-    hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, (const double *)M, uvr, G,
-                       dtau);
-    // W reuses the first three matrices of M (no longer needed)
-    hipLaunchKernelGGL(k_wreduce, dim3(3, 16), dim3(256), 0, s, (const double *)G, set_off, set_list,
-                       M);
-    hipLaunchKernelGGL(k_zform, dim3(3), dim3(256), 0, s, (const double *)M, uvr, Z);
+    hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, Mpart, nchunk, uvr, G, dtau);
+    hipLaunchKernelGGL(k_wreduce, dim3(3, 16, kWParts), dim3(256), 0, s, (const double *)G, set_off,
+                       set_list, Wpart);
+    hipLaunchKernelGGL(k_zform, dim3(3), dim3(256), 0, s, (const double *)Wpart, uvr, Z);
 }
 
 template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int,
