@@ -111,15 +111,39 @@ def cpu_baseline(tree, seqs, p, ncodon_full, npatt_full, np_full, sample_codons)
                        f"evaluations per forward-difference gradient")}
 
 
-def roofline(work_bytes, work_flops, ms_sum, nev, launches, kernel, hbm=True):
+def roofline(work_bytes, work_flops, ms_sum, nev, launches, kernel):
+    """Roofline entry of one kernel from its algorithmic work per evaluation
+    and its summed HIP-event time: bound = the larger of the HBM time
+    (bytes / 8 TB/s) and the FP64 matrix-core time (flops / 78.6 TFLOP/s)."""
     sec = ms_sum / 1e3 / nev
     gbs = work_bytes / sec / 1e9
     tf = work_flops / sec / 1e12
-    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+    mfma = work_flops / (FP64_PEAK_TFLOPS * 1e12) > work_bytes / (HBM_PEAK_GBS * 1e9)
+    ach, peak, unit = (tf, FP64_PEAK_TFLOPS, "TFLOP/s") if mfma else (gbs, HBM_PEAK_GBS, "GB/s")
+    return {"bound": "mfma" if mfma else "hbm", "achieved": ach, "peak": peak, "unit": unit,
+            "frac": ach / peak, "traffic": None, "kernel": kernel,
             "bytes_per_eval": work_bytes, "flops_per_eval": work_flops,
             "avg_launch_us": ms_sum * 1e3 / max(launches, 1), "launches_per_eval": launches / nev,
+            "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
             "fp64_tflops": tf, "fp64_frac": tf / FP64_PEAK_TFLOPS}
+
+
+def genome_scan(ngenes, workers, precision):
+    """genes/hour of the genome scan (H1 + H0 fits and LRT per gene) on a
+    sample of the 2,000-gene workload."""
+    from cucodeml_amd import scan
+    genes = scan.simulate_genes(ngenes)
+    scan.scan(genes[:min(2, ngenes)], workers=1, precision=precision)   # warm-up
+    res, wall = scan.scan(genes, workers=workers, precision=precision)
+    return {"value": ngenes / wall * 3600.0, "unit": "genes/hour",
+            "sample": (f"first {ngenes} genes of the seeded 2,000-gene scan workload (Yule "
+                       f"trees of {scan.SCAN_TAXA[0]}-{scan.SCAN_TAXA[1]} taxa, lognormal codon "
+                       f"counts median {scan.SCAN_MEDIAN}, sigma {scan.SCAN_SIGMA}, clipped to "
+                       f"{list(scan.SCAN_CLIP)})"),
+            "per_gene": "H1 fit + H0 fit (bounded BFGS, adjoint gradient) + LRT",
+            "precision": precision, "workers": workers, "wall_s": wall,
+            "converged": int(sum(r["converged"] for r in res)),
+            "mean_evaluations": float(np.mean([r["evaluations"] for r in res]))}
 
 
 def main():
@@ -138,6 +162,9 @@ def main():
                     help="steps of the FP64 forward-difference path timed beside (0: skip)")
     ap.add_argument("--cpu-sample-codons", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scan-genes", type=int, default=32,
+                    help="genes of the genome-scan sample timed beside (0: skip)")
+    ap.add_argument("--scan-workers", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding over site patterns "
@@ -164,13 +191,16 @@ def main():
     sto = "double" if args.precision == "fp64" else "float"
     prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
                         f"k_prune_{args.kernel}<{sto}> (FP64 arithmetic, {sto} storage)")
-    rl = prune_rl
-    adj_rl = None
+    rls = [(tm["sum_prune"], prune_rl)]
+    adj_rl = br_rl = None
     if args.gradient == "adjoint":
-        adj_rl = roofline(work[7], work[6], tm["sum_adjoint"], nev, tm["n_adj_launch"],
-                          f"k_adjoint<{sto}> (FP64 arithmetic, {sto} storage)")
-        if tm["sum_adjoint"] > tm["sum_prune"]:
-            rl = adj_rl
+        nlev = tm["n_adj_launch"] - nev   # child-adjoint launches (one branch-matrix launch each)
+        adj_rl = roofline(work[7], work[6], tm["sum_adjoint"], nev, nlev,
+                          f"k_adj_a<{sto}> (child adjoints, FP64 arithmetic)")
+        br_rl = roofline(work[9], work[8], tm["sum_branch"], nev, nev,
+                         f"k_adj_m<{sto}> (branch matrices, FP64 arithmetic)")
+        rls += [(tm["sum_adjoint"], adj_rl), (tm["sum_branch"], br_rl)]
+    rl = max(rls, key=lambda r: r[0])[1]
     value = args.steps / elapsed
     out = {
         "metric": "lnL+gradient evals/sec at 128 taxa x 10k codons (branch-site model A)",
@@ -193,9 +223,10 @@ def main():
                    "tile": args.tile or 64},
         "roofline": rl,
         "roofline_pruning": prune_rl,
-        "roofline_adjoint": adj_rl,
+        "roofline_child_adjoint": adj_rl,
+        "roofline_branch_matrices": br_rl,
         "eval_ms": {k: tm["sum_" + k] / nev for k in
-                    ("pmat", "prune", "root", "adjoint", "finish", "wall")},
+                    ("pmat", "prune", "root", "adjoint", "branch", "finish", "wall")},
         "lnL": -f,
     }
     if args.fd_steps > 0:
@@ -210,6 +241,8 @@ def main():
                            "path": f"FP64 partials, forward differences ({npar + 1} lnL/step)"}
         out["speedup_vs_fd64_gpu"] = value * fd_el
         out["lnL_error_vs_fp64"] = abs(f - f64)
+    if args.scan_genes > 0:
+        out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, "fp64")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tree, seqs, p, args.codons, data.npatt, npar,
                                            args.cpu_sample_codons)

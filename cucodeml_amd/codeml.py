@@ -1,0 +1,268 @@
+"""codeml-compatible driver for branch-site model A on the engine.
+
+Reads a codeml control file (the subset the model-A path uses: seqfile,
+treefile, outfile, seqtype = 1, model = 2, NSsites = 2, CodonFreq = 2,
+kappa, fix_omega, omega, cleandata = 0; codeml.c:1430-1768), the PHYLIP
+codon alignment and the Newick tree with ``#1`` on the foreground branch,
+fits the model by maximum likelihood on the GPU (bsa_fit, in place of ming2
+at codeml.c:688) and writes the estimates in the layout of codeml's ``out``
+file (lnL line codeml.c:738-741, parameter vector 751, site-class table
+989-1001).
+
+Starting values: an ``in.codeml`` file beside the control file is read as
+codeml reads it (treesub.c:3925-3927: the parameter vector in codeml's
+optimisation variables, or, after a leading -1, fixed values that are only
+evaluated); otherwise branch lengths come from the tree (0.1 where absent),
+p0 = 0.6, p1 = 0.2, w0 = 0.3, kappa and w2 from the control file.
+
+    python -m cucodeml_amd.codeml codeml.ctl [--precision fp32] [--fd]
+
+The likelihood-ratio test of H0 (w2 = 1 fixed) against H1 is ``lrt``:
+2 dlnL with its p-values against chi2_1 and the 50:50 mixture of 0 and chi2_1.
+"""
+import argparse
+import math
+import os
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine, formats
+
+
+def read_ctl(path):
+    """key = value pairs of a codeml control file (* starts a comment)."""
+    out = {}
+    with open(path) as f:
+        for line in f:
+            line = line.split("*", 1)[0].strip()
+            if "=" not in line:
+                continue
+            k, v = line.split("=", 1)
+            out[k.strip()] = v.strip().split()[0] if v.strip() else ""
+    return out
+
+
+def _num(ctl, key, default):
+    v = ctl.get(key)
+    return default if v is None or v == "" else float(v)
+
+
+def check_ctl(ctl):
+    """Reject settings outside the path the engine implements."""
+    need = {"seqtype": 1, "model": 2, "NSsites": 2}
+    for k, v in need.items():
+        if int(_num(ctl, k, v)) != v:
+            raise ValueError(f"{k} = {ctl[k]}: only branch-site model A (seqtype = 1, model = 2, "
+                             f"NSsites = 2) is implemented")
+    if int(_num(ctl, "CodonFreq", 2)) != 2:
+        raise ValueError("only CodonFreq = 2 (F3x4) is implemented")
+    if int(_num(ctl, "clock", 0)) != 0:
+        raise ValueError("only clock = 0 is implemented")
+    if int(_num(ctl, "cleandata", 0)) != 0:
+        raise ValueError("only cleandata = 0 is implemented")
+    if int(_num(ctl, "icode", 0)) != 0:
+        raise ValueError("only the universal code (icode = 0) is implemented")
+    if int(_num(ctl, "fix_kappa", 0)) != 0:
+        raise ValueError("fix_kappa = 1 is not implemented")
+
+
+@dataclass
+class Fit:
+    hypothesis: str
+    lnL: float
+    x: np.ndarray                 # optimisation variables (logistic p's)
+    kappa: float
+    p0: float
+    p1: float
+    w0: float
+    w2: float
+    proportions: np.ndarray       # classes 0, 1, 2a, 2b
+    branch: np.ndarray            # branch lengths in parameter order
+    np: int
+    iterations: int
+    evaluations: int
+    converged: bool
+    seconds: float
+    extra: dict = field(default_factory=dict)
+
+    def x_untransformed(self):
+        """x as codeml prints it: p0, p1 themselves in place of the logistic variables."""
+        nb = len(self.branch)
+        x = self.x.copy()
+        x[nb + 1], x[nb + 2] = self.p0, self.p1
+        return x
+
+
+def default_start(tree, kappa=2.0, omega=1.5, fix_omega=False):
+    t = tree.branch_vector().copy()
+    t[t <= 0] = 0.1
+    x = list(t) + [kappa, math.log(0.6 / 0.2), 0.0, 0.3]
+    if not fix_omega:
+        x.append(max(omega, 1.0))
+    return np.array(x)
+
+
+def fit_model_a(tree, data, fix_omega=False, omega=1.0, start=None, precision="fp64",
+                gradient="adjoint", e=1e-8, device=0, kappa=2.0, eng=None, polish=True):
+    """Maximum-likelihood fit of H1 (fix_omega False) or H0 (w2 fixed at
+    omega).  ``start`` is in codeml's optimisation variables.
+
+    With FP32 partials the likelihood carries rounding noise of about 1e-9
+    of |lnL| (stochastic rounding of every stored value), which stalls BFGS
+    where the remaining gains are of that size; with ``polish`` the fit is
+    then finished on an FP64 engine from the FP32 estimates, and the
+    reported lnL is the FP64 one."""
+    own = eng is None
+    if own:
+        eng = engine.BranchSiteEngine(tree, data, precision=precision, device=device,
+                                      fix_omega=fix_omega, omega_fix=omega)
+    else:
+        eng.set_model(fix_omega, omega, True)
+    try:
+        x0 = default_start(tree, kappa, omega if not fix_omega else 1.5, fix_omega) \
+            if start is None else np.asarray(start, dtype=np.float64)
+        t0 = time.perf_counter()
+        x, f, info = eng.fit(x0, e=e, gradient=gradient)
+        extra = {}
+        par = eng.last_params()
+        if eng.precision == "fp32" and polish:
+            extra = {"fp32_lnL": -f, "fp32_iterations": info["iterations"],
+                     "fp32_evaluations": info["evaluations"]}
+            e64 = engine.BranchSiteEngine(tree, data, precision="fp64", device=eng.device,
+                                          fix_omega=fix_omega, omega_fix=omega)
+            try:
+                x, f, info2 = e64.fit(x, e=e, gradient=gradient)
+                par = e64.last_params()
+            finally:
+                e64.close()
+            info = {"iterations": info["iterations"] + info2["iterations"],
+                    "evaluations": info["evaluations"] + info2["evaluations"],
+                    "converged": info2["converged"]}
+        sec = time.perf_counter() - t0
+        nb = tree.nbranch
+        return Fit("H0" if fix_omega else "H1", -f, x, par[0], par[1], par[2], par[3], par[4],
+                   par[7:11].copy(), x[:nb].copy(), eng.np, info["iterations"],
+                   info["evaluations"], info["converged"], sec, extra)
+    finally:
+        if own:
+            eng.close()
+
+
+def lrt(h0, h1):
+    """2 dlnL of H1 over H0 and its p-values: chi2 with 1 df, and the 50:50
+    mixture of a point mass at 0 and chi2_1 (the boundary null of w2 = 1)."""
+    stat = max(0.0, 2.0 * (h1.lnL - h0.lnL))
+    p1 = math.erfc(math.sqrt(stat / 2.0))
+    return {"stat": stat, "p_chi2_1": p1, "p_mixture": 0.5 * p1}
+
+
+def format_out(tree, data, fit, names):
+    """Text in the layout of codeml's out file for the fitted model."""
+    nb = tree.nbranch
+    t = tree.branch.copy()
+    for v in range(tree.nnode):
+        if v != tree.root:
+            t[v] = fit.branch[tree.ibranch[v]]
+    tr = formats.Tree(tree.ns, tree.nnode, tree.root, tree.father, tree.sons, t, tree.label,
+                      tree.ibranch, names)
+    lines = [f"Model A: branch-site (model = 2, NSsites = 2), {fit.hypothesis}",
+             f"ns = {data.ns}  ls = {data.ls}  patterns = {data.npatt}", ""]
+    lines.append(f"lnL(ntime: {nb}  np: {fit.np}):  {fit.lnL:12.6f}      +0.000000")
+    lines.append("  ".join(f"{v:8.6f}" for v in fit.x_untransformed()))
+    lines.append("")
+    lines.append(tr.newick(lengths=True, labels=True, names=names, digits=6))
+    lines.append("")
+    lines.append(f"kappa (ts/tv) = {fit.kappa:8.5f}")
+    lines.append("")
+    lines.append("MLEs of dN/dS (w) for site classes (K=4)")
+    lines.append("")
+    lines.append("site class             0        1       2a       2b")
+    pr = fit.proportions
+    lines.append("proportion       " + " ".join(f"{v:8.5f}" for v in pr))
+    lines.append("background w     " + " ".join(f"{v:8.5f}" for v in (fit.w0, 1.0, fit.w0, 1.0)))
+    lines.append("foreground w     " + " ".join(f"{v:8.5f}" for v in (fit.w0, 1.0, fit.w2, fit.w2)))
+    lines.append("")
+    lines.append(f"iterations {fit.iterations}  evaluations {fit.evaluations}  "
+                 f"converged {int(fit.converged)}  time {fit.seconds:.3f} s")
+    return "\n".join(lines) + "\n"
+
+
+def read_in_codeml(path):
+    """(fixed, values) from an in.codeml file (treesub.c:3925-3927)."""
+    with open(path) as f:
+        vals = [float(v) for v in f.read().split()]
+    if vals and vals[0] == -1:
+        return True, np.array(vals[1:])
+    return False, np.array(vals)
+
+
+def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
+    ctl = read_ctl(ctl_path)
+    check_ctl(ctl)
+    base = os.path.dirname(os.path.abspath(ctl_path))
+    names, seqs = formats.read_phylip(os.path.join(base, ctl["seqfile"]))
+    with open(os.path.join(base, ctl["treefile"])) as f:
+        txt = f.read()
+    # a treefile may start with "ns ntree"
+    first = txt.lstrip().split("\n", 1)
+    if len(first) == 2 and all(tok.isdigit() for tok in first[0].split()):
+        txt = first[1]
+    tree = formats.parse_newick(txt, names)
+    if len(tree.sons[tree.root]) < 3:
+        raise ValueError("the tree is rooted; give it unrooted (root with 3 sons), as codeml "
+                         "uses with clock = 0")
+    if not np.any(tree.label == 1):
+        raise ValueError("no foreground branch (#1) in the tree")
+    data = formats.load_codon_data(names, seqs)
+    fix_omega = int(_num(ctl, "fix_omega", 0)) == 1
+    omega = _num(ctl, "omega", 1.0)
+    kappa = _num(ctl, "kappa", 2.0)
+    eng = engine.BranchSiteEngine(tree, data, precision=precision, device=device,
+                                  fix_omega=fix_omega, omega_fix=omega)
+    try:
+        start = None
+        inpath = os.path.join(base, "in.codeml")
+        if os.path.exists(inpath):
+            fixed, vals = read_in_codeml(inpath)
+            if fixed:
+                eng.set_model(fix_omega, omega, p_transformed=False)
+                negl = eng.lnl(vals[:eng.np])
+                par = eng.last_params()
+                nb = tree.nbranch
+                fit = Fit("H0" if fix_omega else "H1", -negl, vals[:eng.np].copy(), par[0],
+                          par[1], par[2], par[3], par[4], par[7:11].copy(), vals[:nb].copy(),
+                          eng.np, 0, 1, True, 0.0)
+                fit.x = engine.x_from_params(vals[:nb], par[0], par[1], par[2], par[3],
+                                             None if fix_omega else par[4])
+                text = format_out(tree, data, fit, names)
+                with open(os.path.join(base, ctl.get("outfile", "out")), "w") as f:
+                    f.write(text)
+                return fit
+            start = vals[:eng.np]
+        fit = fit_model_a(tree, data, fix_omega, omega, start=start, gradient=gradient,
+                          kappa=kappa, eng=eng)
+    finally:
+        eng.close()
+    with open(os.path.join(base, ctl.get("outfile", "out")), "w") as f:
+        f.write(format_out(tree, data, fit, names))
+    return fit
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("ctl", nargs="?", default="codeml.ctl")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--fd", action="store_true", help="forward-difference gradient (gradientB)")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    fit = run(a.ctl, a.precision, "fd" if a.fd else "adjoint", a.device)
+    print(f"lnL = {fit.lnL:.6f}  np = {fit.np}  iterations = {fit.iterations}  "
+          f"time = {fit.seconds:.2f} s")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

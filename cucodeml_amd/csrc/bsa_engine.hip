@@ -63,7 +63,7 @@ struct bsa_engine {
 
     // device
     hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *d_uvr = nullptr;
     PmatJob *d_pjobs = nullptr;
     PruneJob *d_prune = nullptr;
@@ -84,11 +84,12 @@ struct bsa_engine {
     bool have_eigen = false;
     double cache_kappa = NAN, cache_w[3] = {NAN, NAN, NAN};
     double last_lnl = NAN;
-    double timing[6] = {0, 0, 0, 0, 0, 0};      // pmat, prune, root, adjoint, finish, wall
-    double timing_sum[6] = {0, 0, 0, 0, 0, 0};
+    // pmat, prune, root, child adjoints, branch matrices, adjoint finish, wall
+    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
+    double timing_sum[7] = {0, 0, 0, 0, 0, 0, 0};
     double n_eval = 0, n_launch = 0, n_adj_launch = 0;
     bool evaluated = false;
-    double work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double work[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // adjoint (allocated on the first gradient call)
     bool adj_ready = false;
@@ -475,6 +476,7 @@ static int run_device(bsa_engine *e, bool adjoint)
             launch_adjoint_level<T>(s, e->d_adj + e->adj_off[d], e->adj_nint[d], e->npad, e->d_z,
                                     (const double *)e->d_PT, (const double *)e->d_AT, e->namb,
                                     e->d_E, (const T *)e->d_C, (T *)e->d_A, e->d_EA);
+        HIPCHK(hipEventRecord(e->ev[6], s));
         launch_adjoint_m<T>(s, e->d_adj, (int)e->adj.size(), e->npad, e->nchunk, e->d_z,
                             (const double *)e->d_PT, (const double *)e->d_AT, e->namb, e->d_mask64,
                             (const T *)e->d_L, e->d_E, (const T *)e->d_C, (const T *)e->d_A,
@@ -499,15 +501,17 @@ static int run_device(bsa_engine *e, bool adjoint)
     e->timing[1] = ms;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[2], e->ev[3]));
     e->timing[2] = ms;
-    e->timing[3] = e->timing[4] = 0;
+    e->timing[3] = e->timing[4] = e->timing[5] = 0;
     if (adjoint) {
-        HIPCHK(hipEventElapsedTime(&ms, e->ev[3], e->ev[4]));
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[3], e->ev[6]));
         e->timing[3] = ms;
-        HIPCHK(hipEventElapsedTime(&ms, e->ev[4], e->ev[5]));
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[6], e->ev[4]));
         e->timing[4] = ms;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[4], e->ev[5]));
+        e->timing[5] = ms;
         e->n_adj_launch += (double)e->adj_off.size();
     }
-    for (int i = 0; i < 5; i++) e->timing_sum[i] += e->timing[i];
+    for (int i = 0; i < 6; i++) e->timing_sum[i] += e->timing[i];
     e->n_launch += (double)(e->level_off.size() - 1);
     return BSA_OK;
 }
@@ -520,8 +524,8 @@ static int evaluate(bsa_engine *e, const double *x, double *lnl, bool adjoint = 
     int rc = evaluate_inner(e, x, lnl, adjoint);
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc == BSA_OK) {
-        e->timing[5] = ms;
-        e->timing_sum[5] += ms;
+        e->timing[6] = ms;
+        e->timing_sum[6] += ms;
         e->n_eval += 1;
     }
     return rc;
@@ -737,25 +741,36 @@ static int setup_adjoint(bsa_engine *e)
     HIPCHK(hipMalloc(&e->d_mask64, sizeof(double) * m64.size()));
     HIPCHK(hipMemcpy(e->d_mask64, m64.data(), sizeof(double) * m64.size(), hipMemcpyHostToDevice));
     HIPCHK(hipHostMalloc((void **)&e->h_adjout, sizeof(double) * (nej + 3 * 4096), hipHostMallocDefault));
-    // work of one adjoint sweep on 61 states: per job, one contraction per
-    // interior sibling, one for the child's adjoint (interior child) and the
-    // branch-matrix product; bytes: parent adjoint, interior siblings' and
-    // the child's partials read, the child's adjoint written, exponents
+    // work of one gradient on 61 states, per kernel.  Child adjoints (interior
+    // children only): one 61 x 61 contraction per pattern; bytes: the
+    // parent's adjoint per term, the siblings' kept contributions (interior)
+    // or codes (tips), exponents, the child's adjoint and exponent written.
+    // Branch matrices (every job): one 61 x 61 outer-product accumulation per
+    // pattern; bytes: the same X operands, plus the child's partials and
+    // exponent (interior) or code (tip).  The contributions C kept by the
+    // forward pass are the siblings' partials times P, so no contraction is
+    // repeated for them.
     {
         const double w = (double)ts, np_ = e->npatt;
-        double fl = 0, by = 0;
+        double fa = 0, ba = 0, fm = 0, bm = 0;
         for (const AdjJob &j : e->adj) {
             int nint = 0;
             for (int q = 0; q < j.nsib; q++) nint += j.sib_tip[q] < 0;
             const bool vint = j.v_tip < 0;
             const int nt = j.nterm;
-            fl += 2.0 * kS * kS * np_ * (vint ? 2 : 1);
-            by += np_ * (kS * w * (nt * (1 + nint) + (vint ? 2 : 0)) +
-                         4.0 * (nt * (1 + nint) + (vint ? 2 : 0)) + nt * (j.nsib - nint) +
-                         (vint ? 0 : 1));
+            const double xbytes = kS * w * nt * (1 + nint) + 4.0 * nt * (1 + nint) +
+                                  (double)nt * (j.nsib - nint);
+            if (vint) {
+                fa += 2.0 * kS * kS * np_;
+                ba += np_ * (xbytes + kS * w + 4.0);
+            }
+            fm += 2.0 * kS * kS * np_;
+            bm += np_ * (xbytes + (vint ? kS * w + 4.0 : 1.0));
         }
-        e->work[6] = fl;
-        e->work[7] = by;
+        e->work[6] = fa;
+        e->work[7] = ba;
+        e->work[8] = fm;
+        e->work[9] = bm;
     }
     e->adj_ready = true;
     return BSA_OK;
@@ -866,10 +881,10 @@ extern "C" int bsa_timing(bsa_engine *e, double *o, int reset)
 {
     if (!e || !o) return fail(BSA_EINVAL, "null argument");
     std::memcpy(o, e->timing, sizeof e->timing);
-    std::memcpy(o + 6, e->timing_sum, sizeof e->timing_sum);
-    o[12] = e->n_eval;
-    o[13] = e->n_launch;
-    o[14] = e->n_adj_launch;
+    std::memcpy(o + 7, e->timing_sum, sizeof e->timing_sum);
+    o[14] = e->n_eval;
+    o[15] = e->n_launch;
+    o[16] = e->n_adj_launch;
     if (reset) {
         for (double &v : e->timing_sum) v = 0;
         e->n_eval = 0;
@@ -906,6 +921,31 @@ extern "C" int bsa_work(bsa_engine *e, double *o)
     std::memcpy(o, e->work, sizeof e->work);
     return BSA_OK;
 }
+
+extern "C" int bsa_default_bounds(const bsa_engine *e, double *lo, double *hi)
+{
+    if (!e || !lo || !hi) return fail(BSA_EINVAL, "null argument");
+    const int nb = e->nbranch;
+    for (int b = 0; b < nb; b++) {
+        lo[b] = 4e-6;
+        hi[b] = 50;
+    }
+    lo[nb] = 1e-4;   // kappa
+    hi[nb] = 999;
+    for (int k = 1; k <= 2; k++) {
+        lo[nb + k] = e->opts.p_transformed ? -99 : 1e-6;
+        hi[nb + k] = e->opts.p_transformed ? 99 : 1 - 1e-6;
+    }
+    lo[nb + 3] = 1e-6;   // w0 < 1
+    hi[nb + 3] = 1;
+    if (!e->opts.fix_omega) {   // w2 > 1
+        lo[nb + 4] = 1;
+        hi[nb + 4] = 999;
+    }
+    return BSA_OK;
+}
+
+int bsa_error(int code, const char *msg) { return fail(code, "%s", msg); }
 
 extern "C" const char *bsa_last_error(void) { return g_err.c_str(); }
 extern "C" const char *bsa_version(void) { return "bsa 0.1 gfx950"; }

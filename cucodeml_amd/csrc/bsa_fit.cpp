@@ -1,0 +1,260 @@
+// Bounded quasi-Newton maximum-likelihood fit of branch-site model A on one
+// engine: the optimiser that takes the place of the reference's ming2
+// (tools.c:6361-6545) at its call in the ML driver (codeml.c:688), with the
+// engine's adjoint gradient (or its forward differences, gradientB
+// tools.c:6326-6350) instead of the reference's finite differences.
+//
+// Method (an active-set projected BFGS; Nocedal & Wright, Numerical
+// Optimization, 2nd ed., sections 6.1 and 16.7):
+//   * x is kept inside the box [lo, hi]; a variable sitting on a bound whose
+//     gradient points out of the box is held there for the iteration;
+//   * direction d = -H g on the free variables, H the inverse-Hessian
+//     approximation (identity, scaled by s'y / y'y after the first step, as
+//     in N&W eq. 6.20);
+//   * projected backtracking line search: x(a) = clamp(x + a d), Armijo
+//     condition f(x(a)) <= f + 1e-4 g'(x(a) - x), step cut by safeguarded
+//     quadratic interpolation;
+//   * BFGS inverse update restricted to the free variables, skipped when the
+//     curvature s'y is not positive;
+//   * stop when two successive steps change f by less than e |f| and x by
+//     less than e |x| (the reference's H_end rule, tools.c:5724-5740),
+//     confirmed once more after restarting H from the scaled identity, or
+//     when the projected gradient vanishes or no descent step can be found
+//     along the scaled steepest descent.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "bsa.h"
+
+int bsa_error(int code, const char *msg);   // sets bsa_last_error (engine)
+
+namespace {
+
+// objective: f and its gradient g at x; returns a bsa status code
+using ObjFn = std::function<int(const double *x, double *f, double *g)>;
+
+inline double clampd(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
+
+struct FitResult {
+    double f = 0;
+    int iterations = 0, evaluations = 0, converged = 0;
+};
+
+int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
+                 const bsa_fit_opts *opts, FitResult *res);
+
+}  // namespace
+
+namespace {
+
+int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const double *hi,
+                 const bsa_fit_opts *opts, FitResult *res)
+{
+    const int max_iter = opts && opts->max_iter > 0 ? opts->max_iter : 2000;
+    const double e = opts && opts->e > 0 ? opts->e : 1e-8;
+    const double max_step = opts && opts->max_step > 0 ? opts->max_step : 5.0;
+    int nfev = 0;
+    auto fg = [&](const double *xx, double *ff, double *gg) {
+        nfev++;
+        return fgfun(xx, ff, gg);
+    };
+    std::vector<double> g(n), xt(n), gt(n), d(n), s(n), y(n), Hy(n), H((size_t)n * n, 0.0);
+    std::vector<char> act(n);
+    for (int i = 0; i < n; i++) x[i] = clampd(x[i], lo[i], hi[i]);
+    double f;
+    int rc = fg(x, &f, g.data());
+    if (rc) return rc;
+    auto reset_h = [&](double gamma) {
+        std::fill(H.begin(), H.end(), 0.0);
+        for (int i = 0; i < n; i++) H[(size_t)i * n + i] = gamma;
+    };
+    reset_h(1.0);
+    bool scaled = false, fresh = true;
+    double gamma = 1.0;
+    int it = 0, good = 0, converged = 0;
+    bool confirmed = false;
+    for (; it < max_iter; it++) {
+        // working set and projected-gradient test
+        double pg = 0;
+        for (int i = 0; i < n; i++) {
+            const double tol = 1e-12 * (1 + std::fabs(lo[i]) + std::fabs(hi[i]));
+            act[i] = (x[i] <= lo[i] + tol && g[i] > 0) || (x[i] >= hi[i] - tol && g[i] < 0);
+            pg = std::max(pg, std::fabs(clampd(x[i] - g[i], lo[i], hi[i]) - x[i]));
+        }
+        if (pg < 1e-10) {
+            converged = 1;
+            break;
+        }
+        double gd = 0, dmax = 0;
+        for (int i = 0; i < n; i++) {
+            d[i] = 0;
+            if (act[i]) continue;
+            for (int j = 0; j < n; j++)
+                if (!act[j]) d[i] -= H[(size_t)i * n + j] * g[j];
+            gd += g[i] * d[i];
+        }
+        if (!(gd < 0)) {   // not a descent direction: restart from scaled identity
+            reset_h(gamma);
+            fresh = true;
+            gd = 0;
+            for (int i = 0; i < n; i++) {
+                d[i] = act[i] ? 0 : -gamma * g[i];
+                gd += g[i] * d[i];
+            }
+        }
+        for (int i = 0; i < n; i++) dmax = std::max(dmax, std::fabs(d[i]));
+        if (!(dmax > 0)) {
+            converged = 1;
+            break;
+        }
+        double a = 1.0;
+        if (!scaled) a = std::min(1.0, 0.1 / dmax);   // first step: move at most 0.1
+        a = std::min(a, max_step / dmax);
+        // projected backtracking line search
+        double ft = 0;
+        bool ok = false;
+        for (int ls = 0; ls < 60; ls++) {
+            double slope = 0, dx = 0;
+            for (int i = 0; i < n; i++) {
+                xt[i] = clampd(x[i] + a * d[i], lo[i], hi[i]);
+                slope += g[i] * (xt[i] - x[i]);
+                dx = std::max(dx, std::fabs(xt[i] - x[i]));
+            }
+            if (dx == 0) break;
+            rc = fg(xt.data(), &ft, gt.data());
+            if (rc && rc != BSA_ENUMERIC) return rc;
+            if (rc == BSA_OK && std::isfinite(ft) && ft <= f + 1e-4 * slope) {
+                ok = true;
+                break;
+            }
+            double an = 0.5 * a;
+            if (rc == BSA_OK && std::isfinite(ft)) {
+                // minimiser of the quadratic through f, slope and ft along the path
+                const double sl = slope / a, den = 2 * (ft - f - sl * a);
+                if (den > 0) an = -sl * a * a / den;
+            }
+            a = clampd(an, 0.1 * a, 0.5 * a);
+        }
+        if (!ok) {
+            if (fresh) {   // no progress even along the scaled steepest descent
+                converged = 1;
+                break;
+            }
+            reset_h(gamma);
+            fresh = true;
+            continue;
+        }
+        // curvature pair on the free variables
+        double sy = 0, yy = 0, s2 = 0;
+        for (int i = 0; i < n; i++) {
+            s[i] = xt[i] - x[i];
+            y[i] = act[i] ? 0 : gt[i] - g[i];
+            if (act[i]) s[i] = 0;
+            sy += s[i] * y[i];
+            yy += y[i] * y[i];
+            s2 += s[i] * s[i];
+        }
+        if (sy > 1e-12 * std::sqrt(yy * s2) && yy > 0) {
+            if (!scaled) {
+                gamma = sy / yy;
+                reset_h(gamma);
+                scaled = true;
+            }
+            double yHy = 0;
+            for (int i = 0; i < n; i++) {
+                double v = 0;
+                for (int j = 0; j < n; j++) v += H[(size_t)i * n + j] * y[j];
+                Hy[i] = v;
+                yHy += y[i] * v;
+            }
+            const double c1 = (sy + yHy) / (sy * sy);
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++)
+                    H[(size_t)i * n + j] += c1 * s[i] * s[j] - (Hy[i] * s[j] + s[i] * Hy[j]) / sy;
+            fresh = false;
+        }
+        // two successive steps that change f by less than e |f| and x by
+        // less than e |x| (2-norms, as H_end) end the fit, after one restart
+        // of H from the scaled identity has confirmed it
+        double sn = 0, xn = 0;
+        for (int i = 0; i < n; i++) {
+            sn += (xt[i] - x[i]) * (xt[i] - x[i]);
+            xn += x[i] * x[i];
+        }
+        const bool small_step = std::fabs(f - ft) < e * std::max(1.0, std::fabs(f)) &&
+                                std::sqrt(sn) < e * std::max(1.0, std::sqrt(xn));
+        std::memcpy(x, xt.data(), sizeof(double) * n);
+        std::swap(g, gt);
+        f = ft;
+        good = small_step ? good + 1 : 0;
+        if (good >= 2) {
+            if (confirmed) {
+                converged = 1;
+                it++;
+                break;
+            }
+            confirmed = true;
+            good = 0;
+            reset_h(gamma);
+            fresh = true;
+        }
+    }
+    res->f = f;
+    res->iterations = it;
+    res->evaluations = nfev;
+    res->converged = converged;
+    return BSA_OK;
+}
+
+}  // namespace
+
+extern "C" int bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, const double *hi,
+                       const bsa_fit_opts *opts, double *neg_lnl, int *info)
+{
+    if (!eng || !x || !lo || !hi || !neg_lnl) return bsa_error(BSA_EINVAL, "null argument");
+    if (np != bsa_num_params(eng)) return bsa_error(BSA_EINVAL, "np does not match the engine");
+    for (int i = 0; i < np; i++)
+        if (!(lo[i] <= hi[i])) return bsa_error(BSA_EINVAL, "bounds with lo > hi");
+    const bool fd = opts && opts->gradient == BSA_GRAD_FD;
+    ObjFn fg = [&](const double *xx, double *f, double *g) {
+        return fd ? bsa_lnl_grad_fd(eng, xx, np, 0, 1e-6, f, g) : bsa_lnl_grad(eng, xx, np, f, g);
+    };
+    FitResult r;
+    int rc = minimize_box(fg, np, x, lo, hi, opts, &r);
+    if (rc) return rc;
+    // leave the engine's state (per-site values, parameters) at the returned x
+    double f;
+    if ((rc = bsa_lnl(eng, x, np, &f))) return rc;
+    *neg_lnl = f;
+    if (info) {
+        info[0] = r.iterations;
+        info[1] = r.evaluations + 1;
+        info[2] = r.converged;
+    }
+    return BSA_OK;
+}
+
+extern "C" int bsa_debug_minimize(int n, bsa_objective fn, void *ctx, double *x, const double *lo,
+                                  const double *hi, const bsa_fit_opts *opts, double *fmin,
+                                  int *info)
+{
+    if (n <= 0 || !fn || !x || !lo || !hi || !fmin) return bsa_error(BSA_EINVAL, "bad argument");
+    for (int i = 0; i < n; i++)
+        if (!(lo[i] <= hi[i])) return bsa_error(BSA_EINVAL, "bounds with lo > hi");
+    ObjFn fg = [&](const double *xx, double *f, double *g) {
+        return fn(xx, f, g, ctx) ? BSA_ENUMERIC : BSA_OK;
+    };
+    FitResult r;
+    int rc = minimize_box(fg, n, x, lo, hi, opts, &r);
+    if (rc) return rc;
+    *fmin = r.f;
+    if (info) {
+        info[0] = r.iterations;
+        info[1] = r.evaluations;
+        info[2] = r.converged;
+    }
+    return BSA_OK;
+}

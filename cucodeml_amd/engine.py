@@ -23,7 +23,7 @@ BSA_KERNEL_MFMA, BSA_KERNEL_VALU = 0, 1
 EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
             "bsa_last_error", "bsa_version", "bsa_debug_mfma_probe", "bsa_work",
-            "bsa_model_a_setup"]
+            "bsa_model_a_setup", "bsa_default_bounds", "bsa_fit", "bsa_debug_minimize"]
 
 
 class BsaError(RuntimeError):
@@ -42,6 +42,16 @@ class _Problem(C.Structure):
 
 class _Opts(C.Structure):
     _fields_ = [("fix_omega", C.c_int), ("omega_fix", C.c_double), ("p_transformed", C.c_int)]
+
+
+class _FitOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("e", C.c_double), ("max_step", C.c_double),
+                ("gradient", C.c_int)]
+
+
+BSA_GRAD_ADJOINT, BSA_GRAD_FD = 0, 1
+_OBJECTIVE = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.POINTER(C.c_double), C.c_void_p)
 
 
 _lib = None
@@ -69,6 +79,11 @@ def lib():
         L.bsa_last_error.restype = C.c_char_p
         L.bsa_version.restype = C.c_char_p
         L.bsa_debug_mfma_probe.argtypes = [C.c_int, dp, dp, dp, dp]
+        L.bsa_default_bounds.argtypes = [C.c_void_p, dp, dp]
+        L.bsa_fit.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, C.POINTER(_FitOpts), dp,
+                              C.POINTER(C.c_int)]
+        L.bsa_debug_minimize.argtypes = [C.c_int, _OBJECTIVE, C.c_void_p, dp, dp, dp,
+                                         C.POINTER(_FitOpts), dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -97,6 +112,7 @@ class BranchSiteEngine:
     def __init__(self, tree, data, precision="fp64", kernel="mfma", device=0,
                  fix_omega=False, omega_fix=1.0, p_transformed=True, pi=None, tile=0):
         self.tree, self.data = tree, data
+        self.precision, self.device = precision, device
         self._keep = []
         ns, nn = tree.ns, tree.nnode
 
@@ -166,6 +182,32 @@ class BranchSiteEngine:
                                      C.byref(out), _dptr(g)))
         return out.value, g
 
+    def default_bounds(self):
+        """codeml's bounds (lo, hi) for this engine's x layout (SetxBound)."""
+        lo, hi = np.zeros(self.np), np.zeros(self.np)
+        _check(lib().bsa_default_bounds(self._h, _dptr(lo), _dptr(hi)))
+        return lo, hi
+
+    def fit(self, x0, lo=None, hi=None, e=1e-8, max_iter=2000, max_step=5.0,
+            gradient="adjoint"):
+        """Bounded BFGS maximum-likelihood fit from x0 (replaces ming2 at
+        codeml.c:688).  Returns (x, -lnL, {"iterations", "evaluations",
+        "converged"})."""
+        x = np.array(x0, dtype=np.float64)
+        if len(x) != self.np:
+            raise BsaError(f"x has {len(x)} values, the engine expects {self.np}")
+        dlo, dhi = self.default_bounds()
+        lo = dlo if lo is None else np.ascontiguousarray(lo, dtype=np.float64)
+        hi = dhi if hi is None else np.ascontiguousarray(hi, dtype=np.float64)
+        o = _FitOpts(int(max_iter), float(e), float(max_step),
+                     BSA_GRAD_FD if gradient == "fd" else BSA_GRAD_ADJOINT)
+        f = C.c_double()
+        info = (C.c_int * 3)()
+        _check(lib().bsa_fit(self._h, _dptr(x), len(x), _dptr(lo), _dptr(hi), C.byref(o),
+                             C.byref(f), info))
+        return x, f.value, {"iterations": info[0], "evaluations": info[1],
+                            "converged": bool(info[2])}
+
     def fhK(self):
         """(per-class log f (4, npatt), per-pattern mixture log f (npatt,))"""
         f = np.zeros(4 * self.data.npatt)
@@ -179,19 +221,49 @@ class BranchSiteEngine:
         return o
 
     def work(self):
-        o = np.zeros(8)
+        o = np.zeros(10)
         _check(lib().bsa_work(self._h, _dptr(o)))
         return o
 
     def timing(self, reset=False):
         """dict of last / summed device times (ms) and counts"""
-        o = np.zeros(15)
+        o = np.zeros(17)
         _check(lib().bsa_timing(self._h, _dptr(o), int(reset)))
-        keys = ["pmat", "prune", "root", "adjoint", "finish", "wall"]
+        keys = ["pmat", "prune", "root", "adjoint", "branch", "finish", "wall"]
         d = {f"last_{k}": float(o[i]) for i, k in enumerate(keys)}
-        d.update({f"sum_{k}": float(o[6 + i]) for i, k in enumerate(keys)})
-        d["n_eval"], d["n_launch"], d["n_adj_launch"] = float(o[12]), float(o[13]), float(o[14])
+        d.update({f"sum_{k}": float(o[7 + i]) for i, k in enumerate(keys)})
+        d["n_eval"], d["n_launch"], d["n_adj_launch"] = float(o[14]), float(o[15]), float(o[16])
         return d
+
+
+def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0):
+    """The engine's bounded BFGS on a Python objective fun(x) -> (f, grad)
+    (diagnostic; runs on the host).  Returns (x, f, info)."""
+    n = len(x0)
+
+    def cb(xp, fp, gp, _ctx):
+        try:
+            xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+            f, g = fun(xv)
+            if not np.isfinite(f):
+                return 1
+            fp[0] = float(f)
+            for i in range(n):
+                gp[i] = float(g[i])
+            return 0
+        except Exception:
+            return 1
+
+    cfun = _OBJECTIVE(cb)
+    x = np.array(x0, dtype=np.float64)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
+    f = C.c_double()
+    info = (C.c_int * 3)()
+    _check(lib().bsa_debug_minimize(n, cfun, None, _dptr(x), _dptr(lo), _dptr(hi), C.byref(o),
+                                    C.byref(f), info))
+    return x, f.value, {"iterations": info[0], "evaluations": info[1], "converged": bool(info[2])}
 
 
 def model_a_setup(kappa, p0, p1, w0, w2, pi):

@@ -35,11 +35,16 @@ def model_a_classes(kappa, w0, w2, p0, p1, pi):
     return props, omegas, (1.0 / mr_b, 1.0 / mr_f)
 
 
-def _expm_rev(Q, pi, t):
+def _eig_rev(Q, pi):
     s = np.sqrt(pi)
     S = (s[:, None] * Q) / s[None, :]
     S = 0.5 * (S + S.T)
     lam, R = np.linalg.eigh(S)
+    return lam, R, s
+
+
+def _expm_rev(Q, pi, t, eig=None):
+    lam, R, s = _eig_rev(Q, pi) if eig is None else eig
     P = (R * np.exp(lam * t)[None, :]) @ R.T
     P = P / s[:, None] * s[None, :]
     P = np.clip(P, 0.0, None)
@@ -161,9 +166,10 @@ def simulate_model_a(tree, ncodon, kappa, w0, w2, p0, p1, pi, rng, gap_frac=0.0)
     """Codon alignment (list of nucleotide strings) under branch-site model A."""
     props, omegas, qf = model_a_classes(kappa, w0, w2, p0, p1, pi)
     cls = rng.choice(4, size=ncodon, p=props / props.sum())
-    Qs = {}
+    Qs, eig = {}, {}
     for w in {w0, 1.0, w2}:
         Qs[w], _ = gy94_q(kappa, w, pi)
+        eig[w] = _eig_rev(Qs[w], pi)
     states = np.zeros((tree.nnode, ncodon), dtype=np.int64)
     cum_pi = np.cumsum(pi)
     states[tree.root] = np.minimum(np.searchsorted(cum_pi, rng.random(ncodon) * cum_pi[-1]), 60)
@@ -183,7 +189,7 @@ def simulate_model_a(tree, ncodon, kappa, w0, w2, p0, p1, pi, rng, gap_frac=0.0)
             if idx.size == 0:
                 continue
             w = omegas[c][lab]
-            P = _expm_rev(Qs[w], pi, tree.branch[v] * qf[lab])
+            P = _expm_rev(Qs[w], pi, tree.branch[v] * qf[lab], eig[w])
             cum = np.cumsum(P, axis=1)
             u = rng.random(idx.size) * cum[par[idx], -1]
             row = cum[par[idx]]

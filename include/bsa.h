@@ -123,14 +123,15 @@ int  bsa_class_lnl(bsa_engine *eng, double *fhK, double *site_lnf);
 int  bsa_last_params(const bsa_engine *eng, double *out11);
 
 /* timing in ms, from HIP events on the engine's stream: out[0] P(t) and tip
- * tables, out[1] pruning (all levels), out[2] root reduction, out[3]
- * adjoint sweep (all levels; 0 without gradient), out[4] adjoint finish
- * (branch-matrix reduction and eigen-basis kernels), out[5] host wall time
- * of the whole evaluation, all for the last evaluation; out[6..11] the same
- * summed since creation or the last reset; out[12] evaluations, out[13]
- * pruning launches and out[14] adjoint-sweep launches in that sum.
- * reset != 0 clears the sums after reading them. */
-int  bsa_timing(bsa_engine *eng, double *out15, int reset);
+ * tables, out[1] pruning (all levels), out[2] root reduction, out[3] child
+ * adjoints (all depths; 0 without gradient), out[4] branch matrices (one
+ * launch), out[5] adjoint finish (eigen-basis kernels, set sums), out[6]
+ * host wall time of the whole evaluation, all for the last evaluation;
+ * out[7..13] the same summed since creation or the last reset; out[14]
+ * evaluations, out[15] pruning launches and out[16] adjoint launches (child
+ * adjoints + branch matrices) in that sum.  reset != 0 clears the sums
+ * after reading them. */
+int  bsa_timing(bsa_engine *eng, double *out17, int reset);
 
 /* host only (no GPU): model A at natural parameters, as SetParametersNSsites
  * (codeml.c:2470-2520) with eigenQcodon/eigenQREV (codeml.c:3103-3272,
@@ -147,9 +148,46 @@ int  bsa_model_a_setup(double kappa, double p0, double p1, double w0, double w2,
  * (partials read and written, tip codes, scale exponents), out[2] flops and
  * out[3] bytes of the reference algorithm's pruning (4 classes over the
  * whole tree), out[4] pruning launches (tree levels), out[5] pruning
- * (node, class) jobs, out[6] adjoint-sweep flops and out[7] adjoint-sweep
- * algorithmic bytes per gradient */
-int  bsa_work(bsa_engine *eng, double *out8);
+ * (node, class) jobs; per gradient: out[6] flops and out[7] algorithmic
+ * bytes of the child adjoints, out[8] flops and out[9] algorithmic bytes of
+ * the branch matrices */
+int  bsa_work(bsa_engine *eng, double *out10);
+
+/* codeml's parameter bounds for model A in the engine's x layout
+ * (SetxBound codeml.c:1786-1828, SetxBoundTimes treesub.c:3872-3884):
+ * branch lengths [4e-6, 50], kappa [1e-4, 999], transformed p's [-99, 99]
+ * (untransformed: [1e-6, 1 - 1e-6]), w0 [1e-6, 1], w2 [1, 999] */
+int  bsa_default_bounds(const bsa_engine *eng, double *lo, double *hi);
+
+/* gradient used by bsa_fit */
+#define BSA_GRAD_ADJOINT  0   /* bsa_lnl_grad                               */
+#define BSA_GRAD_FD       1   /* bsa_lnl_grad_fd, forward differences       */
+
+typedef struct {
+    int    max_iter;          /* quasi-Newton iterations (<= 0: 2000)        */
+    double e;                 /* convergence tolerance on relative changes of
+                                 -lnL and x, as ming2's e (<= 0: 1e-8)        */
+    double max_step;          /* largest change of one variable per line
+                                 search start (<= 0: 5)                      */
+    int    gradient;          /* BSA_GRAD_ADJOINT or BSA_GRAD_FD             */
+} bsa_fit_opts;
+
+/* Maximum-likelihood fit from the start x (np values, moved into [lo, hi]):
+ * bounded BFGS (active set, projected line search).  Replaces
+ * ming2(fout, &lnL, CUDA_lfundG, NULL, x, xb, space, e, np) at codeml.c:688
+ * (ming2: tools.c:6361-6545).  On return x holds the estimates, *neg_lnl
+ * = -lnL there, and info (may be NULL) = {iterations, evaluations,
+ * converged (1) or iteration limit (0)}; the engine's last evaluation is
+ * at x, so bsa_class_lnl / bsa_last_params describe the fit. */
+int  bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, const double *hi,
+             const bsa_fit_opts *opts, double *neg_lnl, int *info3);
+
+/* diagnostic: the same optimiser on a caller-supplied objective (returns
+ * nonzero where it is undefined), for testing it without a GPU; info as in
+ * bsa_fit (no counterpart in the reference) */
+typedef int (*bsa_objective)(const double *x, double *f, double *grad, void *ctx);
+int  bsa_debug_minimize(int n, bsa_objective fn, void *ctx, double *x, const double *lo,
+                        const double *hi, const bsa_fit_opts *opts, double *fmin, int *info3);
 
 const char *bsa_last_error(void);
 

@@ -1,0 +1,75 @@
+"""Host side of the codeml-compatible driver: control-file parsing and the
+model-A checks (codeml.c:1430-1768), in.codeml reading (treesub.c:3925-3927),
+the likelihood-ratio test's p-values against scipy's chi-square, and the out
+file layout read back by the parser the tests use for the reference's own
+out files (codeml.c:738-751)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from cucodeml_amd import codeml
+from . import fixtures
+
+
+def write(d, name, text):
+    p = os.path.join(d, name)
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+def test_read_ctl_comments_and_values():
+    with tempfile.TemporaryDirectory() as d:
+        p = write(d, "c.ctl", "seqfile = a.phy * alignment\n* comment line\n  model = 2\n"
+                              "omega = .4  * start\nfix_omega=1\n")
+        c = codeml.read_ctl(p)
+    assert c == {"seqfile": "a.phy", "model": "2", "omega": ".4", "fix_omega": "1"}
+
+
+def test_check_ctl_accepts_model_a_rejects_others():
+    with tempfile.TemporaryDirectory() as d:
+        good = codeml.read_ctl(write(d, "g.ctl", fixtures.CTL.format(fix_omega=0, omega=1.5)))
+    codeml.check_ctl(good)
+    for k, v in [("model", "0"), ("NSsites", "1"), ("CodonFreq", "3"), ("clock", "1"),
+                 ("cleandata", "1"), ("seqtype", "2"), ("fix_kappa", "1")]:
+        bad = dict(good, **{k: v})
+        with pytest.raises(ValueError):
+            codeml.check_ctl(bad)
+
+
+def test_in_codeml_fixed_and_start():
+    with tempfile.TemporaryDirectory() as d:
+        fixed, v = codeml.read_in_codeml(write(d, "in.codeml", "-1 0.1 0.2\n2.0"))
+        assert fixed and np.allclose(v, [0.1, 0.2, 2.0])
+        fixed, v = codeml.read_in_codeml(write(d, "in2", "0.1 0.2 2.0\n"))
+        assert not fixed and np.allclose(v, [0.1, 0.2, 2.0])
+
+
+@pytest.mark.parametrize("d", [0.0, 0.3, 2.71, 3.84, 10.0, 40.0])
+def test_lrt_p_values(d):
+    h0 = codeml.Fit("H0", -100.0, np.zeros(3), 2, .5, .3, .1, 1, np.zeros(4), np.zeros(1), 3, 0,
+                    0, True, 0)
+    h1 = codeml.Fit("H1", -100.0 + d / 2, np.zeros(3), 2, .5, .3, .1, 2, np.zeros(4),
+                    np.zeros(1), 4, 0, 0, True, 0)
+    r = codeml.lrt(h0, h1)
+    assert abs(r["stat"] - d) < 1e-12
+    assert abs(r["p_chi2_1"] - stats.chi2.sf(d, 1)) < 1e-12
+    assert abs(r["p_mixture"] - 0.5 * stats.chi2.sf(d, 1)) < 1e-12
+
+
+def test_out_file_layout_round_trip():
+    fx, tree, data = fixtures.load("cfg0")
+    nb = tree.nbranch
+    x = np.concatenate([np.linspace(0.01, 0.3, nb), [1.94, np.log(0.5 / 0.2), np.log(0.3 / 0.2),
+                                                     0.05, 4.2]])
+    fit = codeml.Fit("H1", -2241.441189, x, 1.94, 0.5, 0.3, 0.05, 4.2,
+                     np.array([0.5, 0.3, 0.12, 0.08]), x[:nb], len(x), 10, 12, True, 0.5)
+    text = codeml.format_out(tree, data, fit, fx["names"])
+    lnl, npar, xr = fixtures.parse_out(text)
+    assert lnl == -2241.441189 and npar == len(x)
+    want = fit.x_untransformed()
+    assert np.allclose(xr, np.round(want, 6), atol=1e-12)
+    assert abs(xr[nb + 1] - 0.5) < 1e-12 and abs(xr[nb + 2] - 0.3) < 1e-12

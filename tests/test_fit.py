@@ -1,0 +1,74 @@
+"""The bounded BFGS that replaces ming2 (tools.c:6361-6545), run on the host
+through bsa_debug_minimize on objectives whose box-constrained minima are
+known in closed form: Rosenbrock's valley, a separable quadratic whose
+minimiser is the projection of the unconstrained one, a minimum on a corner
+of the box, and an objective that is undefined on part of the box (the line
+search must back off, as ming2's does when lfundG fails)."""
+import numpy as np
+import pytest
+
+from cucodeml_amd import engine
+
+
+def rosen(x):
+    f = np.sum(100 * (x[1:] - x[:-1] ** 2) ** 2 + (1 - x[:-1]) ** 2)
+    g = np.zeros_like(x)
+    g[:-1] += -400 * x[:-1] * (x[1:] - x[:-1] ** 2) - 2 * (1 - x[:-1])
+    g[1:] += 200 * (x[1:] - x[:-1] ** 2)
+    return f, g
+
+
+def test_rosenbrock_interior_minimum():
+    x, f, info = engine.minimize(rosen, np.array([-1.2, 1.0, -1.2, 1.0, 0.5]), [-5] * 5, [5] * 5)
+    assert info["converged"]
+    assert np.max(np.abs(x - 1)) < 1e-5
+    assert f < 1e-10
+
+
+def test_rosenbrock_minimum_on_a_bound():
+    # with x0 <= 0.5 the minimum is x0 = 0.5, x1 = 0.25 (KKT: df/dx0 < 0 there)
+    x, f, info = engine.minimize(rosen, np.array([-1.2, 1.0]), [-5, -5], [0.5, 5])
+    assert info["converged"]
+    assert np.allclose(x, [0.5, 0.25], atol=1e-7)
+    assert abs(f - 0.25) < 1e-12
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_quadratic_minimiser_is_projection(seed):
+    rng = np.random.default_rng(seed)
+    n = 12
+    d = 10 ** rng.uniform(-2, 2, n)            # badly scaled diagonal Hessian
+    c = rng.normal(size=n) * 2
+    lo, hi = -np.ones(n), np.ones(n)
+
+    def q(x):
+        return 0.5 * np.sum(d * (x - c) ** 2), d * (x - c)
+
+    # f-based stopping: |dx|^2 d / 2 < e, so ask for e = 1e-16 to pin x to 1e-6
+    x, f, info = engine.minimize(q, rng.uniform(-1, 1, n), lo, hi, e=1e-16)
+    assert info["converged"]
+    assert np.allclose(x, np.clip(c, lo, hi), atol=1e-6)
+
+
+def test_corner_minimum_and_start_outside_box():
+    # f = sum(x) on [1, 2]^3: minimum at the lower corner; start outside
+    x, f, info = engine.minimize(lambda x: (np.sum(x), np.ones_like(x)), np.array([5.0, -3.0, 9.0]),
+                                 [1, 1, 1], [2, 2, 2])
+    assert np.allclose(x, 1) and abs(f - 3) < 1e-12
+
+
+def test_undefined_region_is_avoided():
+    # log barrier undefined for x <= 0 inside the box; minimum of x - log x at x = 1
+    def f(x):
+        if np.any(x <= 0):
+            return np.inf, np.zeros_like(x)
+        return np.sum(x - np.log(x)), 1 - 1 / x
+
+    x, fv, info = engine.minimize(f, np.array([0.05, 4.0, 9.0]), [-10] * 3, [10] * 3)
+    assert info["converged"]
+    assert np.allclose(x, 1, atol=1e-6)
+
+
+def test_bad_bounds_rejected():
+    with pytest.raises(engine.BsaError):
+        engine.minimize(rosen, np.zeros(2), [1, 0], [0, 1])

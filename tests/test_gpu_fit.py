@@ -1,0 +1,118 @@
+"""Maximum-likelihood fits with the engine's own optimiser (bsa_fit, in place
+of ming2 at codeml.c:688) against the reference CPU codeml's fits of the same
+data from the same seeded starts (golden fixture cfg0, H1 and H0), through
+the Python API and through the codeml-compatible driver (control file in,
+``out`` file out), with the adjoint gradient, with forward differences, and
+with FP32 partials; and the likelihood-ratio statistic of H1 over H0."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from cucodeml_amd import codeml
+from . import fixtures
+
+pytestmark = pytest.mark.gpu
+
+# lnL of a converged fit: the reference prints 6 decimals and stops on
+# relative changes of 1e-8 (ming2's e); estimates agree to the precision the
+# reference's convergence leaves (its x is printed with 6 decimals).  With
+# forward differences (the reference's own gradient) w2 is the least
+# determined estimate of cfg0's H1: at its MLE (p0 = p1 = 0) lnL changes by
+# less than 1e-5 over dw2 = 0.02, which is what the difference gradient can
+# resolve there.
+TOL_LNL = 2e-4
+TOL_X = {"adjoint": 2e-3, "fd": 5e-2}
+
+
+def hyp_args(hyp):
+    return (False, 1.5) if hyp == "H1" else (True, 1.0)
+
+
+def compare(fit, ml, what, tol_x):
+    x = fit.x_untransformed()
+    ref = np.array(ml["mle_x"])
+    dx = np.abs(x - ref)
+    print(f"  {what}: lnL {fit.lnL:.6f} vs reference {ml['mle_lnL']:.6f} "
+          f"(d {fit.lnL - ml['mle_lnL']:+.2e}); max |dx| {dx.max():.2e} at {int(dx.argmax())}; "
+          f"{fit.iterations} iterations, {fit.evaluations} evaluations, {fit.seconds:.2f} s",
+          flush=True)
+    assert fit.converged
+    # the fit may not end below the reference's optimum, and should not
+    # improve on it by more than the reference's own convergence slack
+    assert abs(fit.lnL - ml["mle_lnL"]) <= TOL_LNL
+    assert dx.max() <= tol_x
+
+
+@pytest.mark.parametrize("gradient", ["adjoint", "fd"])
+@pytest.mark.parametrize("hyp", ["H1", "H0"])
+def test_fit_matches_reference_mle(hyp, gradient):
+    print(f"case: fit cfg0 {hyp} {gradient}", flush=True)
+    fx, tree, data = fixtures.load("cfg0")
+    ml = fx["ml"][hyp]
+    fo, om = hyp_args(hyp)
+    fit = codeml.fit_model_a(tree, data, fo, om, start=ml["start"], gradient=gradient)
+    compare(fit, ml, f"{hyp} {gradient}", TOL_X[gradient])
+
+
+def test_fp32_fit_with_fp64_polish_reaches_the_reference_optimum():
+    print("case: fit cfg0 H1 fp32 + fp64 polish", flush=True)
+    fx, tree, data = fixtures.load("cfg0")
+    ml = fx["ml"]["H1"]
+    fit = codeml.fit_model_a(tree, data, False, 1.5, start=ml["start"], precision="fp32")
+    ex = fit.extra
+    print(f"  fp32 stage lnL {ex['fp32_lnL']:.6f} ({ex['fp32_iterations']} iterations); "
+          f"after fp64 polish {fit.lnL:.6f} ({fit.iterations} iterations in all)", flush=True)
+    # the FP32 stage gets close (it stalls at its rounding-noise floor)...
+    assert ex["fp32_lnL"] >= ml["mle_lnL"] - 0.05
+    # ...and the polished fit is the reference's
+    compare(fit, ml, "H1 fp32+polish", TOL_X["adjoint"])
+
+
+def test_driver_control_file_to_out_file():
+    print("case: driver cfg0 H0 from control file", flush=True)
+    fx, _, _ = fixtures.load("cfg0")
+    ml = fx["ml"]["H0"]
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "seq.phy"), "w").write(fx["phylip"])
+        open(os.path.join(d, "tree.nwk"), "w").write(f"{fx['ns']} 1\n{fx['newick']}\n")
+        open(os.path.join(d, "in.codeml"), "w").write(" ".join(f"{v:.12f}" for v in ml["start"]))
+        open(os.path.join(d, "codeml.ctl"), "w").write(fixtures.CTL.format(fix_omega=1,
+                                                                           omega=1.0))
+        fit = codeml.run(os.path.join(d, "codeml.ctl"))
+        lnl, npar, x = fixtures.parse_out(open(os.path.join(d, "out")).read())
+    assert npar == len(ml["mle_x"])
+    assert abs(lnl - ml["mle_lnL"]) <= TOL_LNL
+    assert np.max(np.abs(np.array(x) - np.array(ml["mle_x"]))) <= TOL_X["adjoint"]
+    assert abs(lnl - round(fit.lnL, 6)) < 1e-9
+
+
+def test_driver_fixed_parameters_match_reference_lnl():
+    # in.codeml starting with -1: evaluate only (treesub.c:3925-3927)
+    print("case: driver cfg0 fixed parameters", flush=True)
+    fx, tree, _ = fixtures.load("cfg0")
+    p = fx["fixed_params"]
+    vals = list(fx["true_branch"]) + [p["kappa"], p["p0"], p["p1"], p["w0"], p["w2"]]
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "seq.phy"), "w").write(fx["phylip"])
+        open(os.path.join(d, "tree.nwk"), "w").write(fx["newick"] + "\n")
+        open(os.path.join(d, "in.codeml"), "w").write("-1 " + " ".join(f"{v:.12f}" for v in vals))
+        open(os.path.join(d, "codeml.ctl"), "w").write(fixtures.CTL.format(fix_omega=0,
+                                                                           omega=1.5))
+        codeml.run(os.path.join(d, "codeml.ctl"))
+        lnl, _, _ = fixtures.parse_out(open(os.path.join(d, "out")).read())
+    assert abs(lnl - fx["fixed_lnL"]) <= 2e-6
+
+
+def test_lrt_statistic():
+    print("case: LRT cfg0", flush=True)
+    fx, tree, data = fixtures.load("cfg0")
+    fits = {}
+    for hyp in ("H1", "H0"):
+        fo, om = hyp_args(hyp)
+        fits[hyp] = codeml.fit_model_a(tree, data, fo, om, start=fx["ml"][hyp]["start"])
+    r = codeml.lrt(fits["H0"], fits["H1"])
+    ref = 2 * (fx["ml"]["H1"]["mle_lnL"] - fx["ml"]["H0"]["mle_lnL"])
+    print(f"  2dlnL {r['stat']:.6f} vs reference {ref:.6f}; p {r['p_mixture']:.4f}", flush=True)
+    assert abs(r["stat"] - ref) <= 2 * TOL_LNL
