@@ -15,7 +15,7 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # hipcc -O3 forms packed FP32 ops by itself; they are kept off on purpose
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument",
+    "-Wno-unused-command-line-argument", "-pthread",
 ]
 
 

@@ -6,6 +6,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <thread>
+#include <utility>
 
 namespace bsa {
 
@@ -98,7 +100,11 @@ double mean_rate(double kappa, double omega, const double *pi)
 // Automatic Computation vol. II, procedures tred2 and tql2).
 static bool sym_eigen(int n, double *z, double *d, double *e)
 {
-#define Z(i, j) z[(i) * n + (j)]
+    // The reduction reads and updates columns of the (symmetric) input and
+    // of the accumulated transform; it runs on the transposed view so that
+    // those are contiguous rows.  Afterwards row v of the buffer holds
+    // column v of the transform, which is the layout the QL rotations want.
+#define Z(i, j) z[(j) * n + (i)]
     for (int j = 0; j < n; j++) d[j] = Z(n - 1, j);
     for (int i = n - 1; i > 0; i--) {
         double scale = 0, h = 0;
@@ -169,6 +175,9 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
     }
     Z(n - 1, n - 1) = 1;
     e[0] = 0;
+#undef Z
+    // ZT(v, k) = component k of vector v
+#define ZT(v, k) z[(v) * n + (k)]
 
     // QL with implicit shifts
     for (int i = 1; i < n; i++) e[i - 1] = e[i];
@@ -210,10 +219,11 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
                     c = p / r;
                     p = c * d[i] - s * g;
                     d[i + 1] = h + s * (c * g + s * d[i]);
+                    double *zi = &ZT(i, 0), *zi1 = &ZT(i + 1, 0);
                     for (int k = 0; k < n; k++) {
-                        h = Z(k, i + 1);
-                        Z(k, i + 1) = s * Z(k, i) + c * h;
-                        Z(k, i) = c * Z(k, i) - s * h;
+                        const double a = zi[k], b = zi1[k];
+                        zi1[k] = s * a + c * b;
+                        zi[k] = c * a - s * b;
                     }
                 }
                 p = -s * s2 * c3 * el1 * e[l] / dl1;
@@ -233,15 +243,14 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
         if (k != i) {
             d[k] = d[i];
             d[i] = p;
-            for (int j = 0; j < n; j++) {
-                double t = Z(j, i);
-                Z(j, i) = Z(j, k);
-                Z(j, k) = t;
-            }
+            for (int j = 0; j < n; j++) std::swap(ZT(i, j), ZT(k, j));
         }
     }
+    // back to eigenvectors in columns
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) std::swap(ZT(i, j), ZT(j, i));
     return true;
-#undef Z
+#undef ZT
 }
 
 // Q = U diag(root) V with UV = I, via the symmetric S = Pi^1/2 Q Pi^-1/2;
@@ -301,13 +310,19 @@ bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, con
     m->Qfactor[0] = 1 / m->mr[0];
     m->Qfactor[1] = 1 / m->mr[1];
     if (!need_eigen) return true;
+    // the three eigen systems are independent: two on helper threads
     const double w[3] = {w0, 1.0, w2};
-    for (int s = 0; s < 3; s++) {
+    bool ok[3] = {false, false, false};
+    auto one = [&](int s) {
         double Q[kS * kS];
         build_q(kappa, w[s], pi, Q);
-        if (!eigen_rev(Q, pi, m->root[s], m->U[s], m->V[s])) return false;
-    }
-    return true;
+        ok[s] = eigen_rev(Q, pi, m->root[s], m->U[s], m->V[s]);
+    };
+    std::thread t1(one, 1), t2(one, 2);
+    one(0);
+    t1.join();
+    t2.join();
+    return ok[0] && ok[1] && ok[2];
 }
 
 // <Z, dQ/dkappa> and <Z, dQ/domega> at Q(kappa, omega); Z has row stride 64.
