@@ -128,6 +128,21 @@ def roofline(work_bytes, work_flops, ms_sum, nev, launches, kernel):
             "fp64_tflops": tf, "fp64_frac": tf / FP64_PEAK_TFLOPS}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured by the rocprofv3 PMC runs
+    summarised under profiles/ for this same workload, or None."""
+    import glob
+    import re
+    short = kernel.split("<")[0]
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+([0-9.eE+]+)", line)
+                if m and m.group(1) == short:
+                    return float(m.group(2)), os.path.relpath(path, ROOT)
+    return None
+
+
 def genome_scan(ngenes, workers, precision):
     """genes/hour of the genome scan (H1 + H0 fits and LRT per gene) on a
     sample of the 2,000-gene workload."""
@@ -205,6 +220,11 @@ def main():
                          f"k_adj_m<{sto}> (branch matrices, FP64 arithmetic)")
         rls += [(tm["sum_adjoint"], adj_rl), (tm["sum_branch"], br_rl)]
     rl = max(rls, key=lambda r: r[0])[1]
+    default_workload = (args.taxa, args.codons, args.seed, args.precision, args.gradient,
+                        args.kernel, args.tile) == (128, 10000, 2, "fp32", "adjoint", "mfma", 0)
+    tr = measured_traffic(rl["kernel"]) if default_workload else None
+    if tr:
+        rl["traffic"], rl["traffic_source"] = tr
     value = args.steps / elapsed
     out = {
         "metric": "lnL+gradient evals/sec at 128 taxa x 10k codons (branch-site model A)",

@@ -509,7 +509,9 @@ static int run_device(bsa_engine *e, bool adjoint)
         e->timing[4] = ms;
         HIPCHK(hipEventElapsedTime(&ms, e->ev[4], e->ev[5]));
         e->timing[5] = ms;
-        e->n_adj_launch += (double)e->adj_off.size();
+        int nl = 1;   // the branch-matrix launch, plus one per depth with interior children
+        for (int v : e->adj_nint) nl += v > 0;
+        e->n_adj_launch += nl;
     }
     for (int i = 0; i < 6; i++) e->timing_sum[i] += e->timing[i];
     e->n_launch += (double)(e->level_off.size() - 1);
