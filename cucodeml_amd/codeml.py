@@ -190,6 +190,59 @@ def format_out(tree, data, fit, names):
     return "\n".join(lines) + "\n"
 
 
+def format_site_table(title, post, data, names):
+    """Per-site class posteriors in the layout of the reference's rst file
+    (PrintProbNSsites, codeml.c:4975-5016)."""
+    lines = [title, f"(amino acids refer to 1st sequence: {names[0]})", ""]
+    for i in range(data.ls):
+        h = data.pose[i]
+        p = post[:, h]
+        best = 0
+        for k in range(1, p.shape[0]):
+            if p[k] > p[best]:
+                best = k
+        lines.append(f"{i + 1:4d} {formats.site_amino_acid(data, 0, i)}  "
+                     + "".join(f" {v:5.5f}" for v in p) + f" ({best + 1:2d})")
+    return "\n".join(lines) + "\n"
+
+
+def positive_sites(post, data, cutoff=0.5):
+    """(site, amino acid, P(w2 > 1), stars) for sites with P(class 2a or 2b)
+    above the cutoff (codeml.c:6786-6799): * above 0.95, ** above 0.99."""
+    out = []
+    for i in range(data.ls):
+        t = post[2, data.pose[i]] + post[3, data.pose[i]]
+        if t > cutoff:
+            out.append((i + 1, formats.site_amino_acid(data, 0, i), t,
+                        "**" if t > 0.99 else ("*" if t > 0.95 else "")))
+    return out
+
+
+def format_beb_out(beb, data):
+    """BEB part of the out file (codeml.c:6782-6822)."""
+    lines = ["", "Bayes Empirical Bayes (BEB) analysis (Yang, Wong & Nielsen 2005. "
+             "Mol. Biol. Evol. 22:1107-1118)", "Positive sites for foreground lineages Prob(w>1):"]
+    for site, aa, t, sig in positive_sites(beb["post"], data):
+        lines.append(f"{site:6d} {aa} {t:.3f}{sig}")
+    grid = {"w0": (np.arange(10) + 0.5) / 10, "w2": 1 + (np.arange(10) + 0.5)}
+    lines += ["", "", "The grid (see ternary graph for p0-p1)", ""]
+    for k in ("w0", "w2"):
+        lines.append(f"{k}: " + "".join(f" {v:6.3f}" for v in grid[k]))
+    lines += ["", "", "Posterior on the grid", ""]
+    for k in ("w0", "w2"):
+        lines.append(f"{k}: " + "".join(f" {v:6.3f}" for v in beb["post_" + k]))
+    lines += ["", "Posterior for p0-p1 (see the ternary graph) (YWN2015, fig. 1)", ""]
+    row = []
+    for k, v in enumerate(beb["post_p0p1"]):
+        row.append(f" {v:5.3f}")
+        if int(math.isqrt(k + 1)) ** 2 == k + 1:
+            lines.append("".join(row))
+            row = []
+    lines.append(f"sum of density on p0-p1 = {np.sum(beb['post_p0p1']):10.6f}")
+    lines.append(f"log(fX) = {beb['log_fX']:12.6f}")
+    return "\n".join(lines) + "\n"
+
+
 def read_in_codeml(path):
     """(fixed, values) from an in.codeml file (treesub.c:3925-3927)."""
     with open(path) as f:
@@ -223,11 +276,12 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
     eng = engine.BranchSiteEngine(tree, data, precision=precision, device=device,
                                   fix_omega=fix_omega, omega_fix=omega)
     try:
-        start = None
+        start, fixed_run = None, False
         inpath = os.path.join(base, "in.codeml")
         if os.path.exists(inpath):
             fixed, vals = read_in_codeml(inpath)
             if fixed:
+                fixed_run = True
                 eng.set_model(fix_omega, omega, p_transformed=False)
                 negl = eng.lnl(vals[:eng.np])
                 par = eng.last_params()
@@ -235,19 +289,34 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
                 fit = Fit("H0" if fix_omega else "H1", -negl, vals[:eng.np].copy(), par[0],
                           par[1], par[2], par[3], par[4], par[7:11].copy(), vals[:nb].copy(),
                           eng.np, 0, 1, True, 0.0)
+                xe = vals[:eng.np].copy()
                 fit.x = engine.x_from_params(vals[:nb], par[0], par[1], par[2], par[3],
                                              None if fix_omega else par[4])
-                text = format_out(tree, data, fit, names)
-                with open(os.path.join(base, ctl.get("outfile", "out")), "w") as f:
-                    f.write(text)
-                return fit
-            start = vals[:eng.np]
-        fit = fit_model_a(tree, data, fix_omega, omega, start=start, gradient=gradient,
-                          kappa=kappa, eng=eng)
+            else:
+                start = vals[:eng.np]
+        if not fixed_run:
+            fit = fit_model_a(tree, data, fix_omega, omega, start=start, gradient=gradient,
+                              kappa=kappa, eng=eng)
+            xe = fit.x
+        # site posteriors under H1, as the reference prints after the fit
+        # (NEB codeml.c:5079-5129, BEB 6617-6826)
+        if not fix_omega:
+            neb = eng.neb(xe)
+            beb = eng.beb(xe)
+            fit.extra["neb"], fit.extra["beb"] = neb, beb
     finally:
         eng.close()
+    text = format_out(tree, data, fit, names)
+    if not fix_omega:
+        text += format_beb_out(fit.extra["beb"], data)
+        with open(os.path.join(base, "rst"), "w") as f:
+            f.write(format_site_table("Naive Empirical Bayes (NEB) probabilities for 4 classes",
+                                      fit.extra["neb"], data, names))
+            f.write("\n\n")
+            f.write(format_site_table("Bayes Empirical Bayes (BEB) probabilities for 4 classes "
+                                      "(class)", fit.extra["beb"]["post"], data, names))
     with open(os.path.join(base, ctl.get("outfile", "out")), "w") as f:
-        f.write(format_out(tree, data, fit, names))
+        f.write(text)
     return fit
 
 

@@ -533,10 +533,24 @@ static int evaluate(bsa_engine *e, const double *x, double *lnl, bool adjoint = 
     return rc;
 }
 
+static int evaluate_natural(bsa_engine *e, const double *x, double kappa, double p0, double p1,
+                            double w0, double w2, const double *qf_fixed, double *lnl,
+                            bool adjoint);
+
 static int evaluate_inner(bsa_engine *e, const double *x, double *lnl, bool adjoint)
 {
     double kappa, p0, p1, w0, w2;
     natural_params(e, x, &kappa, &p0, &p1, &w0, &w2);
+    return evaluate_natural(e, x, kappa, p0, p1, w0, w2, nullptr, lnl, adjoint);
+}
+
+// One evaluation at natural parameters; branch lengths from x.  qf_fixed,
+// when given, replaces the two Qfactors the model would give (BEB keeps them
+// at the MLE, GetPMatBranch treesub.c:7289-7301).
+static int evaluate_natural(bsa_engine *e, const double *x, double kappa, double p0, double p1,
+                            double w0, double w2, const double *qf_fixed, double *lnl,
+                            bool adjoint)
+{
     for (int b = 0; b < e->nbranch; b++)
         if (!std::isfinite(x[b])) return fail(BSA_EINVAL, "branch length %d not finite", b);
     if (!(kappa > 0) || !(w0 >= 0) || !(w2 >= 0) || !(p0 >= 0) || !(p1 >= 0) || !(p0 + p1 > 0) ||
@@ -570,6 +584,10 @@ static int evaluate_inner(bsa_engine *e, const double *x, double *lnl, bool adjo
                 }
             for (int k = 0; k < kS; k++) R[k] = e->model.root[s][k];
         }
+    }
+    if (qf_fixed) {
+        e->model.Qfactor[0] = qf_fixed[0];
+        e->model.Qfactor[1] = qf_fixed[1];
     }
     for (size_t i = 0; i < e->pjobs.size(); i++) {
         PmatJob p = e->pjobs[i];
@@ -852,6 +870,177 @@ extern "C" int bsa_lnl_grad_fd(bsa_engine *e, const double *x, int np, int centr
     if ((rc = evaluate(e, x, &f0))) return rc;
     *neg_lnl = -f0;
     return BSA_OK;
+}
+
+// Bayes empirical Bayes for model A (lfunNSsites_ACD, codeml.c:6617-6826;
+// get_grid_para_like_ACD 6419-6607; get_pclassM_iw_ACD 6609-6615 with
+// GetIndexTernary, tools.c:4441-4461).  Branch lengths, kappa and the two
+// Qfactors are fixed at x; the 121 w sets (w0 on 10 points in (0, 1), w2 on
+// 10 points in (1, 11), w = 1) come from 100 forward passes with
+// (w0, w2) on the 10 x 10 grid (class 2a gives (w0, w2), class 0 (w0, w0),
+// class 2b (1, w2), class 1 (1, 1)); the 10^4-point grid over
+// (p0-p1 triangle, w0, w2) is integrated on the GPU.
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+}  // namespace
+
+extern "C" int bsa_beb(bsa_engine *e, const double *x, int np, double *post, double *ml3,
+                       double *postw20, double *postp0p1, double *lnf_sets)
+{
+    if (!e || !x || !post) return fail(BSA_EINVAL, "null argument");
+    if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    HIPCHK(hipSetDevice(e->dev));
+    constexpr int n1d = 10, dim = 4, ngrid = 10000, nset = n1d + 1 + n1d * n1d + n1d;
+    const int npatt = e->npatt, npad = e->npad;
+    double kappa, p0, p1, w0, w2;
+    natural_params(e, x, &kappa, &p0, &p1, &w0, &w2);
+    ModelA mle;
+    if (!setup_model_a(kappa, p0, p1, w0, w2, e->pi.data(), &mle, false))
+        return fail(BSA_EINVAL, "parameters out of range for BEB");
+    const double qf[2] = {mle.Qfactor[0], mle.Qfactor[1]};
+    double w0g[n1d], w2g[n1d];
+    for (int i = 0; i < n1d; i++) {
+        w0g[i] = (i + 0.5) / n1d;
+        w2g[i] = 1.0 + (i + 0.5) * (11.0 - 1.0) / n1d;
+    }
+    DevBuf F, mx, pc, iw, lnfx, gl, W, dpost;
+    HIPCHK(hipMalloc(&F.p, sizeof(double) * nset * (size_t)npad));
+    hipStream_t s = e->stream;
+    double *dF = (double *)F.p;
+    auto row = [&](int set) { return dF + (size_t)set * npad; };
+    for (int i = 0; i < n1d; i++)
+        for (int j = 0; j < n1d; j++) {
+            double l;
+            int rc = evaluate_natural(e, x, kappa, p0, p1, w0g[i], w2g[j], qf, &l, false);
+            if (rc) return rc;
+            const size_t rb = sizeof(double) * npad;
+            const double *fh = e->d_fh;
+            HIPCHK(hipMemcpyAsync(row(n1d + 1 + i * n1d + j), fh + 2 * (size_t)npad, rb,
+                                  hipMemcpyDeviceToDevice, s));
+            if (j == 0)
+                HIPCHK(hipMemcpyAsync(row(i), fh, rb, hipMemcpyDeviceToDevice, s));
+            if (i == 0)
+                HIPCHK(hipMemcpyAsync(row(n1d + 1 + n1d * n1d + j), fh + 3 * (size_t)npad, rb,
+                                      hipMemcpyDeviceToDevice, s));
+            if (i == 0 && j == 0)
+                HIPCHK(hipMemcpyAsync(row(n1d), fh + (size_t)npad, rb, hipMemcpyDeviceToDevice, s));
+        }
+    if (lnf_sets) {
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < nset; k++)
+            HIPCHK(hipMemcpy(lnf_sets + (size_t)k * npatt, row(k), sizeof(double) * npatt,
+                             hipMemcpyDeviceToHost));
+    }
+    // normalise per pattern; S1 = sum_h fpatt_h max_k log f
+    HIPCHK(hipMalloc(&mx.p, sizeof(double) * npad));
+    launch_beb_norm(s, dF, nset, npatt, npad, (double *)mx.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hmx(npatt);
+    HIPCHK(hipMemcpyAsync(hmx.data(), mx.p, sizeof(double) * npatt, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double S1 = 0, S1c = 0;
+    for (int h = 0; h < npatt; h++) {
+        const double v = e->fpatt[h] * hmx[h], t = S1 + v;
+        S1c += std::fabs(S1) >= std::fabs(v) ? (S1 - t) + v : (v - t) + S1;
+        S1 = t;
+    }
+    S1 += S1c;
+    // grid: ip = (p0-p1 triangle row, column, w0 index, w2 index)
+    std::vector<double> hpc(4 * ngrid);
+    std::vector<int> hiw(4 * ngrid);
+    for (int g = 0; g < ngrid; g++) {
+        const int ip0 = g / 1000, ip1 = (g / 100) % 10, ip2 = (g / 10) % 10, ip3 = g % 10;
+        const int tri = ip0 * n1d + ip1;
+        const int ix = (int)std::sqrt((double)tri), iy = tri - ix * ix;
+        const double q0 = (1 + (iy / 2) * 3 + (iy % 2)) / (3.0 * n1d);
+        const double q1 = (1 + (n1d - 1 - ix) * 3 + (iy % 2)) / (3.0 * n1d);
+        const double q2 = 1 - q0 - q1;
+        hpc[g * 4 + 0] = q0;
+        hpc[g * 4 + 1] = q1;
+        hpc[g * 4 + 2] = q2 * q0 / (1 - q2);
+        hpc[g * 4 + 3] = q2 * q1 / (1 - q2);
+        hiw[g * 4 + 0] = ip2;
+        hiw[g * 4 + 1] = n1d;
+        hiw[g * 4 + 2] = n1d + 1 + ip2 * n1d + ip3;
+        hiw[g * 4 + 3] = n1d + 1 + n1d * n1d + ip3;
+    }
+    HIPCHK(hipMalloc(&pc.p, sizeof(double) * 4 * ngrid));
+    HIPCHK(hipMalloc(&iw.p, sizeof(int) * 4 * ngrid));
+    HIPCHK(hipMalloc(&lnfx.p, sizeof(double) * ngrid));
+    HIPCHK(hipMemcpyAsync(pc.p, hpc.data(), sizeof(double) * 4 * ngrid, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(iw.p, hiw.data(), sizeof(int) * 4 * ngrid, hipMemcpyHostToDevice, s));
+    launch_beb_lnfx(s, dF, npad, npatt, e->d_fpatt, (const double *)pc.p, (const int *)iw.p, ngrid,
+                    (double *)lnfx.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hl(ngrid);
+    HIPCHK(hipMemcpyAsync(hl.data(), lnfx.p, sizeof(double) * ngrid, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // marginal likelihood and posterior on the grid, accumulated in the
+    // reference's order with its rescaling (codeml.c:6724-6750)
+    double fX = 1, S2 = -1e300, pp[dim][n1d] = {}, p01[n1d * n1d] = {};
+    for (int g = 0; g < ngrid; g++) {
+        const int ip[4] = {g / 1000, (g / 100) % 10, (g / 10) % 10, g % 10};
+        double t = hl[g] - S2;
+        if (t > 10) {
+            t = t < 200 ? std::exp(-t) : 0;
+            fX = fX * t + 1;
+            for (int j = 0; j < dim; j++)
+                for (int k = 0; k < n1d; k++) pp[j][k] *= t;
+            for (int k = 0; k < n1d * n1d; k++) p01[k] *= t;
+            for (int j = 0; j < dim; j++) pp[j][ip[j]] += 1;
+            p01[ip[0] * n1d + ip[1]] += 1;
+            S2 = hl[g];
+        } else if (t > -200) {
+            t = std::exp(t);
+            fX += t;
+            for (int j = 0; j < dim; j++) pp[j][ip[j]] += t;
+            p01[ip[0] * n1d + ip[1]] += t;
+        }
+    }
+    for (int j = 0; j < dim; j++)
+        for (int k = 0; k < n1d; k++) pp[j][k] /= fX;
+    for (int k = 0; k < n1d * n1d; k++) p01[k] /= fX;
+    const double lfX = std::log(fX) + S2;
+    // grid points whose weight exp(lnfX - log f(X)) is representable
+    std::vector<int> hg;
+    std::vector<double> hw;
+    for (int g = 0; g < ngrid; g++) {
+        const double L = hl[g] - lfX;
+        if (L > -745) {
+            hg.push_back(g);
+            hw.push_back(std::exp(L));
+        }
+    }
+    const int ng = (int)hg.size();
+    HIPCHK(hipMalloc(&gl.p, sizeof(int) * std::max(ng, 1)));
+    HIPCHK(hipMalloc(&W.p, sizeof(double) * std::max(ng, 1)));
+    HIPCHK(hipMalloc(&dpost.p, sizeof(double) * 4 * (size_t)npatt));
+    HIPCHK(hipMemcpyAsync(gl.p, hg.data(), sizeof(int) * ng, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(W.p, hw.data(), sizeof(double) * ng, hipMemcpyHostToDevice, s));
+    launch_beb_post(s, dF, npad, npatt, (const double *)pc.p, (const int *)iw.p,
+                    (const int *)gl.p, (const double *)W.p, ng, (double *)dpost.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(post, dpost.p, sizeof(double) * 4 * (size_t)npatt, hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ml3) {
+        ml3[0] = lfX + S1 - dim * std::log((double)n1d);   // log f(X) as the reference prints it
+        ml3[1] = S1;
+        ml3[2] = lfX;
+    }
+    if (postw20)
+        for (int k = 0; k < n1d; k++) {
+            postw20[k] = pp[2][k];
+            postw20[n1d + k] = pp[3][k];
+        }
+    if (postp0p1)
+        for (int k = 0; k < n1d * n1d; k++) postp0p1[k] = p01[k];
+    // leave the engine's last evaluation at x
+    double l;
+    return evaluate(e, x, &l);
 }
 
 extern "C" int bsa_class_lnl(bsa_engine *e, double *fhK, double *site)

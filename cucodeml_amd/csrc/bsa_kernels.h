@@ -115,4 +115,14 @@ void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, con
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,
                        const float *Af, const float *Bf, float *D32);
 
+// Bayes empirical Bayes: per-pattern normalisation of the w-set table
+// F[nset][npad] (log f in, exp(log f - max) out, max in mx); log f(X | g)
+// for every grid point; per-pattern class posteriors over the grid points in
+// glist with weights W.  pc/iw: [ngrid][4] class proportions and set indices.
+void launch_beb_norm(hipStream_t s, double *F, int nset, int npatt, int npad, double *mx);
+void launch_beb_lnfx(hipStream_t s, const double *F, int npad, int npatt, const double *fpatt,
+                     const double *pc, const int *iw, int ngrid, double *lnfx);
+void launch_beb_post(hipStream_t s, const double *F, int npad, int npatt, const double *pc,
+                     const int *iw, const int *glist, const double *W, int ng, double *post);
+
 }  // namespace bsa

@@ -950,4 +950,100 @@ template void launch_root<float>(hipStream_t, const RootArgs &, const float *, c
                                  const double *, const double *, double *, double *, double *,
                                  int, double *, float *, int *);
 
+// ------------------------------------------------------------------ Bayes empirical Bayes
+
+// Per pattern: m_h = max over the w sets of log f(x_h | w), F[s][h] =
+// exp(log f - m_h) in place (get_grid_para_like_ACD, codeml.c:6600-6606).
+__global__ __launch_bounds__(256) void k_beb_norm(double *__restrict__ F, int nset, int npatt,
+                                                  int npad, double *__restrict__ mx)
+{
+    const int h = blockIdx.x * 256 + threadIdx.x;
+    if (h >= npatt) return;
+    double m = F[h];
+    for (int s = 1; s < nset; s++) m = fmax(m, F[(size_t)s * npad + h]);
+    for (int s = 0; s < nset; s++) F[(size_t)s * npad + h] = exp(F[(size_t)s * npad + h] - m);
+    mx[h] = m;
+}
+
+// One block per grid point g: lnfX[g] = sum_h fpatt_h log(sum_k pc[g][k]
+// F[iw[g][k]][h]); patterns with a sum below 1e-300 are left out, as in the
+// reference (lfunNSsites_ACD, codeml.c:6713-6722).  Fixed summation order.
+__global__ __launch_bounds__(256) void k_beb_lnfx(const double *__restrict__ F, int npad,
+                                                  int npatt, const double *__restrict__ fpatt,
+                                                  const double *__restrict__ pc,
+                                                  const int *__restrict__ iw,
+                                                  double *__restrict__ lnfx)
+{
+    __shared__ double sred[4];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    double p[4];
+    const double *Fk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        p[k] = pc[g * 4 + k];
+        Fk[k] = F + (size_t)iw[g * 4 + k] * npad;
+    }
+    double acc = 0;
+    for (int h = tid; h < npatt; h += 256) {
+        double fh = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) fh += p[k] * Fk[k][h];
+        if (fh >= 1e-300) acc += log(fh) * fpatt[h];
+    }
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) sred[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) lnfx[g] = ((sred[0] + sred[1]) + sred[2]) + sred[3];
+}
+
+// Per pattern: post[k][h] = sum over listed grid points of
+// (pc[g][k] F_k / sum_k' pc[g][k'] F_k') W_g, W_g = exp(lnfX[g] - log f(X))
+// (codeml.c:6759-6769), grid points in ascending order.
+__global__ __launch_bounds__(256) void k_beb_post(const double *__restrict__ F, int npad,
+                                                  int npatt, const double *__restrict__ pc,
+                                                  const int *__restrict__ iw,
+                                                  const int *__restrict__ glist,
+                                                  const double *__restrict__ W, int ng,
+                                                  double *__restrict__ post)
+{
+    const int h = blockIdx.x * 256 + threadIdx.x;
+    if (h >= npatt) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int i = 0; i < ng; i++) {
+        const int g = glist[i];
+        double f[4], fh = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            f[k] = pc[g * 4 + k] * F[(size_t)iw[g * 4 + k] * npad + h];
+            fh += f[k];
+        }
+        if (!(fh > 0)) continue;
+        const double w = W[i] / fh;
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] += f[k] * w;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) post[(size_t)k * npatt + h] = a[k];
+}
+
+void launch_beb_norm(hipStream_t s, double *F, int nset, int npatt, int npad, double *mx)
+{
+    hipLaunchKernelGGL(k_beb_norm, dim3((npatt + 255) / 256), dim3(256), 0, s, F, nset, npatt, npad,
+                       mx);
+}
+
+void launch_beb_lnfx(hipStream_t s, const double *F, int npad, int npatt, const double *fpatt,
+                     const double *pc, const int *iw, int ngrid, double *lnfx)
+{
+    hipLaunchKernelGGL(k_beb_lnfx, dim3(ngrid), dim3(256), 0, s, F, npad, npatt, fpatt, pc, iw,
+                       lnfx);
+}
+
+void launch_beb_post(hipStream_t s, const double *F, int npad, int npatt, const double *pc,
+                     const int *iw, const int *glist, const double *W, int ng, double *post)
+{
+    hipLaunchKernelGGL(k_beb_post, dim3((npatt + 255) / 256), dim3(256), 0, s, F, npad, npatt, pc,
+                       iw, glist, W, ng, post);
+}
+
 }  // namespace bsa

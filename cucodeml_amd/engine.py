@@ -23,7 +23,8 @@ BSA_KERNEL_MFMA, BSA_KERNEL_VALU = 0, 1
 EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
             "bsa_last_error", "bsa_version", "bsa_debug_mfma_probe", "bsa_work",
-            "bsa_model_a_setup", "bsa_default_bounds", "bsa_fit", "bsa_debug_minimize"]
+            "bsa_model_a_setup", "bsa_default_bounds", "bsa_fit", "bsa_debug_minimize",
+            "bsa_beb"]
 
 
 class BsaError(RuntimeError):
@@ -80,6 +81,7 @@ def lib():
         L.bsa_version.restype = C.c_char_p
         L.bsa_debug_mfma_probe.argtypes = [C.c_int, dp, dp, dp, dp]
         L.bsa_default_bounds.argtypes = [C.c_void_p, dp, dp]
+        L.bsa_beb.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp, dp, C.c_void_p]
         L.bsa_fit.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, C.POINTER(_FitOpts), dp,
                               C.POINTER(C.c_int)]
         L.bsa_debug_minimize.argtypes = [C.c_int, _OBJECTIVE, C.c_void_p, dp, dp, dp,
@@ -207,6 +209,32 @@ class BranchSiteEngine:
                              C.byref(f), info))
         return x, f.value, {"iterations": info[0], "evaluations": info[1],
                             "converged": bool(info[2])}
+
+    def beb(self, x, sets=False):
+        """Bayes empirical Bayes at x (bsa_beb): dict with the class
+        posteriors per pattern "post" (4, npatt), "log_fX", the posteriors on
+        the w0 and w2 grids and on the p0-p1 triangles, and with sets=True
+        the 121-set table "lnf_sets" (121, npatt)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = self.data.npatt
+        post, ml, pw, p01 = np.zeros(4 * n), np.zeros(3), np.zeros(20), np.zeros(100)
+        tab = np.zeros(121 * n) if sets else None
+        _check(lib().bsa_beb(self._h, _dptr(x), len(x), _dptr(post), _dptr(ml), _dptr(pw),
+                             _dptr(p01), None if tab is None else tab.ctypes.data))
+        out = {"post": post.reshape(4, n), "log_fX": float(ml[0]), "S1": float(ml[1]),
+               "log_grid_sum": float(ml[2]), "post_w0": pw[:10], "post_w2": pw[10:],
+               "post_p0p1": p01}
+        if sets:
+            out["lnf_sets"] = tab.reshape(121, n)
+        return out
+
+    def neb(self, x):
+        """Naive empirical Bayes class posteriors per pattern (4, npatt) at x:
+        p_k f(x_h | k) / f(x_h) (lfunNSsites_rate, codeml.c:5079-5129)."""
+        self.lnl(x)
+        fh, site = self.fhK()
+        fk = self.last_params()[7:11]
+        return fk[:, None] * np.exp(fh - site[None, :])
 
     def fhK(self):
         """(per-class log f (4, npatt), per-pattern mixture log f (npatt,))"""

@@ -229,6 +229,7 @@ class CodonData:
     amb_triplets: list     # triplet string of each ambiguity code 61..
     amb_sets: list         # sense codons each ambiguity code stands for
     pi: np.ndarray = None  # (61,) codon frequencies
+    pose: np.ndarray = None  # (ls,) pattern index of every site
 
     @property
     def ncodes(self):
@@ -241,6 +242,20 @@ class CodonData:
         for a, s in enumerate(self.amb_sets):
             m[codons.NSENSE + a, s] = 1.0
         return m
+
+
+def site_amino_acid(data, species, site):
+    """Amino acid of `species` at `site` as codeml prints it (GetAASiteSpecies,
+    codeml.c:4941-4971): '-' for a codon that allows every sense codon, '*'
+    when the codons it allows code for more than one amino acid."""
+    c = int(data.z[species, data.pose[site]])
+    if c < codons.NSENSE:
+        return str(codons.AA61[c])
+    allowed = data.amb_sets[c - codons.NSENSE]
+    if len(allowed) == codons.NSENSE:
+        return "-"
+    aas = {str(codons.AA61[k]) for k in allowed}
+    return aas.pop() if len(aas) == 1 else "*"
 
 
 def encode_codons(names, seqs):
@@ -281,10 +296,12 @@ def encode_codons(names, seqs):
     return z, amb_triplets, amb_sets
 
 
-def compress_patterns(z):
-    """Collapse identical columns of z (ns, ls) into (ns, npatt) + counts."""
-    cols, counts = np.unique(z.T, axis=0, return_counts=True)
-    return np.ascontiguousarray(cols.T), counts.astype(np.float64)
+def compress_patterns(z, return_pose=False):
+    """Collapse identical columns of z (ns, ls) into (ns, npatt) + counts
+    (+ the pattern of every site, codeml's pose, with return_pose)."""
+    cols, inv, counts = np.unique(z.T, axis=0, return_inverse=True, return_counts=True)
+    out = (np.ascontiguousarray(cols.T), counts.astype(np.float64))
+    return out + (inv.ravel().astype(np.int64),) if return_pose else out
 
 
 def f3x4_frequencies(z, fpatt, amb_triplets, niter=20):
@@ -374,8 +391,8 @@ def f3x4_frequencies(z, fpatt, amb_triplets, niter=20):
 
 def load_codon_data(names, seqs):
     z, trip, sets = encode_codons(names, seqs)
-    zp, fpatt = compress_patterns(z)
+    zp, fpatt, pose = compress_patterns(z, return_pose=True)
     d = CodonData(names=list(names), ns=len(names), ls=z.shape[1], npatt=zp.shape[1],
-                  z=zp, fpatt=fpatt, amb_triplets=trip, amb_sets=sets)
+                  z=zp, fpatt=fpatt, amb_triplets=trip, amb_sets=sets, pose=pose)
     d.pi = f3x4_frequencies(zp, fpatt, trip)
     return d

@@ -117,6 +117,21 @@ int  bsa_lnl_grad_fd(bsa_engine *eng, const double *x, int np, int central, doub
  * [npatt]; either pointer may be NULL */
 int  bsa_class_lnl(bsa_engine *eng, double *fhK, double *site_lnf);
 
+/* Bayes empirical Bayes posteriors of the four site classes for model A at
+ * x (normally the H1 estimates): branch lengths, kappa and the two Qfactors
+ * fixed at x; w0 ~ U(0, 1), w2 ~ U(1, 11) and (p0, p1, p2) ~ Dirichlet(1, 1,
+ * 1) on a 10-point grid per dimension (lfunNSsites_ACD, codeml.c:6617-6826,
+ * which the reference runs after every H1 fit, codeml.c:5215-5216).
+ * post[4*npatt]: class-major posteriors per site pattern (classes 0, 1, 2a,
+ * 2b; P(w2 > 1) = post[2] + post[3]); ml3 (may be NULL) = {log f(X) as the
+ * reference prints it, the per-site scale S1, log of the grid sum};
+ * postw20 (may be NULL) = posterior of w0 on its 10 grid points, then of w2;
+ * postp0p1 (may be NULL) = [100] posterior on the p0-p1 triangles;
+ * lnf_sets (may be NULL) = [121*npatt] log f(x_h | w set) in the
+ * reference's set order.  The engine's last evaluation is left at x. */
+int  bsa_beb(bsa_engine *eng, const double *x, int np, double *post, double *ml3,
+             double *postw20, double *postp0p1, double *lnf_sets);
+
 /* the parameters of the last evaluation in natural form:
  * out[0..4] = kappa, p0, p1, w0, w2; out[5..6] = Qfactor background,
  * foreground; out[7..10] = class proportions */

@@ -17,6 +17,7 @@
  *   orc_model      SetParametersNSsites, model A   codeml.c:2470-2520
  *                  (class proportions, two Qfactors, three U/V/Root sets)
  *   orc_lnl        fx_r + ConditionalPNode + NodeScale + lfundG
+ *   orc_lnl_qf     the same with the Qfactors given (BEB, treesub.c:7289-7301)
  *                  treesub.c:7485-7550, codeml.c:3402-3458,
  *                  treesub.c:6917-6968, treesub.c:7342-7394
  *   orc_scale_marks SetNodeScale (every=15 for codons) treesub.c:6917-6937
@@ -318,11 +319,30 @@ static void cond_node(orc_ctx *c, int v)
  * fpatt[npatt], pi[61].  Outputs (optional): fhK[4*npatt] per-class
  * log f(x_h | class) and site_lnf[npatt] mixture log f_h.  Returns lnL.
  */
+double orc_lnl_qf(int ns, int nnode, int root, int maxson, const int *nson, const int *sons,
+                  const int *label, const double *branch, int npatt, const unsigned char *z,
+                  int ncodes, const double *codemask, const double *fpatt, const double *pi,
+                  double kappa, double p0, double p1, double w0, double w2, int use_scaling,
+                  const double *qf_fixed, double *fhK, double *site_lnf);
+
 double orc_lnl(int ns, int nnode, int root, int maxson, const int *nson, const int *sons,
                const int *label, const double *branch, int npatt, const unsigned char *z,
                int ncodes, const double *codemask, const double *fpatt, const double *pi,
                double kappa, double p0, double p1, double w0, double w2, int use_scaling,
                double *fhK, double *site_lnf)
+{
+    return orc_lnl_qf(ns, nnode, root, maxson, nson, sons, label, branch, npatt, z, ncodes,
+                      codemask, fpatt, pi, kappa, p0, p1, w0, w2, use_scaling, NULL, fhK,
+                      site_lnf);
+}
+
+/* As orc_lnl; qf_fixed (if not NULL) replaces the two Qfactors, as the
+ * reference's BEB does with the MLE's (GetPMatBranch, treesub.c:7289-7301). */
+double orc_lnl_qf(int ns, int nnode, int root, int maxson, const int *nson, const int *sons,
+                  const int *label, const double *branch, int npatt, const unsigned char *z,
+                  int ncodes, const double *codemask, const double *fpatt, const double *pi,
+                  double kappa, double p0, double p1, double w0, double w2, int use_scaling,
+                  const double *qf_fixed, double *fhK, double *site_lnf)
 {
     orc_ctx c;
     double freqK[4], Qf[2], *U3, *V3, R3[3 * N61], lnL = 0;
@@ -333,6 +353,10 @@ double orc_lnl(int ns, int nnode, int root, int maxson, const int *nson, const i
     U3 = malloc(sizeof(double) * 3 * n * n);
     V3 = malloc(sizeof(double) * 3 * n * n);
     orc_model(kappa, p0, p1, w0, w2, pi, freqK, Qf, R3, U3, V3);
+    if (qf_fixed) {
+        Qf[0] = qf_fixed[0];
+        Qf[1] = qf_fixed[1];
+    }
 
     mark = calloc(nnode, 1);
     scale_k = calloc(nnode, sizeof(int));

@@ -42,6 +42,8 @@ def lib():
                               _up, C.c_int, _dp, _dp, _dp] + [C.c_double] * 5 + \
                              [C.c_int, C.c_void_p, C.c_void_p]
         L.orc_lnl.restype = C.c_double
+        L.orc_lnl_qf.argtypes = L.orc_lnl.argtypes[:20] + [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_lnl_qf.restype = C.c_double
         _lib = L
     return _lib
 
@@ -92,18 +94,121 @@ def scale_marks(tree):
     return n, np.frombuffer(buf.raw, dtype=np.uint8).copy()
 
 
-def lnl(tree, data, kappa, p0, p1, w0, w2, use_scaling=True, pi=None):
-    """lnL, per-class log f (4, npatt) and per-pattern mixture log f."""
+def lnl(tree, data, kappa, p0, p1, w0, w2, use_scaling=True, pi=None, qfactor=None):
+    """lnL, per-class log f (4, npatt) and per-pattern mixture log f.
+    ``qfactor`` (background, foreground) replaces the model's Qfactors."""
     maxson, nson, sons, lab = _tree_arrays(tree)
     pi = data.pi if pi is None else pi
     fhK = np.zeros(4 * data.npatt)
     site = np.zeros(data.npatt)
     z = np.ascontiguousarray(data.z, dtype=np.uint8)
     mask = np.ascontiguousarray(data.code_table())
-    v = lib().orc_lnl(tree.ns, tree.nnode, tree.root, maxson, nson, sons, lab,
-                      np.ascontiguousarray(tree.branch, dtype=np.float64), data.npatt, z,
-                      mask.shape[0], mask.ravel(), np.ascontiguousarray(data.fpatt),
-                      np.ascontiguousarray(pi, dtype=np.float64),
-                      kappa, p0, p1, w0, w2, int(use_scaling),
-                      fhK.ctypes.data, site.ctypes.data)
+    qf = None if qfactor is None else np.ascontiguousarray(qfactor, dtype=np.float64)
+    v = lib().orc_lnl_qf(tree.ns, tree.nnode, tree.root, maxson, nson, sons, lab,
+                         np.ascontiguousarray(tree.branch, dtype=np.float64), data.npatt, z,
+                         mask.shape[0], mask.ravel(), np.ascontiguousarray(data.fpatt),
+                         np.ascontiguousarray(pi, dtype=np.float64),
+                         kappa, p0, p1, w0, w2, int(use_scaling),
+                         None if qf is None else qf.ctypes.data, fhK.ctypes.data,
+                         site.ctypes.data)
     return v, fhK.reshape(4, data.npatt), site
+
+
+# ---------------------------------------------------------------- site posteriors
+
+BEB_N1D = 10
+
+
+def neb(tree, data, kappa, p0, p1, w0, w2):
+    """Naive empirical Bayes class posteriors per pattern (4, npatt) at the
+    given parameters (lfunNSsites_rate, codeml.c:5079-5129): p_k f_k / f."""
+    _, fh, site = lnl(tree, data, kappa, p0, p1, w0, w2)
+    t = p0 + p1
+    fk = np.array([p0, p1, (1 - t) * p0 / t, (1 - t) * p1 / t])
+    return fk[:, None] * np.exp(fh - site[None, :])
+
+
+def beb_sets(tree, data, kappa, p0, p1, w0, w2):
+    """log f(x_h | w set) for the 121 w sets of model A's BEB, in the
+    reference's order (get_grid_para_like_ACD, codeml.c:6419-6607): branch
+    lengths, kappa and the Qfactors of the given parameters held fixed."""
+    n1d = BEB_N1D
+    fk, qf, *_ = model(kappa, p0, p1, w0, w2, data.pi)
+    w0g = (np.arange(n1d) + 0.5) / n1d
+    w2g = 1 + (np.arange(n1d) + 0.5) * (11 - 1) / n1d
+    F = np.zeros((n1d + 1 + n1d * n1d + n1d, data.npatt))
+    for i in range(n1d):
+        for j in range(n1d):
+            _, fh, _ = lnl(tree, data, kappa, p0, p1, w0g[i], w2g[j], qfactor=qf)
+            F[n1d + 1 + i * n1d + j] = fh[2]
+            if j == 0:
+                F[i] = fh[0]
+            if i == 0:
+                F[n1d + 1 + n1d * n1d + j] = fh[3]
+            if i == 0 and j == 0:
+                F[n1d] = fh[1]
+    return F
+
+
+def beb_grid():
+    """(pclassM[ngrid, 4], iw[ngrid, 4]) of model A's 10^4-point grid
+    (get_pclassM_iw_ACD codeml.c:6609-6615, GetIndexTernary tools.c:4441-4461)."""
+    n1d = BEB_N1D
+    g = np.arange(n1d ** 4)
+    ip = np.stack([g // 1000, (g // 100) % 10, (g // 10) % 10, g % 10], axis=1)
+    tri = ip[:, 0] * n1d + ip[:, 1]
+    ix = np.floor(np.sqrt(tri)).astype(int)
+    iy = tri - ix * ix
+    q0 = (1 + (iy // 2) * 3 + iy % 2) / (3.0 * n1d)
+    q1 = (1 + (n1d - 1 - ix) * 3 + iy % 2) / (3.0 * n1d)
+    q2 = 1 - q0 - q1
+    pc = np.stack([q0, q1, q2 * q0 / (1 - q2), q2 * q1 / (1 - q2)], axis=1)
+    iw = np.stack([ip[:, 2], np.full_like(g, n1d), n1d + 1 + ip[:, 2] * n1d + ip[:, 3],
+                   n1d + 1 + n1d * n1d + ip[:, 3]], axis=1)
+    return pc, iw, ip
+
+
+def beb_integrate(F, fpatt):
+    """The grid integration of lfunNSsites_ACD (codeml.c:6699-6769) on a
+    121-set table of log f: returns (post (4, npatt), log f(X) as printed,
+    posterior on the w0 and w2 grids (2, 10), on the p0-p1 triangles (100,))."""
+    n1d, dim = BEB_N1D, 4
+    mx = F.max(axis=0)
+    S1 = float(np.sum(fpatt * mx))
+    E = np.exp(F - mx[None, :])
+    pc, iw, ip = beb_grid()
+    ngrid = pc.shape[0]
+    lnfx = np.zeros(ngrid)
+    for g in range(ngrid):
+        fh = (pc[g][:, None] * E[iw[g]]).sum(axis=0)
+        ok = fh >= 1e-300
+        lnfx[g] = np.sum(np.log(fh[ok]) * fpatt[ok])
+    fX, S2 = 1.0, -1e300
+    pp = np.zeros((dim, n1d))
+    p01 = np.zeros(n1d * n1d)
+    for g in range(ngrid):
+        t = lnfx[g] - S2
+        if t > 10:
+            t = np.exp(-t) if t < 200 else 0.0
+            fX = fX * t + 1
+            pp *= t
+            p01 *= t
+            pp[np.arange(dim), ip[g]] += 1
+            p01[ip[g, 0] * n1d + ip[g, 1]] += 1
+            S2 = lnfx[g]
+        elif t > -200:
+            t = np.exp(t)
+            fX += t
+            pp[np.arange(dim), ip[g]] += t
+            p01[ip[g, 0] * n1d + ip[g, 1]] += t
+    pp /= fX
+    p01 /= fX
+    lfX = np.log(fX) + S2
+    post = np.zeros((4, F.shape[1]))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for g in range(ngrid):
+            fk = pc[g][:, None] * E[iw[g]]
+            fh = fk.sum(axis=0)
+            t = np.log(fk / fh) + lnfx[g] - lfX
+            post += np.where(t > -300, np.exp(t), 0.0)
+    return post, lfX + S1 - dim * np.log(n1d), pp[2:], p01
