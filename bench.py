@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--scan-genes", type=int, default=32,
                     help="genes of the genome-scan sample timed beside (0: skip)")
     ap.add_argument("--scan-workers", type=int, default=8)
+    ap.add_argument("--no-beb", dest="beb", action="store_false",
+                    help="skip timing one Bayes empirical Bayes analysis beside")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding over site patterns "
@@ -253,6 +255,13 @@ def main():
                     ("pmat", "prune", "root", "adjoint", "branch", "finish", "wall")},
         "lnL": -f,
     }
+    if args.beb:
+        eng.beb(x)
+        t1 = time.perf_counter()
+        eng.beb(x)
+        out["beb"] = {"seconds": time.perf_counter() - t1, "precision": args.precision,
+                      "what": "Bayes empirical Bayes site posteriors at x: 100 forward passes "
+                              "(121 w sets) + 10^4-point grid integration"}
     if args.fd_steps > 0:
         e64 = engine.BranchSiteEngine(tree, data, precision="fp64", kernel=args.kernel)
         e64.gradientB(x)
