@@ -266,8 +266,9 @@ static int upload_static(bsa_engine *e, const bsa_problem *p)
     HIPCHK(hipMalloc(&e->d_prune, sizeof(PruneJob) * std::max<size_t>(1, e->prune.size())));
     HIPCHK(hipMalloc(&e->d_PT, sizeof(double) * 4096 * (size_t)e->nnode * 3));
     HIPCHK(hipMemset(e->d_PT, 0, sizeof(double) * 4096 * (size_t)e->nnode * 3));
+    // tip table: every code's row for every tip branch and eigen set
+    HIPCHK(hipMalloc(&e->d_AT, sizeof(double) * 64 * (size_t)e->ncodes * e->ns * 3));
     if (e->namb > 0) {
-        HIPCHK(hipMalloc(&e->d_AT, sizeof(double) * 64 * (size_t)e->namb * e->ns * 3));
         HIPCHK(hipMalloc(&e->d_mask_amb, 64 * (size_t)e->namb));
     }
     HIPCHK(hipMalloc(&e->d_L, ts * 64 * (size_t)np * e->nslot));
@@ -448,13 +449,13 @@ static int run_device(bsa_engine *e, bool adjoint)
                               hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(e->ev[0], s));
     launch_pmat(s, e->d_uvr, e->d_pjobs, (int)e->pjobs.size(), (double *)e->d_PT, e->d_mask_amb,
-                e->namb, (double *)e->d_AT);
+                e->ncodes, (double *)e->d_AT);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev[1], s));
     for (size_t l = 0; l + 1 < e->level_off.size(); l++) {
         int a = e->level_off[l], b = e->level_off[l + 1];
         launch_prune<T>(s, e->kernel, e->tile, e->d_prune + a, b - a, e->npad, e->d_z,
-                        (const double *)e->d_PT, (const double *)e->d_AT, e->namb, (T *)e->d_L,
+                        (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes, (T *)e->d_L,
                         e->d_E, adjoint ? (T *)e->d_C : nullptr);
     }
     HIPCHK(hipGetLastError());
@@ -474,11 +475,11 @@ static int run_device(bsa_engine *e, bool adjoint)
     if (adjoint) {
         for (size_t d = 0; d + 1 < e->adj_off.size(); d++)
             launch_adjoint_level<T>(s, e->d_adj + e->adj_off[d], e->adj_nint[d], e->npad, e->d_z,
-                                    (const double *)e->d_PT, (const double *)e->d_AT, e->namb,
+                                    (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes,
                                     e->d_E, (const T *)e->d_C, (T *)e->d_A, e->d_EA);
         HIPCHK(hipEventRecord(e->ev[6], s));
         launch_adjoint_m<T>(s, e->d_adj, (int)e->adj.size(), e->npad, e->nchunk, e->d_z,
-                            (const double *)e->d_PT, (const double *)e->d_AT, e->namb, e->d_mask64,
+                            (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes, e->d_mask64,
                             (const T *)e->d_L, e->d_E, (const T *)e->d_C, (const T *)e->d_A,
                             e->d_EA, e->d_Mpart);
         HIPCHK(hipEventRecord(e->ev[4], s));

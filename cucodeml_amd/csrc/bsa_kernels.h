@@ -75,13 +75,13 @@ constexpr int kWParts = 8;              // partial sums per eigen set in the W r
 // adjoints of one depth's interior children (jobs with v_tip < 0 only)
 template <typename T>
 void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad,
-                          const unsigned char *z, const double *PT, const double *AT, int namb,
+                          const unsigned char *z, const double *PT, const double *TT, int ncodes,
                           const int *E, const T *Cb, T *A, int *EA);
 
 // branch-matrix chunk partials of all jobs (after the sweep)
 template <typename T>
 void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
-                      const unsigned char *z, const double *PT, const double *AT, int namb,
+                      const unsigned char *z, const double *PT, const double *TT, int ncodes,
                       const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
                       const int *EA, double *Mpart);
 
@@ -95,13 +95,13 @@ void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const
 
 // launchers; T = storage type of the conditional likelihoods (double or float)
 void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
-                 const unsigned char *mask_amb, int namb, double *AT);
+                 const unsigned char *mask_amb, int ncodes, double *TT);
 
 // kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64)
 // Cb: contribution buffer (interior children's P I per class) or nullptr
 template <typename T>
 void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
-                  const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
+                  const unsigned char *z, const double *PT, const double *TT, int ncodes, T *L,
                   int *E, T *Cb);
 
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with

@@ -81,35 +81,38 @@ __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
     }
 }
 
-// Rows of ambiguous codes for tip branches: AT[pm][a][j] = sum_{k in a} P_{j,k}
-// (full P, not P - I).
-__global__ __launch_bounds__(256) void k_amb(const PmatJob *__restrict__ jobs,
-                                             const double *__restrict__ PT,
-                                             const unsigned char *__restrict__ mask_amb, int namb,
-                                             double *__restrict__ AT)
+// Tip table of every tip branch: TT[pm][c][j] = sum over the sense codons k
+// code c allows of P_{j,k} (row c of P itself for a sense codon), so that a
+// tip's contribution to parent state j is one load, whatever the code.
+__global__ __launch_bounds__(256) void k_tiptab(const PmatJob *__restrict__ jobs,
+                                                const double *__restrict__ PT,
+                                                const unsigned char *__restrict__ mask_amb,
+                                                int ncodes, double *__restrict__ TT)
 {
     const PmatJob jb = jobs[blockIdx.x];
     if (jb.tip < 0) return;
     const double *P = PT + (size_t)jb.pm * 4096;
-    double *out = AT + (size_t)jb.pm * namb * 64;
+    double *out = TT + (size_t)jb.pm * ncodes * 64;
     const int j = threadIdx.x & 63;
-    for (int a = threadIdx.x >> 6; a < namb; a += 4) {
+    for (int c = threadIdx.x >> 6; c < ncodes; c += 4) {
         double s = 0;
-        for (int k = 0; k < 61; k++)
-            if (mask_amb[a * 64 + k]) s += P[k * 64 + j] + (k == j ? 1.0 : 0.0);
-        out[a * 64 + j] = s;
+        if (c < 61) {
+            s = P[c * 64 + j] + (c == j ? 1.0 : 0.0);
+        } else {
+            const unsigned char *m = mask_amb + (size_t)(c - 61) * 64;
+            for (int k = 0; k < 61; k++)
+                if (m[k]) s += P[k * 64 + j] + (k == j ? 1.0 : 0.0);
+        }
+        out[c * 64 + j] = s;
     }
 }
 
 // ------------------------------------------------------------------ helpers
 
-// contribution of a tip with code `code` to parent state j: P_{j,code}, or the
-// ambiguity row sum
-__device__ inline double tip_value(const double *P, const double *AT, int pm, int namb, int code,
-                                   int j)
+// contribution of a tip with code `code` to parent state j (tip table)
+__device__ inline double tip_value(const double *TT, int pm, int ncodes, int code, int j)
 {
-    if (code < 61) return P[code * 64 + j] + (code == j ? 1.0 : 0.0);
-    return AT[((size_t)pm * namb + (code - 61)) * 64 + j];
+    return TT[((size_t)pm * ncodes + code) * 64 + j];
 }
 
 __device__ inline uint32_t mix32(uint32_t a, uint32_t b, uint32_t c)
@@ -186,7 +189,7 @@ template <typename T, int RG>
 __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const PruneJob *__restrict__ jobs, int npad,
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
-                                                    const double *__restrict__ AT, int namb,
+                                                    const double *__restrict__ TT, int ncodes,
                                                     T *__restrict__ L, int *__restrict__ E,
                                                     T *__restrict__ Cb)
 {
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
                 for (int b = 0; b < 4; b++)
 #pragma unroll
                     for (int i = 0; i < 4; i++)
-                        acc[q][b][i] *= tip_value(P, AT, pm, namb, code[i], 16 * b + r);
+                        acc[q][b][i] *= tip_value(TT, pm, ncodes, code[i], 16 * b + r);
             }
         } else {
             const int slot = jb.ch_slot[c];
@@ -298,7 +301,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__ jobs, int npad,
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
-                                                    const double *__restrict__ AT, int namb,
+                                                    const double *__restrict__ TT, int ncodes,
                                                     T *__restrict__ L, int *__restrict__ E,
                                                     T *__restrict__ Cb)
 {
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__
         if (tip >= 0) {
             const unsigned char *zt = z + (size_t)tip * npad + hb;
 #pragma unroll
-            for (int i = 0; i < 16; i++) acc[i] *= tip_value(P, AT, pm, namb, (int)zt[i], j);
+            for (int i = 0; i < 16; i++) acc[i] *= tip_value(TT, pm, ncodes, (int)zt[i], j);
         } else {
             const int slot = jb.ch_slot[c];
             const T *A = L + ((size_t)slot * npad + h0) * 64;
@@ -513,7 +516,7 @@ __device__ inline double pow2i(int d)
 // consecutive states j0..j0+15 of pattern h (one lane), and its exponent.
 template <typename T>
 __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npad,
-                              const unsigned char *z, const double *PT, const double *AT, int namb,
+                              const unsigned char *z, const double *PT, const double *TT, int ncodes,
                               const int *E, const T *Cb, const T *A, const int *EA, double *x)
 {
     const int asl = jb.a_aslot[term];
@@ -524,9 +527,8 @@ __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npa
         if (tip >= 0) {
             const int pm = jb.sib_pmat[term][q];
             const int code = z[(size_t)tip * npad + h];
-            const double *P = PT + (size_t)pm * 4096;
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) x[kk] *= tip_value(P, AT, pm, namb, code, j0 + kk);
+            for (int kk = 0; kk < 16; kk++) x[kk] *= tip_value(TT, pm, ncodes, code, j0 + kk);
         } else {
             double c[16];
             Store<T>::load16(Cb + ((size_t)jb.sib_cslot[term][q] * npad + h) * 64 + j0, c);
@@ -542,13 +544,13 @@ __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npa
 // largest; returns the exponent.
 template <typename T>
 __device__ inline int x_sum16(const AdjJob &jb, int h, int j0, int npad, const unsigned char *z,
-                              const double *PT, const double *AT, int namb, const int *E,
+                              const double *PT, const double *TT, int ncodes, const int *E,
                               const T *Cb, const T *A, const int *EA, double *x)
 {
-    int ex = x_row16<T>(jb, 0, h, j0, npad, z, PT, AT, namb, E, Cb, A, EA, x);
+    int ex = x_row16<T>(jb, 0, h, j0, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
     for (int term = 1; term < jb.nterm; term++) {
         double t[16];
-        const int et = x_row16<T>(jb, term, h, j0, npad, z, PT, AT, namb, E, Cb, A, EA, t);
+        const int et = x_row16<T>(jb, term, h, j0, npad, z, PT, TT, ncodes, E, Cb, A, EA, t);
         const int em = ex > et ? ex : et;
         const double f0 = pow2i(ex - em), f1 = pow2i(et - em);
 #pragma unroll
@@ -567,7 +569,7 @@ template <typename T>
 __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ jobs, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ PT,
-                                                  const double *__restrict__ AT, int namb,
+                                                  const double *__restrict__ TT, int ncodes,
                                                   const int *__restrict__ E,
                                                   const T *__restrict__ Cb, T *__restrict__ A,
                                                   int *__restrict__ EA)
@@ -584,7 +586,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
         }
     }
     double x[16];
-    const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, PT, AT, namb, E, Cb, A, EA, x);
+    const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
     __syncthreads();
     d4 Y[4];
 #pragma unroll
@@ -630,7 +632,7 @@ template <typename T>
 __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ PT,
-                                                  const double *__restrict__ AT, int namb,
+                                                  const double *__restrict__ TT, int ncodes,
                                                   const double *__restrict__ mask64,
                                                   const T *__restrict__ L,
                                                   const int *__restrict__ E,
@@ -654,7 +656,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
         if (tile >= ntile) break;
         const int h0 = tile * 64, hx = h0 + 16 * w + r;
         double x[16];
-        int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, PT, AT, namb, E, Cb, A, EA, x);
+        int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
         if (!vtip) ex += E[(size_t)jb.v_slot * npad + hx];
         const double sc = pow2i(ex);
         __syncthreads();   // the previous tile's reads of sX are done
@@ -662,11 +664,12 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
         for (int kk = 0; kk < 16; kk++) sX[(16 * w + r) * kLdsStride + 16 * g + kk] = x[kk] * sc;
         double bv[16];
         if (vtip) {
+            // indicator of the codons each code allows (identity rows for sense codons)
+            int code[16];
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) {
-                const int code = z[(size_t)jb.v_tip * npad + h0 + 4 * kk + g];
-                bv[kk] = code < 61 ? (code == col ? 1.0 : 0.0) : mask64[code * 64 + col];
-            }
+            for (int kk = 0; kk < 16; kk++) code[kk] = z[(size_t)jb.v_tip * npad + h0 + 4 * kk + g];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) bv[kk] = mask64[code[kk] * 64 + col];
         } else {
             const T *Lv = L + ((size_t)jb.v_slot * npad + h0 + g) * 64 + col;
 #pragma unroll
@@ -849,30 +852,29 @@ __global__ void k_mfma_probe(const double *A, const double *B, double *D, const 
 // ------------------------------------------------------------------ launchers
 
 void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
-                 const unsigned char *mask_amb, int namb, double *AT)
+                 const unsigned char *mask_amb, int ncodes, double *TT)
 {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), 0, s, uvr, jobs, PT);
-    if (namb > 0)
-        hipLaunchKernelGGL(k_amb, dim3(njobs), dim3(256), 0, s, jobs, (const double *)PT, mask_amb,
-                           namb, AT);
+    hipLaunchKernelGGL(k_tiptab, dim3(njobs), dim3(256), 0, s, jobs, (const double *)PT, mask_amb,
+                       ncodes, TT);
 }
 
 template <typename T>
 void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
-                  const unsigned char *z, const double *PT, const double *AT, int namb, T *L,
+                  const unsigned char *z, const double *PT, const double *TT, int ncodes, T *L,
                   int *E, T *Cb)
 {
     if (njobs <= 0) return;
     if (kernel == 0 && tile == 128)
         hipLaunchKernelGGL((k_prune_mfma<T, 2>), dim3(npad / 128, njobs), dim3(256), 0, s, jobs,
-                           npad, z, PT, AT, namb, L, E, Cb);
+                           npad, z, PT, TT, ncodes, L, E, Cb);
     else if (kernel == 0)
         hipLaunchKernelGGL((k_prune_mfma<T, 1>), dim3(npad / 64, njobs), dim3(256), 0, s, jobs,
-                           npad, z, PT, AT, namb, L, E, Cb);
+                           npad, z, PT, TT, ncodes, L, E, Cb);
     else
         hipLaunchKernelGGL(k_prune_valu<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z,
-                           PT, AT, namb, L, E, Cb);
+                           PT, TT, ncodes, L, E, Cb);
 }
 
 template <typename T>
@@ -887,23 +889,23 @@ void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, con
 
 template <typename T>
 void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad,
-                          const unsigned char *z, const double *PT, const double *AT, int namb,
+                          const unsigned char *z, const double *PT, const double *TT, int ncodes,
                           const int *E, const T *Cb, T *A, int *EA)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_adj_a<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
-                       namb, E, Cb, A, EA);
+    hipLaunchKernelGGL(k_adj_a<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z, PT, TT,
+                       ncodes, E, Cb, A, EA);
 }
 
 template <typename T>
 void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
-                      const unsigned char *z, const double *PT, const double *AT, int namb,
+                      const unsigned char *z, const double *PT, const double *TT, int ncodes,
                       const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
                       const int *EA, double *Mpart)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, AT,
-                       namb, mask64, L, E, Cb, A, EA, Mpart, nchunk);
+    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, TT,
+                       ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
