@@ -19,6 +19,8 @@
 // FP32 storage rounds stochastically with a hash of (slot, pattern, state) as
 // the random bits: unbiased, uncorrelated across patterns that share
 // sub-trees, and bitwise repeatable.
+#include <algorithm>
+
 #include "bsa_kernels.h"
 
 #include <math.h>
@@ -124,6 +126,24 @@ __device__ inline uint32_t mix32(uint32_t a, uint32_t b, uint32_t c)
     x *= 0x846CA68Bu;
     x ^= x >> 16;
     return x;
+}
+
+// one xorshift32 step (Marsaglia, J. Stat. Softw. 8(14), 2003): further
+// random bits for the next value a lane stores after a hashed first one
+__device__ inline uint32_t xorshift32(uint32_t x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+// random bits for the values a lane stores in one pattern row: a hash of
+// (buffer, pattern, lane), never 0 (a fixed point of xorshift32)
+__device__ inline uint32_t row_seed(uint32_t buf, uint32_t h, uint32_t lane)
+{
+    const uint32_t x = mix32(buf, h, lane);
+    return x ? x : 0x6B43A9B5u;
 }
 
 // storage of conditional likelihoods
@@ -251,12 +271,16 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
                     const int cs = jb.ch_cslot[c];
                     T *Co = Cb + (size_t)cs * npad * 64;
 #pragma unroll
-                    for (int b = 0; b < 4; b++)
+                    for (int i = 0; i < 4; i++) {
+                        uint32_t rb = row_seed((uint32_t)cs ^ 0x3C6EF372u, (uint32_t)(hb + 4 * i),
+                                               (uint32_t)r);
 #pragma unroll
-                        for (int i = 0; i < 4; i++)
-                            Store<T>::put(Co + (size_t)(hb + 4 * i) * 64 + 16 * b + r, tmp[b][i],
-                                          mix32((uint32_t)cs ^ 0x3C6EF372u, (uint32_t)(hb + 4 * i),
-                                                (uint32_t)(16 * b + r)));
+                        for (int b = 0; b < 4; b++) {
+                            Store<T>::put(Co + (size_t)(hb + 4 * i) * 64 + 16 * b + r,
+                                          tmp[b][i], rb);
+                            rb = xorshift32(rb);
+                        }
+                    }
                 }
 #pragma unroll
                 for (int b = 0; b < 4; b++) acc[q][b] *= tmp[b];
@@ -284,10 +308,12 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
             int e = 0;
             if (m > 0) frexp(m, &e);
             const int h = hb + 4 * i;
+            uint32_t rb = row_seed((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)r);
 #pragma unroll
-            for (int b = 0; b < 4; b++)
-                Store<T>::put(Lo + (size_t)h * 64 + 16 * b + r, ldexp(acc[q][b][i], -e),
-                              mix32((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)(16 * b + r)));
+            for (int b = 0; b < 4; b++) {
+                Store<T>::put(Lo + (size_t)h * 64 + 16 * b + r, ldexp(acc[q][b][i], -e), rb);
+                rb = xorshift32(rb);
+            }
             if (r == 0) E[(size_t)jb.out_slot * npad + h] = esum[q][i] + e;
         }
     }
@@ -560,11 +586,12 @@ __device__ inline int x_sum16(const AdjJob &jb, int h, int j0, int npad, const u
     return ex;
 }
 
-// Adjoint of the interior children of one tree depth, per (job, 64-pattern
-// tile): Abar_v(h, k) = sum_j X(h, j) P_{j,k}.  Shaped like the pruning
-// kernel: wave w owns patterns 16w..16w+15, each lane forms the 16 X values it
-// feeds to the MFMA (pattern 16w + r, states 16g..16g+15) straight from
-// HBM/L2, and P_v (identity included) is staged in LDS once.
+// Adjoint of the interior children of one tree depth, per (job, run of tpb
+// 64-pattern tiles): Abar_v(h, k) = sum_j X(h, j) P_{j,k}.  Shaped like the
+// pruning kernel: wave w owns patterns 16w..16w+15 of each tile, each lane
+// forms the 16 X values it feeds to the MFMA (pattern 16w + r, states
+// 16g..16g+15) straight from HBM/L2, and P_v (identity included) is staged in
+// LDS once per block.
 template <typename T>
 __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ jobs, int npad,
                                                   const unsigned char *__restrict__ z,
@@ -572,12 +599,11 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
                                                   const double *__restrict__ TT, int ncodes,
                                                   const int *__restrict__ E,
                                                   const T *__restrict__ Cb, T *__restrict__ A,
-                                                  int *__restrict__ EA)
+                                                  int *__restrict__ EA, int tpb)
 {
     __shared__ double sP[64 * kLdsStride];
     const AdjJob &jb = jobs[blockIdx.y];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
-    const int hw = blockIdx.x * 64 + 16 * w;
     {
         const double *P = PT + (size_t)jb.v_pmat * 4096;
         for (int q = tid; q < 4096; q += 256) {
@@ -585,39 +611,46 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
             sP[k * kLdsStride + j] = P[q] + (k == j ? 1.0 : 0.0);
         }
     }
-    double x[16];
-    const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
-    __syncthreads();
-    d4 Y[4];
+    const int ntile = npad / 64;
+    for (int t = 0; t < tpb; t++) {
+        const int tile = blockIdx.x * tpb + t;
+        if (tile >= ntile) break;
+        const int hw = tile * 64 + 16 * w;
+        double x[16];
+        const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
+        if (t == 0) __syncthreads();
+        d4 Y[4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) Y[b] = d4{0, 0, 0, 0};
+        for (int b = 0; b < 4; b++) Y[b] = d4{0, 0, 0, 0};
 #pragma unroll
-    for (int kk = 0; kk < 16; kk++)
+        for (int kk = 0; kk < 16; kk++)
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            Y[b] = mfma64(x[kk], sP[(16 * b + r) * kLdsStride + 16 * g + kk], Y[b]);
-    T *Av = A + (size_t)jb.v_aslot * npad * 64;
+            for (int b = 0; b < 4; b++)
+                Y[b] = mfma64(x[kk], sP[(16 * b + r) * kLdsStride + 16 * g + kk], Y[b]);
+        T *Av = A + (size_t)jb.v_aslot * npad * 64;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        // output row g + 4i of the wave; its X exponent sits in lane g + 4i
-        const int exr = __shfl(ex, g + 4 * i, 64);
-        double m = 0;
+        for (int i = 0; i < 4; i++) {
+            // output row g + 4i of the wave; its X exponent sits in lane g + 4i
+            const int exr = __shfl(ex, g + 4 * i, 64);
+            double m = 0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) m = Y[b][i] > m ? Y[b][i] : m;
+            for (int b = 0; b < 4; b++) m = Y[b][i] > m ? Y[b][i] : m;
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            double o = __shfl_xor(m, off, 64);
-            m = o > m ? o : m;
+            for (int off = 1; off < 16; off <<= 1) {
+                double o = __shfl_xor(m, off, 64);
+                m = o > m ? o : m;
+            }
+            int e = 0;
+            if (m > 0) frexp(m, &e);
+            const int h = hw + g + 4 * i;
+            uint32_t rb = row_seed((uint32_t)jb.v_aslot ^ 0xA5A5A5A5u, (uint32_t)h, (uint32_t)r);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                Store<T>::put(Av + (size_t)h * 64 + 16 * b + r, ldexp(Y[b][i], -e), rb);
+                rb = xorshift32(rb);
+            }
+            if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = exr + e;
         }
-        int e = 0;
-        if (m > 0) frexp(m, &e);
-        const int h = hw + g + 4 * i;
-#pragma unroll
-        for (int b = 0; b < 4; b++)
-            Store<T>::put(Av + (size_t)h * 64 + 16 * b + r, ldexp(Y[b][i], -e),
-                          mix32((uint32_t)jb.v_aslot ^ 0xA5A5A5A5u, (uint32_t)h,
-                                (uint32_t)(16 * b + r)));
-        if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = exr + e;
     }
 }
 
@@ -893,8 +926,11 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad
                           const int *E, const T *Cb, T *A, int *EA)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_adj_a<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z, PT, TT,
-                       ncodes, E, Cb, A, EA);
+    // several tiles per block (P_v staged once) when the level has blocks to spare
+    const int ntile = npad / 64;
+    const int tpb = std::max(1, std::min(8, njobs * ntile / 2048));
+    hipLaunchKernelGGL(k_adj_a<T>, dim3((ntile + tpb - 1) / tpb, njobs), dim3(256), 0, s, jobs,
+                       npad, z, PT, TT, ncodes, E, Cb, A, EA, tpb);
 }
 
 template <typename T>
