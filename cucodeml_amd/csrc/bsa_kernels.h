@@ -69,7 +69,7 @@ struct EigJob {
     double tau;                 // t * Qfactor
 };
 
-constexpr int kAdjTilesPerChunk = 16;   // 64-pattern tiles per adjoint block
+constexpr int kAdjTilesPerChunk = 32;   // 64-pattern tiles per adjoint block
 constexpr int kWParts = 8;              // partial sums per eigen set in the W reduction
 
 // adjoints of one depth's interior children (jobs with v_tip < 0 only)
