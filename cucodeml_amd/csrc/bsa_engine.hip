@@ -99,7 +99,7 @@ struct bsa_engine {
     std::vector<EigJob> ejobs;
     std::vector<int> ejob_node;           // node of each EigJob
     std::vector<int> set_off, set_list;
-    int nchunk = 0, nmrow = 0;
+    int nchunk = 0, tpc = 0, nmrow = 0;   // branch-matrix chunks, tiles per chunk
     int root_aslot[4];
     AdjJob *d_adj = nullptr;
     EigJob *d_ejobs = nullptr;
@@ -478,7 +478,7 @@ static int run_device(bsa_engine *e, bool adjoint)
                                     (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes,
                                     e->d_E, (const T *)e->d_C, (T *)e->d_A, e->d_EA);
         HIPCHK(hipEventRecord(e->ev[6], s));
-        launch_adjoint_m<T>(s, e->d_adj, (int)e->adj.size(), e->npad, e->nchunk, e->d_z,
+        launch_adjoint_m<T>(s, e->d_adj, (int)e->adj.size(), e->npad, e->nchunk, e->tpc, e->d_z,
                             (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes, e->d_mask64,
                             (const T *)e->d_L, e->d_E, (const T *)e->d_C, (const T *)e->d_A,
                             e->d_EA, e->d_Mpart);
@@ -734,7 +734,10 @@ static int setup_adjoint(bsa_engine *e)
     }
     e->nmrow = (int)e->adj.size();   // one branch matrix per job
     const int ntile = e->npad / 64;
-    e->nchunk = (ntile + kAdjTilesPerChunk - 1) / kAdjTilesPerChunk;
+    // tiles per branch-matrix block: up to kAdjTilesPerChunk, fewer when the
+    // gene is small, so that the launch still has about 1024 blocks
+    e->tpc = std::max(1, std::min(kAdjTilesPerChunk, ntile * (int)e->adj.size() / 1024));
+    e->nchunk = (ntile + e->tpc - 1) / e->tpc;
     const size_t ts = tsize(e), nej = e->ejobs.size();
     const size_t nasl = (size_t)(nn - ns) * 4;
     HIPCHK(hipSetDevice(e->dev));

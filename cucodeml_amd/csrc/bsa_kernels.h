@@ -69,7 +69,7 @@ struct EigJob {
     double tau;                 // t * Qfactor
 };
 
-constexpr int kAdjTilesPerChunk = 32;   // 64-pattern tiles per adjoint block
+constexpr int kAdjTilesPerChunk = 32;   // most 64-pattern tiles per branch-matrix block
 constexpr int kWParts = 8;              // partial sums per eigen set in the W reduction
 
 // adjoints of one depth's interior children (jobs with v_tip < 0 only)
@@ -80,7 +80,7 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad
 
 // branch-matrix chunk partials of all jobs (after the sweep)
 template <typename T>
-void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk, int tpc,
                       const unsigned char *z, const double *PT, const double *TT, int ncodes,
                       const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
                       const int *EA, double *Mpart);

@@ -672,7 +672,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
                                                   const T *__restrict__ Cb,
                                                   const T *__restrict__ A,
                                                   const int *__restrict__ EA,
-                                                  double *__restrict__ Mpart, int nchunk)
+                                                  double *__restrict__ Mpart, int nchunk, int tpc)
 {
     __shared__ double sX[64 * kLdsStride];
     const AdjJob &jb = jobs[blockIdx.y];
@@ -684,8 +684,8 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
     d4 Macc[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) Macc[a] = d4{0, 0, 0, 0};
-    for (int t = 0; t < kAdjTilesPerChunk; t++) {
-        const int tile = chunk * kAdjTilesPerChunk + t;
+    for (int t = 0; t < tpc; t++) {
+        const int tile = chunk * tpc + t;
         if (tile >= ntile) break;
         const int h0 = tile * 64, hx = h0 + 16 * w + r;
         double x[16];
@@ -934,14 +934,14 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad
 }
 
 template <typename T>
-void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk,
+void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk, int tpc,
                       const unsigned char *z, const double *PT, const double *TT, int ncodes,
                       const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
                       const int *EA, double *Mpart)
 {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, TT,
-                       ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk);
+                       ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk, tpc);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
@@ -960,11 +960,11 @@ template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int
 template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, int, int,
                                           const unsigned char *, const double *, const double *,
                                           int, const int *, const float *, float *, int *);
-template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, int, int, int,
+template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, int, int, int, int,
                                        const unsigned char *, const double *, const double *, int,
                                        const double *, const double *, const int *, const double *,
                                        const double *, const int *, double *);
-template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, int, int, int,
+template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, int, int, int, int,
                                       const unsigned char *, const double *, const double *, int,
                                       const double *, const float *, const int *, const float *,
                                       const float *, const int *, double *);
