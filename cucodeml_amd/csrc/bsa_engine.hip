@@ -15,12 +15,20 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
 using namespace bsa;
 
 static thread_local std::string g_err;
+
+// A graph capture must not see HIP calls of other threads that would
+// invalidate it (allocations, synchronous copies): every entry point that
+// calls HIP holds this lock shared, a capture holds it exclusively.
+static std::shared_mutex g_hip_mu;
+#define HIP_SHARED std::shared_lock<std::shared_mutex> hip_shared_lock_(g_hip_mu)
 
 static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 static int fail(int code, const char *fmt, ...)
@@ -43,6 +51,9 @@ static int fail(int code, const char *fmt, ...)
     } while (0)
 
 constexpr int kUVRSet = 64 * 64 * 2 + 64;
+// per-evaluation model block on the device: the three U/V/root sets, then
+// the four class proportions
+constexpr int kModelBlock = 3 * kUVRSet + 8;
 
 struct bsa_engine {
     int dev = 0;
@@ -64,6 +75,9 @@ struct bsa_engine {
     // device
     hipStream_t stream = nullptr;
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // device work of one evaluation captured as a graph (lnL only, with gradient)
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     double *d_uvr = nullptr;
     PmatJob *d_pjobs = nullptr;
     PruneJob *d_prune = nullptr;
@@ -133,6 +147,8 @@ static void free_all(bsa_engine *e)
     if (e->h_adjout) hipHostFree(e->h_adjout);
     for (auto &v : e->ev)
         if (v) hipEventDestroy(v);
+    for (auto &g : e->graph)
+        if (g) hipGraphExecDestroy(g);
     if (e->stream) hipStreamDestroy(e->stream);
 }
 
@@ -261,7 +277,7 @@ static int upload_static(bsa_engine *e, const bsa_problem *p)
     HIPCHK(hipSetDevice(e->dev));
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto &v : e->ev) HIPCHK(hipEventCreate(&v));
-    HIPCHK(hipMalloc(&e->d_uvr, sizeof(double) * 3 * kUVRSet));
+    HIPCHK(hipMalloc(&e->d_uvr, sizeof(double) * kModelBlock));
     HIPCHK(hipMalloc(&e->d_pjobs, sizeof(PmatJob) * e->pjobs.size()));
     HIPCHK(hipMalloc(&e->d_prune, sizeof(PruneJob) * std::max<size_t>(1, e->prune.size())));
     HIPCHK(hipMalloc(&e->d_PT, sizeof(double) * 4096 * (size_t)e->nnode * 3));
@@ -281,7 +297,7 @@ static int upload_static(bsa_engine *e, const bsa_problem *p)
     e->nroot_blk = e->npad / 16;   // k_root: 16 patterns per block, padding included
     HIPCHK(hipMalloc(&e->d_partial, sizeof(double) * 2 * kRootSums * e->nroot_blk));
     HIPCHK(hipMalloc(&e->d_out, sizeof(double) * 8));
-    HIPCHK(hipHostMalloc((void **)&e->h_uvr, sizeof(double) * 3 * kUVRSet, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&e->h_uvr, sizeof(double) * kModelBlock, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&e->h_out, sizeof(double) * 8, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
                          hipHostMallocDefault));
@@ -312,6 +328,7 @@ extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
 {
     if (!p || !out) return fail(BSA_EINVAL, "null argument");
     *out = nullptr;
+    HIP_SHARED;
     if (p->ns < 3) return fail(BSA_EINVAL, "need at least 3 sequences (ns=%d)", p->ns);
     if (p->nnode <= p->ns || p->nnode > 2 * p->ns - 1)
         return fail(BSA_EINVAL, "nnode=%d inconsistent with ns=%d", p->nnode, p->ns);
@@ -398,6 +415,7 @@ extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
 extern "C" int bsa_destroy(bsa_engine *e)
 {
     if (!e) return BSA_OK;
+    HIP_SHARED;
     free_all(e);
     delete e;
     return BSA_OK;
@@ -438,20 +456,15 @@ static void natural_params(const bsa_engine *e, const double *x, double *kappa, 
 }
 
 template <typename T>
-static int run_device(bsa_engine *e, bool adjoint)
+static int enqueue_device(bsa_engine *e, hipStream_t s, bool adjoint, bool events)
 {
-    hipStream_t s = e->stream;
-    HIPCHK(hipMemcpyAsync(e->d_uvr, e->h_uvr, sizeof(double) * 3 * kUVRSet, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(e->d_pjobs, e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
-                          hipMemcpyHostToDevice, s));
-    if (adjoint)
-        HIPCHK(hipMemcpyAsync(e->d_ejobs, e->h_ejobs, sizeof(EigJob) * e->ejobs.size(),
-                              hipMemcpyHostToDevice, s));
-    HIPCHK(hipEventRecord(e->ev[0], s));
+#define REC(i) \
+    if (events) HIPCHK(hipEventRecord(e->ev[i], s))
+    REC(0);
     launch_pmat(s, e->d_uvr, e->d_pjobs, (int)e->pjobs.size(), (double *)e->d_PT, e->d_mask_amb,
                 e->ncodes, (double *)e->d_AT);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e->ev[1], s));
+    REC(1);
     for (size_t l = 0; l + 1 < e->level_off.size(); l++) {
         int a = e->level_off[l], b = e->level_off[l + 1];
         launch_prune<T>(s, e->kernel, e->tile, e->d_prune + a, b - a, e->npad, e->d_z,
@@ -459,35 +472,64 @@ static int run_device(bsa_engine *e, bool adjoint)
                         e->d_E, adjoint ? (T *)e->d_C : nullptr);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e->ev[2], s));
+    REC(2);
     RootArgs ra;
     ra.npatt = e->npatt;
     ra.npad = e->npad;
     for (int c = 0; c < 4; c++) {
         ra.root_slot[c] = e->root_slot[c];
-        ra.freqK[c] = e->model.freqK[c];
         ra.seed_aslot[c] = adjoint ? e->root_aslot[c] : -1;
     }
-    launch_root<T>(s, ra, (const T *)e->d_L, e->d_E, e->d_pi, e->d_fpatt, e->d_fh, e->d_site,
-                   e->d_partial, e->nroot_blk, e->d_out, (T *)e->d_A, e->d_EA);
+    launch_root<T>(s, ra, (const T *)e->d_L, e->d_E, e->d_pi, e->d_uvr + 3 * kUVRSet, e->d_fpatt,
+                   e->d_fh, e->d_site, e->d_partial, e->nroot_blk, e->d_out, (T *)e->d_A,
+                   e->d_EA);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e->ev[3], s));
+    REC(3);
     if (adjoint) {
         for (size_t d = 0; d + 1 < e->adj_off.size(); d++)
             launch_adjoint_level<T>(s, e->d_adj + e->adj_off[d], e->adj_nint[d], e->npad, e->d_z,
                                     (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes,
                                     e->d_E, (const T *)e->d_C, (T *)e->d_A, e->d_EA);
-        HIPCHK(hipEventRecord(e->ev[6], s));
+        REC(6);
         launch_adjoint_m<T>(s, e->d_adj, (int)e->adj.size(), e->npad, e->nchunk, e->tpc, e->d_z,
                             (const double *)e->d_PT, (const double *)e->d_AT, e->ncodes, e->d_mask64,
                             (const T *)e->d_L, e->d_E, (const T *)e->d_C, (const T *)e->d_A,
                             e->d_EA, e->d_Mpart);
-        HIPCHK(hipEventRecord(e->ev[4], s));
+        REC(4);
         launch_adjoint_finish(s, e->d_Mpart, e->nchunk, e->d_ejobs, (int)e->ejobs.size(),
                               e->d_set_off, e->d_set_list, e->d_uvr, e->d_G, e->d_dtau, e->d_W,
                               e->d_Z);
         HIPCHK(hipGetLastError());
+        REC(5);
+    }
+#undef REC
+    return BSA_OK;
+}
+
+template <typename T>
+static int run_device(bsa_engine *e, bool adjoint)
+{
+    hipStream_t s = e->stream;
+    HIPCHK(hipMemcpyAsync(e->d_uvr, e->h_uvr, sizeof(double) * kModelBlock, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->d_pjobs, e->h_pjobs, sizeof(PmatJob) * e->pjobs.size(),
+                          hipMemcpyHostToDevice, s));
+    if (adjoint)
+        HIPCHK(hipMemcpyAsync(e->d_ejobs, e->h_ejobs, sizeof(EigJob) * e->ejobs.size(),
+                              hipMemcpyHostToDevice, s));
+    if (e->use_graph) {
+        // the launch sequence depends only on the tree and the data, so it is
+        // captured once and replayed; per-evaluation inputs live in device
+        // buffers refreshed by the copies above
+        hipGraphExec_t g = e->graph[adjoint ? 1 : 0];
+        if (!g) return fail(BSA_EDEVICE, "evaluation graph not captured");
+        HIPCHK(hipEventRecord(e->ev[0], s));
+        HIPCHK(hipGraphLaunch(g, s));
         HIPCHK(hipEventRecord(e->ev[5], s));
+    } else {
+        const int rc = enqueue_device<T>(e, s, adjoint, true);
+        if (rc) return rc;
+    }
+    if (adjoint) {
         HIPCHK(hipMemcpyAsync(e->h_adjout, e->d_dtau, sizeof(double) * e->ejobs.size(),
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(e->h_adjout + e->ejobs.size(), e->d_Z, sizeof(double) * 3 * 4096,
@@ -496,6 +538,14 @@ static int run_device(bsa_engine *e, bool adjoint)
     HIPCHK(hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * kRootSums, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     float ms;
+    if (e->use_graph) {
+        // one device time for the whole evaluation (out[0] of bsa_timing)
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[5]));
+        for (int i = 0; i < 6; i++) e->timing[i] = 0;
+        e->timing[0] = ms;
+        for (int i = 0; i < 6; i++) e->timing_sum[i] += e->timing[i];
+        return BSA_OK;
+    }
     HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
     e->timing[0] = ms;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));
@@ -517,6 +567,48 @@ static int run_device(bsa_engine *e, bool adjoint)
     for (int i = 0; i < 6; i++) e->timing_sum[i] += e->timing[i];
     e->n_launch += (double)(e->level_off.size() - 1);
     return BSA_OK;
+}
+
+// Capture the device work of one evaluation as a graph (the launch sequence
+// depends only on the tree and the data; per-evaluation inputs are copied to
+// device buffers before each replay).
+template <typename T>
+static int capture_graph(bsa_engine *e, bool adjoint)
+{
+    hipStream_t s = e->stream;
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_device<T>(e, s, adjoint, false);
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(BSA_EDEVICE, "graph capture: %s", hipGetErrorString(ec));
+    hipGraphExec_t g = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(BSA_EDEVICE, "graph instantiation: %s", hipGetErrorString(ei));
+    e->graph[adjoint ? 1 : 0] = g;
+    return BSA_OK;
+}
+
+static int setup_adjoint(bsa_engine *e);
+
+// In graph mode, capture the evaluation's graph before its first use, with
+// no other HIP work of this process running.
+static int prepare_graph(bsa_engine *e, bool adjoint)
+{
+    if (!e->use_graph || e->graph[adjoint ? 1 : 0]) return BSA_OK;
+    std::unique_lock<std::shared_mutex> lock(g_hip_mu);
+    if (e->graph[adjoint ? 1 : 0]) return BSA_OK;
+    HIPCHK(hipSetDevice(e->dev));
+    if (adjoint) {
+        const int rc = setup_adjoint(e);
+        if (rc) return rc;
+    }
+    return e->precision == BSA_FP32 ? capture_graph<float>(e, adjoint)
+                                    : capture_graph<double>(e, adjoint);
 }
 
 static int evaluate_inner(bsa_engine *e, const double *x, double *lnl, bool adjoint);
@@ -590,6 +682,7 @@ static int evaluate_natural(bsa_engine *e, const double *x, double kappa, double
         e->model.Qfactor[0] = qf_fixed[0];
         e->model.Qfactor[1] = qf_fixed[1];
     }
+    for (int c = 0; c < 4; c++) e->h_uvr[3 * kUVRSet + c] = e->model.freqK[c];
     for (size_t i = 0; i < e->pjobs.size(); i++) {
         PmatJob p = e->pjobs[i];
         p.t = x[e->ibranch[p.node]] * e->model.Qfactor[e->label[p.node]];
@@ -614,6 +707,9 @@ extern "C" int bsa_lnl(bsa_engine *e, const double *x, int np, double *neg_lnl)
 {
     if (!e || !x || !neg_lnl) return fail(BSA_EINVAL, "null argument");
     if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    const int rg = prepare_graph(e, false);
+    if (rg) return rg;
+    HIP_SHARED;
     HIPCHK(hipSetDevice(e->dev));
     double lnl;
     int rc = evaluate(e, x, &lnl);
@@ -805,6 +901,9 @@ extern "C" int bsa_lnl_grad(bsa_engine *e, const double *x, int np, double *neg_
 {
     if (!e || !x || !neg_lnl || !grad) return fail(BSA_EINVAL, "null argument");
     if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    const int rg = prepare_graph(e, true);
+    if (rg) return rg;
+    HIP_SHARED;
     HIPCHK(hipSetDevice(e->dev));
     int rc = setup_adjoint(e);
     if (rc) return rc;
@@ -849,6 +948,9 @@ extern "C" int bsa_lnl_grad_fd(bsa_engine *e, const double *x, int np, int centr
     if (!e || !x || !neg_lnl || !grad) return fail(BSA_EINVAL, "null argument");
     if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
     if (!(small > 0)) small = 1e-6;
+    const int rg = prepare_graph(e, false);
+    if (rg) return rg;
+    HIP_SHARED;
     HIPCHK(hipSetDevice(e->dev));
     std::vector<double> x1(x, x + np);
     double f0, f1, f2;
@@ -896,6 +998,9 @@ extern "C" int bsa_beb(bsa_engine *e, const double *x, int np, double *post, dou
 {
     if (!e || !x || !post) return fail(BSA_EINVAL, "null argument");
     if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    const int rg = prepare_graph(e, false);
+    if (rg) return rg;
+    HIP_SHARED;
     HIPCHK(hipSetDevice(e->dev));
     constexpr int n1d = 10, dim = 4, ngrid = 10000, nset = n1d + 1 + n1d * n1d + n1d;
     const int npatt = e->npatt, npad = e->npad;
@@ -1051,6 +1156,7 @@ extern "C" int bsa_class_lnl(bsa_engine *e, double *fhK, double *site)
 {
     if (!e) return fail(BSA_EINVAL, "null engine");
     if (!e->evaluated) return fail(BSA_EINVAL, "no evaluation yet");
+    HIP_SHARED;
     HIPCHK(hipSetDevice(e->dev));
     if (fhK)
         for (int c = 0; c < 4; c++)
@@ -1111,9 +1217,17 @@ extern "C" int bsa_model_a_setup(double kappa, double p0, double p1, double w0, 
 extern "C" int bsa_work(bsa_engine *e, double *o)
 {
     if (!e || !o) return fail(BSA_EINVAL, "null argument");
+    HIP_SHARED;
     int rc = setup_adjoint(e);
     if (rc) return rc;
     std::memcpy(o, e->work, sizeof e->work);
+    return BSA_OK;
+}
+
+extern "C" int bsa_set_graph(bsa_engine *e, int enable)
+{
+    if (!e) return fail(BSA_EINVAL, "null engine");
+    e->use_graph = enable != 0;
     return BSA_OK;
 }
 
@@ -1150,6 +1264,7 @@ extern "C" const char *bsa_version(void) { return "bsa 0.1 gfx950"; }
 extern "C" int bsa_debug_mfma_probe(int device, const double *A, const double *B, double *D64,
                                     double *D32)
 {
+    HIP_SHARED;
     HIPCHK(hipSetDevice(device));
     double *dA, *dB, *dD;
     float *fA, *fB, *fD;

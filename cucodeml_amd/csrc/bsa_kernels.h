@@ -30,7 +30,6 @@ struct PmatJob {
 struct RootArgs {
     int npatt, npad;
     int root_slot[4];           // root partial slot per class
-    double freqK[4];
     int seed_aslot[4];          // adjoint slot of the root per class, or -1:
                                 // no adjoint seeds are written
 };
@@ -105,11 +104,13 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int
                   int *E, T *Cb);
 
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with
-// a.seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA)
+// a.seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA); fk: the
+// four class proportions (device memory, so that a captured graph reads the
+// current ones)
 template <typename T>
 void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,
-                 const double *fpatt, double *fh, double *site, double *partial, int nblk,
-                 double *out, T *A, int *EA);
+                 const double *fk, const double *fpatt, double *fh, double *site, double *partial,
+                 int nblk, double *out, T *A, int *EA);
 
 // raw MFMA lane-layout probe: one wave, writes each lane's accumulator
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,

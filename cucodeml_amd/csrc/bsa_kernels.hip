@@ -423,6 +423,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ L,
                                               const int *__restrict__ E,
                                               const double *__restrict__ pi,
+                                              const double *__restrict__ fk,
                                               const double *__restrict__ fpatt,
                                               double *__restrict__ fh, double *__restrict__ site,
                                               double *__restrict__ partial, T *__restrict__ A,
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
         for (int c = 1; c < 4; c++) m = l[c] > m ? l[c] : m;
         double f = 0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) f += a.freqK[c] * exp(l[c] - m);
+        for (int c = 0; c < 4; c++) f += fk[c] * exp(l[c] - m);
         const double st = m + log(f);
         const double wgt = fpatt[h] > 0 ? fpatt[h] : 0.0;
         if (seed) {
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const size_t row = (size_t)a.seed_aslot[c] * a.npad + h;
-                const double post = a.freqK[c] * exp(l[c] - st);
+                const double post = fk[c] * exp(l[c] - st);
                 const double val = v[c] > 0 ? wgt * pil * post / v[c] : 0.0;
                 Store<T>::put(A + row * 64 + lane, val, (uint32_t)(h * 64 + lane) ^ 0x5bd1e995u);
                 if (lane == 0) EA[row] = -e[c];
@@ -912,11 +913,11 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int
 
 template <typename T>
 void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,
-                 const double *fpatt, double *fh, double *site, double *partial, int nblk,
-                 double *out, T *A, int *EA)
+                 const double *fk, const double *fpatt, double *fh, double *site, double *partial,
+                 int nblk, double *out, T *A, int *EA)
 {
-    hipLaunchKernelGGL(k_root<T>, dim3(nblk), dim3(256), 0, s, a, L, E, pi, fpatt, fh, site, partial,
-                       A, EA);
+    hipLaunchKernelGGL(k_root<T>, dim3(nblk), dim3(256), 0, s, a, L, E, pi, fk, fpatt, fh, site,
+                       partial, A, EA);
     hipLaunchKernelGGL(k_final, dim3(kRootSums), dim3(64), 0, s, (const double *)partial, nblk, out);
 }
 
@@ -982,10 +983,12 @@ template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, int, 
                                   const unsigned char *, const double *, const double *, int,
                                   float *, int *, float *);
 template void launch_root<double>(hipStream_t, const RootArgs &, const double *, const int *,
-                                  const double *, const double *, double *, double *, double *,
+                                  const double *, const double *, const double *, double *,
+                                  double *, double *,
                                   int, double *, double *, int *);
 template void launch_root<float>(hipStream_t, const RootArgs &, const float *, const int *,
-                                 const double *, const double *, double *, double *, double *,
+                                 const double *, const double *, const double *, double *,
+                                 double *, double *,
                                  int, double *, float *, int *);
 
 // ------------------------------------------------------------------ Bayes empirical Bayes

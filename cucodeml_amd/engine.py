@@ -24,7 +24,7 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
             "bsa_last_error", "bsa_version", "bsa_debug_mfma_probe", "bsa_work",
             "bsa_model_a_setup", "bsa_default_bounds", "bsa_fit", "bsa_debug_minimize",
-            "bsa_beb"]
+            "bsa_beb", "bsa_set_graph"]
 
 
 class BsaError(RuntimeError):
@@ -82,6 +82,7 @@ def lib():
         L.bsa_debug_mfma_probe.argtypes = [C.c_int, dp, dp, dp, dp]
         L.bsa_default_bounds.argtypes = [C.c_void_p, dp, dp]
         L.bsa_beb.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp, dp, C.c_void_p]
+        L.bsa_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.bsa_fit.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, C.POINTER(_FitOpts), dp,
                               C.POINTER(C.c_int)]
         L.bsa_debug_minimize.argtypes = [C.c_int, _OBJECTIVE, C.c_void_p, dp, dp, dp,
@@ -145,6 +146,10 @@ class BranchSiteEngine:
         o = _Opts(int(bool(fix_omega)), float(omega_fix), int(bool(p_transformed)))
         _check(lib().bsa_set_model(self._h, C.byref(o)))
         self.np = lib().bsa_num_params(self._h)
+
+    def use_graph(self, enable=True):
+        """Replay each evaluation's device work as a captured HIP graph."""
+        _check(lib().bsa_set_graph(self._h, int(bool(enable))))
 
     def close(self):
         if getattr(self, "_h", None):

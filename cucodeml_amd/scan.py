@@ -57,13 +57,16 @@ def simulate_genes(n, seed=2024, sigma=SCAN_SIGMA, median=SCAN_MEDIAN, clip=SCAN
     return genes
 
 
-def fit_gene(gene, precision="fp64", gradient="adjoint", device=0, e=1e-8):
-    """H1 and H0 fits of one gene on its own engine, and the LRT."""
+def fit_gene(gene, precision="fp64", gradient="adjoint", device=0, e=1e-8, graph=False):
+    """H1 and H0 fits of one gene on its own engine, and the LRT.  With
+    ``graph`` each evaluation's device work is replayed from a HIP graph
+    (measured slower on MI355X once several genes run at once, so off)."""
     t0 = time.perf_counter()
     # a tree file without branch lengths: every branch starts at 0.1
     start = codeml.default_start(gene.tree, 2.0, 1.5, False)
     start[:gene.tree.nbranch] = 0.1
     eng = engine.BranchSiteEngine(gene.tree, gene.data, precision=precision, device=device)
+    eng.use_graph(graph)
     try:
         h1 = codeml.fit_model_a(gene.tree, gene.data, False, 1.5, start=start,
                                 gradient=gradient, e=e, eng=eng)
@@ -90,7 +93,7 @@ def fit_gene(gene, precision="fp64", gradient="adjoint", device=0, e=1e-8):
             "converged": h1.converged and h0.converged, "seconds": time.perf_counter() - t0}
 
 
-def scan(genes, workers=8, precision="fp64", gradient="adjoint", device=0, e=1e-8):
+def scan(genes, workers=8, precision="fp64", gradient="adjoint", device=0, e=1e-8, graph=False):
     """Fit every gene (H1, H0, LRT) with `workers` concurrent engines.
     Returns (results in input order, wall seconds)."""
     order = sorted(range(len(genes)),
@@ -98,7 +101,8 @@ def scan(genes, workers=8, precision="fp64", gradient="adjoint", device=0, e=1e-
     out = [None] * len(genes)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
-        futs = {i: ex.submit(fit_gene, genes[i], precision, gradient, device, e) for i in order}
+        futs = {i: ex.submit(fit_gene, genes[i], precision, gradient, device, e, graph)
+                for i in order}
         for i, f in futs.items():
             out[i] = f.result()
     return out, time.perf_counter() - t0

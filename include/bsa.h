@@ -137,6 +137,13 @@ int  bsa_beb(bsa_engine *eng, const double *x, int np, double *post, double *ml3
  * foreground; out[7..10] = class proportions */
 int  bsa_last_params(const bsa_engine *eng, double *out11);
 
+/* enable != 0: run each evaluation's device work as one HIP graph, captured
+ * at the first evaluation of each kind (lnL only, lnL + gradient) and
+ * replayed afterwards (the launch sequence depends only on the tree and the
+ * data); cuts launch overhead for small genes.  bsa_timing then reports the
+ * whole device time in out[0] and 0 for the phases. */
+int  bsa_set_graph(bsa_engine *eng, int enable);
+
 /* timing in ms, from HIP events on the engine's stream: out[0] P(t) and tip
  * tables, out[1] pruning (all levels), out[2] root reduction, out[3] child
  * adjoints (all depths; 0 without gradient), out[4] branch matrices (one

@@ -1,6 +1,7 @@
 """Genome scan on one GPU: genes fitted on concurrent engines (one host
-thread and one HIP stream each) give exactly the results of fitting them one
-after another, every fit converges, and the H1 likelihood at the estimates
+thread and one HIP stream each, evaluations replayed from HIP graphs) give
+exactly the results of fitting them one after another with eager launches,
+every fit converges, and the H1 likelihood at the estimates
 agrees with the oracle's likelihood of the same gene at the same parameters."""
 import numpy as np
 import pytest
@@ -12,9 +13,9 @@ pytestmark = pytest.mark.gpu
 
 
 def test_concurrent_scan_equals_sequential_and_oracle():
-    print("case: scan 6 genes, 3 workers vs 1", flush=True)
+    print("case: scan 6 genes, 3 workers with graphs vs 1 without", flush=True)
     genes = scan.simulate_genes(6, seed=7, median=150, clip=(50, 400), taxa=(8, 20))
-    par, wall_p = scan.scan(genes, workers=3)
+    par, wall_p = scan.scan(genes, workers=3, graph=True)
     seq, wall_s = scan.scan(genes, workers=1)
     print(f"  wall {wall_p:.2f} s (3 workers) vs {wall_s:.2f} s (1)", flush=True)
     for a, b in zip(par, seq):
