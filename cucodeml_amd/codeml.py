@@ -151,6 +151,31 @@ def fit_model_a(tree, data, fix_omega=False, omega=1.0, start=None, precision="f
             eng.close()
 
 
+def standard_errors(eng, x, small_diff=1e-6):
+    """SEs of the parameters at x in codeml's printed form (p0, p1
+    untransformed) from the Hessian approximation of Seo et al. 2004, as the
+    reference computes it with getSE = 1 (HessianSKT2004,
+    treesub.c:6979-7048): per-site log f at x +- 2 Small_Diff (|x_i| + 1) for
+    every parameter (2 np GPU evaluations), H = -sum_h fpatt_h d_i d_j,
+    SE = sqrt(diag((-H)^-1)), -1 where not positive (codeml.c:779-784).
+    The engine is left set to untransformed p's."""
+    x = np.asarray(x, dtype=np.float64)
+    eng.set_model(eng.fix_omega, eng.omega_fix, p_transformed=False)
+    D = np.zeros((len(x), eng.data.npatt))
+    for i in range(len(x)):
+        eh = 2 * small_diff * (abs(x[i]) + 1)
+        xs = x.copy()
+        xs[i] = x[i] + eh
+        eng.lnl(xs)
+        up = eng.fhK()[1]
+        xs[i] = x[i] - eh
+        eng.lnl(xs)
+        D[i] = (up - eng.fhK()[1]) / (2 * eh)
+    H = -(D * eng.data.fpatt[None, :]) @ D.T
+    d = np.diag(np.linalg.inv(-H))
+    return np.where(d > 0, np.sqrt(np.abs(d)), -1.0)
+
+
 def lrt(h0, h1):
     """2 dlnL of H1 over H0 and its p-values: chi2 with 1 df, and the 50:50
     mixture of a point mass at 0 and chi2_1 (the boundary null of w2 = 1)."""
@@ -298,6 +323,10 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
             fit = fit_model_a(tree, data, fix_omega, omega, start=start, gradient=gradient,
                               kappa=kappa, eng=eng)
             xe = fit.x
+        if int(_num(ctl, "getSE", 0)) >= 1:
+            fit.extra["se"] = standard_errors(eng, fit.x_untransformed(),
+                                              _num(ctl, "Small_Diff", 1e-6))
+            eng.set_model(fix_omega, omega, p_transformed=not fixed_run)
         # site posteriors under H1, as the reference prints after the fit
         # (NEB codeml.c:5079-5129, BEB 6617-6826)
         if not fix_omega:
@@ -307,6 +336,8 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
     finally:
         eng.close()
     text = format_out(tree, data, fit, names)
+    if "se" in fit.extra:
+        text += "SEs for parameters:\n" + "".join(f" {v:8.6f}" for v in fit.extra["se"]) + "\n"
     if not fix_omega:
         text += format_beb_out(fit.extra["beb"], data)
         with open(os.path.join(base, "rst"), "w") as f:

@@ -146,6 +146,8 @@ class BranchSiteEngine:
         o = _Opts(int(bool(fix_omega)), float(omega_fix), int(bool(p_transformed)))
         _check(lib().bsa_set_model(self._h, C.byref(o)))
         self.np = lib().bsa_num_params(self._h)
+        self.fix_omega, self.omega_fix = bool(fix_omega), float(omega_fix)
+        self.p_transformed = bool(p_transformed)
 
     def use_graph(self, enable=True):
         """Replay each evaluation's device work as a captured HIP graph."""

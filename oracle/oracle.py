@@ -212,3 +212,36 @@ def beb_integrate(F, fpatt):
             t = np.log(fk / fh) + lnfx[g] - lfX
             post += np.where(t > -300, np.exp(t), 0.0)
     return post, lfX + S1 - dim * np.log(n1d), pp[2:], p01
+
+
+# ---------------------------------------------------------------- standard errors
+
+def standard_errors(tree, data, x, fix_omega, omega_fix=1.0, small_diff=1e-6):
+    """SEs of codeml's parameters at x (branch lengths, kappa, p0, p1, w0
+    (, w2), p's untransformed) from the Hessian approximation of Seo et al.
+    2004 as HessianSKT2004 computes it (treesub.c:6979-7048): per-site central
+    differences with step 2 Small_Diff (|x_i| + 1), H = -sum_h fpatt_h
+    d_i d_j, SE_i = sqrt([(-H)^-1]_ii) (-1 where not positive;
+    codeml.c:779-784)."""
+    nb = tree.nbranch
+    x = np.asarray(x, dtype=np.float64)
+
+    def site(xv):
+        tree.set_branch_vector(xv[:nb])
+        w2 = omega_fix if fix_omega else xv[nb + 4]
+        _, _, s = lnl(tree, data, xv[nb], xv[nb + 1], xv[nb + 2], xv[nb + 3], w2)
+        return s
+
+    eh0 = small_diff * 2
+    D = np.zeros((len(x), data.npatt))
+    for i in range(len(x)):
+        eh = eh0 * (abs(x[i]) + 1)
+        xm, xp = x.copy(), x.copy()
+        xm[i] -= eh
+        xp[i] += eh
+        D[i] = (site(xp) - site(xm)) / (2 * eh)
+    tree.set_branch_vector(x[:nb])
+    H = -(D * data.fpatt[None, :]) @ D.T
+    V = np.linalg.inv(-H)
+    d = np.diag(V)
+    return np.where(d > 0, np.sqrt(np.abs(d)), -1.0)
