@@ -28,27 +28,15 @@
 #include <vector>
 
 #include "bsa.h"
+#include "bsa_fit.h"
 
 int bsa_error(int code, const char *msg);   // sets bsa_last_error (engine)
 
 namespace {
-
-// objective: f and its gradient g at x; returns a bsa status code
-using ObjFn = std::function<int(const double *x, double *f, double *g)>;
-
 inline double clampd(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
-
-struct FitResult {
-    double f = 0;
-    int iterations = 0, evaluations = 0, converged = 0;
-};
-
-int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
-                 const bsa_fit_opts *opts, FitResult *res);
-
 }  // namespace
 
-namespace {
+namespace bsa {
 
 int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const double *hi,
                  const bsa_fit_opts *opts, FitResult *res)
@@ -209,7 +197,11 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     return BSA_OK;
 }
 
-}  // namespace
+}  // namespace bsa
+
+using bsa::FitResult;
+using bsa::ObjFn;
+using bsa::minimize_box;
 
 extern "C" int bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, const double *hi,
                        const bsa_fit_opts *opts, double *neg_lnl, int *info)

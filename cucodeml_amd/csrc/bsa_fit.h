@@ -1,0 +1,22 @@
+// Internal header: the bounded quasi-Newton minimiser behind bsa_fit.
+#pragma once
+#include <functional>
+
+#include "bsa.h"
+
+namespace bsa {
+
+// objective: f and its gradient g at x; returns a bsa status code
+using ObjFn = std::function<int(const double *x, double *f, double *g)>;
+
+struct FitResult {
+    double f = 0;
+    int iterations = 0, evaluations = 0, converged = 0;
+};
+
+// minimise fg over the box [lo, hi] from x (moved into the box); x holds
+// the minimiser on return
+int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
+                 const bsa_fit_opts *opts, FitResult *res);
+
+}  // namespace bsa

@@ -24,9 +24,12 @@ struct PruneJob {
 struct PmatJob {
     int node, set, pm;          // pm = node * 3 + set
     int tip;                    // tip index if the node is a tip, else -1
+    int amb;                    // first row of this gene's ambiguity masks
     double t;                   // effective branch length t * Qfactor
 };
 
+// root pass of one gene (blockIdx.y of the root kernel); per-gene inputs and
+// outputs sit at gene-indexed offsets of the buffers the launch gets
 struct RootArgs {
     int npatt, npad;
     int root_slot[4];           // root partial slot per class
@@ -48,6 +51,7 @@ constexpr int kRootSums = 5;
 // its chunk of patterns.
 struct AdjJob {
     int v_tip;                  // tip index of v, or -1
+    int v_mask;                 // first row of this gene's code-indicator table
     int v_slot;                 // inside partial slot of v (interior)
     int v_pmat;                 // P(t) matrix index of v's branch
     int v_aslot;                // adjoint slot written for v (interior) or -1
@@ -86,11 +90,11 @@ void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, in
 
 // for every EigJob sums the chunk partials of its branch matrix M, forms
 // A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set
-// W = sum G (kWParts partials in Wpart[kWParts][3][64*64]) and
-// Z = V^T W U^T.  Outputs dtau[nej] and Z[3][64*64].
+// W = sum G (kWParts partials in Wpart[kWParts][nsets][64*64]) and
+// Z = V^T W U^T.  Outputs dtau[nej] and Z[nsets][64*64] (3 sets per gene).
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
-                           int nej, const int *set_off, const int *set_list, const double *uvr,
-                           double *G, double *dtau, double *Wpart, double *Z);
+                           int nej, int nsets, const int *set_off, const int *set_list,
+                           const double *uvr, double *G, double *dtau, double *Wpart, double *Z);
 
 // launchers; T = storage type of the conditional likelihoods (double or float)
 void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
@@ -104,13 +108,15 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int
                   int *E, T *Cb);
 
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with
-// a.seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA); fk: the
-// four class proportions (device memory, so that a captured graph reads the
-// current ones)
+// seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA).  args:
+// device array of ngene RootArgs; gene g reads pi[g*64], fk[g*8] (its four
+// class proportions, in device memory so that a captured graph reads the
+// current ones), fpatt[g*npad], writes fh[g*4*npad], site[g*npad],
+// partial[g*nblk*2*kRootSums] and out[g*kRootSums]
 template <typename T>
-void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,
-                 const double *fk, const double *fpatt, double *fh, double *site, double *partial,
-                 int nblk, double *out, T *A, int *EA);
+void launch_root(hipStream_t s, const RootArgs *args, int ngene, const T *L, const int *E,
+                 const double *pi, const double *fk, const double *fpatt, double *fh, double *site,
+                 double *partial, int nblk, double *out, T *A, int *EA);
 
 // raw MFMA lane-layout probe: one wave, writes each lane's accumulator
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_tiptab(const PmatJob *__restrict__ jobs
         if (c < 61) {
             s = P[c * 64 + j] + (c == j ? 1.0 : 0.0);
         } else {
-            const unsigned char *m = mask_amb + (size_t)(c - 61) * 64;
+            const unsigned char *m = mask_amb + (size_t)(jb.amb + c - 61) * 64;
             for (int k = 0; k < 61; k++)
                 if (m[k]) s += P[k * 64 + j] + (k == j ? 1.0 : 0.0);
         }
@@ -420,7 +420,8 @@ __device__ inline void neumaier(double &s, double &c, double x)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ L,
+__global__ __launch_bounds__(256) void k_root(const RootArgs *__restrict__ args,
+                                              const T *__restrict__ L,
                                               const int *__restrict__ E,
                                               const double *__restrict__ pi,
                                               const double *__restrict__ fk,
@@ -430,6 +431,14 @@ __global__ __launch_bounds__(256) void k_root(RootArgs a, const T *__restrict__ 
                                               int *__restrict__ EA)
 {
     __shared__ double ss[4][kRootSums], sc[4][kRootSums];
+    const int gi = blockIdx.y;
+    const RootArgs a = args[gi];
+    pi += (size_t)gi * 64;
+    fk += (size_t)gi * 8;
+    fpatt += (size_t)gi * a.npad;
+    fh += (size_t)gi * 4 * a.npad;
+    site += (size_t)gi * a.npad;
+    partial += (size_t)gi * gridDim.x * 2 * kRootSums;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double pil = lane < 61 ? pi[lane] : 0.0;
     const bool seed = a.seed_aslot[0] >= 0;
@@ -510,6 +519,8 @@ __global__ __launch_bounds__(64) void k_final(const double *__restrict__ partial
                                               double *__restrict__ out)
 {
     const int lane = threadIdx.x, q = blockIdx.x;
+    partial += (size_t)blockIdx.y * nblk * 2 * kRootSums;
+    out += (size_t)blockIdx.y * kRootSums;
     double S = 0, Cc = 0;
     for (int i = lane; i < nblk; i += 64) {
         neumaier(S, Cc, partial[(size_t)i * 2 * kRootSums + 2 * q]);
@@ -703,7 +714,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
 #pragma unroll
             for (int kk = 0; kk < 16; kk++) code[kk] = z[(size_t)jb.v_tip * npad + h0 + 4 * kk + g];
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) bv[kk] = mask64[code[kk] * 64 + col];
+            for (int kk = 0; kk < 16; kk++) bv[kk] = mask64[(size_t)(jb.v_mask + code[kk]) * 64 + col];
         } else {
             const T *Lv = L + ((size_t)jb.v_slot * npad + h0 + g) * 64 + col;
 #pragma unroll
@@ -815,7 +826,7 @@ __global__ __launch_bounds__(256) void k_wreduce(const double *__restrict__ G,
     double s = 0;
     for (int i = set_off[set] + p; i < set_off[set + 1]; i += kWParts)
         s += G[(size_t)set_list[i] * 4096 + q];
-    Wpart[((size_t)p * 3 + set) * 4096 + q] = s;
+    Wpart[((size_t)p * gridDim.x + set) * 4096 + q] = s;
 }
 
 // Per eigen set: W = sum of its parts (fixed order), Z = V^T W U^T (so that
@@ -831,7 +842,7 @@ __global__ __launch_bounds__(256) void k_zform(const double *__restrict__ Wpart,
     const double *V = U + 4096;
     for (int q = tid; q < 4096; q += 256) {
         double w = 0;
-        for (int p = 0; p < kWParts; p++) w += Wpart[((size_t)p * 3 + set) * 4096 + q];
+        for (int p = 0; p < kWParts; p++) w += Wpart[((size_t)p * gridDim.x + set) * 4096 + q];
         sW[(q >> 6) * kLdsStride + (q & 63)] = w;
         sB[(q >> 6) * kLdsStride + (q & 63)] = V[q];
     }
@@ -912,13 +923,14 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int
 }
 
 template <typename T>
-void launch_root(hipStream_t s, const RootArgs &a, const T *L, const int *E, const double *pi,
-                 const double *fk, const double *fpatt, double *fh, double *site, double *partial,
-                 int nblk, double *out, T *A, int *EA)
+void launch_root(hipStream_t s, const RootArgs *args, int ngene, const T *L, const int *E,
+                 const double *pi, const double *fk, const double *fpatt, double *fh, double *site,
+                 double *partial, int nblk, double *out, T *A, int *EA)
 {
-    hipLaunchKernelGGL(k_root<T>, dim3(nblk), dim3(256), 0, s, a, L, E, pi, fk, fpatt, fh, site,
-                       partial, A, EA);
-    hipLaunchKernelGGL(k_final, dim3(kRootSums), dim3(64), 0, s, (const double *)partial, nblk, out);
+    hipLaunchKernelGGL(k_root<T>, dim3(nblk, ngene), dim3(256), 0, s, args, L, E, pi, fk, fpatt, fh,
+                       site, partial, A, EA);
+    hipLaunchKernelGGL(k_final, dim3(kRootSums, ngene), dim3(64), 0, s, (const double *)partial,
+                       nblk, out);
 }
 
 template <typename T>
@@ -946,13 +958,13 @@ void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, in
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
-                           int nej, const int *set_off, const int *set_list, const double *uvr,
-                           double *G, double *dtau, double *Wpart, double *Z)
+                           int nej, int nsets, const int *set_off, const int *set_list,
+                           const double *uvr, double *G, double *dtau, double *Wpart, double *Z)
 {
     hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, Mpart, nchunk, uvr, G, dtau);
-    hipLaunchKernelGGL(k_wreduce, dim3(3, 16, kWParts), dim3(256), 0, s, (const double *)G, set_off,
-                       set_list, Wpart);
-    hipLaunchKernelGGL(k_zform, dim3(3), dim3(256), 0, s, (const double *)Wpart, uvr, Z);
+    hipLaunchKernelGGL(k_wreduce, dim3(nsets, 16, kWParts), dim3(256), 0, s, (const double *)G,
+                       set_off, set_list, Wpart);
+    hipLaunchKernelGGL(k_zform, dim3(nsets), dim3(256), 0, s, (const double *)Wpart, uvr, Z);
 }
 
 template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int,
@@ -982,14 +994,12 @@ template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, int,
 template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, int, int,
                                   const unsigned char *, const double *, const double *, int,
                                   float *, int *, float *);
-template void launch_root<double>(hipStream_t, const RootArgs &, const double *, const int *,
+template void launch_root<double>(hipStream_t, const RootArgs *, int, const double *, const int *,
                                   const double *, const double *, const double *, double *,
-                                  double *, double *,
-                                  int, double *, double *, int *);
-template void launch_root<float>(hipStream_t, const RootArgs &, const float *, const int *,
+                                  double *, double *, int, double *, double *, int *);
+template void launch_root<float>(hipStream_t, const RootArgs *, int, const float *, const int *,
                                  const double *, const double *, const double *, double *,
-                                 double *, double *,
-                                 int, double *, float *, int *);
+                                 double *, double *, int, double *, float *, int *);
 
 // ------------------------------------------------------------------ Bayes empirical Bayes
 

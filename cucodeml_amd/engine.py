@@ -24,7 +24,11 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
             "bsa_last_error", "bsa_version", "bsa_debug_mfma_probe", "bsa_work",
             "bsa_model_a_setup", "bsa_default_bounds", "bsa_fit", "bsa_debug_minimize",
-            "bsa_beb", "bsa_set_graph"]
+            "bsa_beb", "bsa_set_graph", "bsa_batch_create", "bsa_batch_destroy",
+            "bsa_batch_size", "bsa_batch_set_model", "bsa_batch_num_params",
+            "bsa_batch_default_bounds", "bsa_batch_lnl", "bsa_batch_lnl_grad", "bsa_batch_fit",
+            "bsa_batch_class_lnl", "bsa_batch_last_params", "bsa_batch_timing", "bsa_batch_work",
+            "bsa_batch_set_graph", "bsa_debug_forest_plan"]
 
 
 class BsaError(RuntimeError):
@@ -87,6 +91,23 @@ def lib():
                               C.POINTER(C.c_int)]
         L.bsa_debug_minimize.argtypes = [C.c_int, _OBJECTIVE, C.c_void_p, dp, dp, dp,
                                          C.POINTER(_FitOpts), dp, C.POINTER(C.c_int)]
+        pp, dpp = C.POINTER(C.POINTER(_Problem)), C.POINTER(dp)
+        ip = C.POINTER(C.c_int)
+        L.bsa_batch_create.argtypes = [pp, C.c_int, C.POINTER(C.c_void_p)]
+        L.bsa_batch_destroy.argtypes = [C.c_void_p]
+        L.bsa_batch_size.argtypes = [C.c_void_p]
+        L.bsa_batch_set_model.argtypes = [C.c_void_p, C.c_int, C.POINTER(_Opts)]
+        L.bsa_batch_num_params.argtypes = [C.c_void_p, C.c_int]
+        L.bsa_batch_default_bounds.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.bsa_batch_lnl.argtypes = [C.c_void_p, dpp, dp]
+        L.bsa_batch_lnl_grad.argtypes = [C.c_void_p, dpp, dp, dpp]
+        L.bsa_batch_fit.argtypes = [C.c_void_p, dpp, dpp, dpp, C.POINTER(_FitOpts), dp, ip]
+        L.bsa_batch_class_lnl.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.bsa_batch_last_params.argtypes = [C.c_void_p, C.c_int, dp]
+        L.bsa_batch_timing.argtypes = [C.c_void_p, dp, C.c_int]
+        L.bsa_batch_work.argtypes = [C.c_void_p, dp]
+        L.bsa_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.bsa_debug_forest_plan.argtypes = [pp, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -111,32 +132,35 @@ def x_from_params(nbranch_t, kappa, p0, p1, w0, w2=None, transformed=True):
     return np.concatenate([np.asarray(nbranch_t, dtype=np.float64), tail])
 
 
+def _problem(tree, data, precision, kernel, device, tile, pi, keep):
+    """bsa_problem of one gene; the arrays it points to are appended to keep"""
+    def iarr(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_int))
+
+    z = np.ascontiguousarray(data.z, dtype=np.uint8)
+    mask = np.ascontiguousarray(data.code_table().astype(np.uint8))
+    fp = np.ascontiguousarray(data.fpatt, dtype=np.float64)
+    pv = np.ascontiguousarray(data.pi if pi is None else pi, dtype=np.float64)
+    keep += [z, mask, fp, pv]
+    lab = np.array([max(int(v), 0) for v in tree.label])
+    return _Problem(ns=tree.ns, nnode=tree.nnode, root=tree.root, father=iarr(tree.father),
+                    label=iarr(lab), ibranch=iarr(tree.ibranch), npatt=data.npatt,
+                    z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
+                    code_mask=mask.ctypes.data_as(C.POINTER(C.c_ubyte)), fpatt=_dptr(fp),
+                    pi=_dptr(pv), precision=BSA_FP32 if precision == "fp32" else BSA_FP64,
+                    kernel=BSA_KERNEL_VALU if kernel == "valu" else BSA_KERNEL_MFMA,
+                    device=device, tile=tile)
+
+
 class BranchSiteEngine:
     def __init__(self, tree, data, precision="fp64", kernel="mfma", device=0,
                  fix_omega=False, omega_fix=1.0, p_transformed=True, pi=None, tile=0):
         self.tree, self.data = tree, data
         self.precision, self.device = precision, device
         self._keep = []
-        ns, nn = tree.ns, tree.nnode
-
-        def iarr(a):
-            a = np.ascontiguousarray(a, dtype=np.int32)
-            self._keep.append(a)
-            return a.ctypes.data_as(C.POINTER(C.c_int))
-
-        z = np.ascontiguousarray(data.z, dtype=np.uint8)
-        mask = np.ascontiguousarray(data.code_table().astype(np.uint8))
-        fp = np.ascontiguousarray(data.fpatt, dtype=np.float64)
-        pv = np.ascontiguousarray(data.pi if pi is None else pi, dtype=np.float64)
-        self._keep += [z, mask, fp, pv]
-        lab = np.array([max(int(v), 0) for v in tree.label])
-        prob = _Problem(ns=ns, nnode=nn, root=tree.root, father=iarr(tree.father),
-                        label=iarr(lab), ibranch=iarr(tree.ibranch), npatt=data.npatt,
-                        z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
-                        code_mask=mask.ctypes.data_as(C.POINTER(C.c_ubyte)), fpatt=_dptr(fp),
-                        pi=_dptr(pv), precision=BSA_FP32 if precision == "fp32" else BSA_FP64,
-                        kernel=BSA_KERNEL_VALU if kernel == "valu" else BSA_KERNEL_MFMA,
-                        device=device, tile=tile)
+        prob = _problem(tree, data, precision, kernel, device, tile, pi, self._keep)
         h = C.c_void_p()
         _check(lib().bsa_create(C.byref(prob), C.byref(h)))
         self._h = h
@@ -271,6 +295,141 @@ class BranchSiteEngine:
         return d
 
 
+def _ptrs(arrays):
+    """C array of double pointers to the given float64 arrays"""
+    return (C.POINTER(C.c_double) * len(arrays))(*[_dptr(a) for a in arrays])
+
+
+class BranchSiteBatch:
+    """Many genes evaluated together (bsa_batch_*): each gene has its own
+    tree, alignment and model options; one evaluation of the batch runs one
+    launch per tree level of the deepest tree for all of them.  Patterns are
+    padded to the largest gene's count, so genes of similar size batch best.
+    ``genes`` is a list of (tree, data) pairs."""
+
+    def __init__(self, genes, precision="fp64", kernel="mfma", device=0, tile=0,
+                 fix_omega=False, omega_fix=1.0, p_transformed=True):
+        self.genes = list(genes)
+        self.precision, self.device = precision, device
+        self._keep = []
+        probs = [_problem(t, d, precision, kernel, device, tile, None, self._keep)
+                 for t, d in self.genes]
+        self._probs = probs
+        arr = (C.POINTER(_Problem) * len(probs))(*[C.pointer(p) for p in probs])
+        h = C.c_void_p()
+        _check(lib().bsa_batch_create(arr, len(probs), C.byref(h)))
+        self._h = h
+        self.np = [0] * len(self.genes)
+        self.fix_omega = [False] * len(self.genes)
+        self.omega_fix = [1.0] * len(self.genes)
+        self.set_model(-1, fix_omega, omega_fix, p_transformed)
+
+    def __len__(self):
+        return len(self.genes)
+
+    def set_model(self, gene=-1, fix_omega=False, omega_fix=1.0, p_transformed=True):
+        """model options of one gene, or of every gene with gene < 0"""
+        o = _Opts(int(bool(fix_omega)), float(omega_fix), int(bool(p_transformed)))
+        _check(lib().bsa_batch_set_model(self._h, int(gene), C.byref(o)))
+        for i in (range(len(self.genes)) if gene < 0 else [gene]):
+            self.np[i] = lib().bsa_batch_num_params(self._h, i)
+            self.fix_omega[i], self.omega_fix[i] = bool(fix_omega), float(omega_fix)
+
+    def use_graph(self, enable=True):
+        _check(lib().bsa_batch_set_graph(self._h, int(bool(enable))))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bsa_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _xs(self, xs):
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in xs]
+        if len(xs) != len(self.genes):
+            raise BsaError(f"{len(xs)} parameter vectors for {len(self.genes)} genes")
+        for i, x in enumerate(xs):
+            if len(x) != self.np[i]:
+                raise BsaError(f"gene {i}: x has {len(x)} values, expected {self.np[i]}")
+        return xs
+
+    def lnl(self, xs):
+        """-lnL of every gene at xs[gene]"""
+        xs = self._xs(xs)
+        out = np.zeros(len(xs))
+        _check(lib().bsa_batch_lnl(self._h, _ptrs(xs), _dptr(out)))
+        return out
+
+    def lnl_grad(self, xs):
+        """(-lnL per gene, list of d(-lnL)/dx per gene) by the adjoint sweep"""
+        xs = self._xs(xs)
+        out = np.zeros(len(xs))
+        gs = [np.zeros(len(x)) for x in xs]
+        _check(lib().bsa_batch_lnl_grad(self._h, _ptrs(xs), _dptr(out), _ptrs(gs)))
+        return out, gs
+
+    def default_bounds(self, gene):
+        lo, hi = np.zeros(self.np[gene]), np.zeros(self.np[gene])
+        _check(lib().bsa_batch_default_bounds(self._h, gene, _dptr(lo), _dptr(hi)))
+        return lo, hi
+
+    def fit(self, x0s, e=1e-8, max_iter=2000, max_step=5.0):
+        """Lockstep ML fits of every gene from x0s[gene] (bounded BFGS,
+        adjoint gradient, codeml's bounds).  Returns (list of x, -lnL per
+        gene, list of info dicts)."""
+        xs = [np.array(x, dtype=np.float64) for x in self._xs(x0s)]
+        o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
+        out = np.zeros(len(xs))
+        info = np.zeros(3 * len(xs), dtype=np.int32)
+        _check(lib().bsa_batch_fit(self._h, _ptrs(xs), None, None, C.byref(o), _dptr(out),
+                                   info.ctypes.data_as(C.POINTER(C.c_int))))
+        infos = [{"iterations": int(info[3 * i]), "evaluations": int(info[3 * i + 1]),
+                  "converged": bool(info[3 * i + 2])} for i in range(len(xs))]
+        return xs, out, infos
+
+    def fhK(self, gene):
+        n = self.genes[gene][1].npatt
+        f, s = np.zeros(4 * n), np.zeros(n)
+        _check(lib().bsa_batch_class_lnl(self._h, gene, f.ctypes.data, s.ctypes.data))
+        return f.reshape(4, -1), s
+
+    def last_params(self, gene):
+        o = np.zeros(11)
+        _check(lib().bsa_batch_last_params(self._h, gene, _dptr(o)))
+        return o
+
+    def work(self):
+        o = np.zeros(10)
+        _check(lib().bsa_batch_work(self._h, _dptr(o)))
+        return o
+
+    def timing(self, reset=False):
+        o = np.zeros(17)
+        _check(lib().bsa_batch_timing(self._h, _dptr(o), int(reset)))
+        keys = ["pmat", "prune", "root", "adjoint", "branch", "finish", "wall"]
+        d = {f"last_{k}": float(o[i]) for i, k in enumerate(keys)}
+        d.update({f"sum_{k}": float(o[7 + i]) for i, k in enumerate(keys)})
+        d["n_eval"], d["n_launch"], d["n_adj_launch"] = float(o[14]), float(o[15]), float(o[16])
+        return d
+
+
+def forest_plan(genes, precision="fp64"):
+    """Host-only plan of a batch of (tree, data) genes, every schedule index
+    checked (bsa_debug_forest_plan): dict of its sizes."""
+    keep = []
+    probs = [_problem(t, d, precision, "mfma", 0, 0, None, keep) for t, d in genes]
+    arr = (C.POINTER(_Problem) * len(probs))(*[C.pointer(p) for p in probs])
+    o = np.zeros(8)
+    _check(lib().bsa_debug_forest_plan(arr, len(probs), _dptr(o)))
+    keys = ["genes", "npad", "nslot", "levels", "prune_jobs", "adj_jobs", "eig_jobs", "nchunk"]
+    return {k: int(v) for k, v in zip(keys, o)}
+
+
 def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0):
     """The engine's bounded BFGS on a Python objective fun(x) -> (f, grad)
     (diagnostic; runs on the host).  Returns (x, f, info)."""
@@ -319,4 +478,4 @@ def mfma_probe(A, B, device=0):
     return d64.reshape(64, 4), d32.reshape(64, 4)
 
 
-__all__ = ["BranchSiteEngine", "BsaError", "x_from_params", "mfma_probe", "lib", "codons"]
+__all__ = ["BranchSiteEngine", "BranchSiteBatch", "BsaError", "x_from_params", "mfma_probe", "lib", "codons"]

@@ -6,11 +6,17 @@ gets the likelihood-ratio statistic; this is the reference's per-gene
 Forestry run of codeml (codeml.c:578-854), repeated per gene by separate
 processes there (codeml.c:357), and one engine per gene here.
 
-Genes run in parallel on a pool of host threads, each owning one engine, so
-one HIP stream, at a time: the optimiser loop runs in the library without
-the interpreter lock, and the small per-gene kernels of different genes
-overlap on the GPU.  Work is handed out largest-first (npatt x taxa), the
-LPT order, so the long genes do not end the scan alone.
+Two ways to run it.  ``scan``: genes in parallel on a pool of host threads,
+each owning one engine, so one HIP stream, at a time: the optimiser loop
+runs in the library without the interpreter lock, and the small per-gene
+kernels of different genes overlap on the GPU; work is handed out
+largest-first (npatt x taxa), the LPT order, so the long genes do not end
+the scan alone.  ``scan_batched``: genes of similar pattern count grouped
+into batches (one forest of genes per batch, bsa_batch_*), each batch's H1
+fits and then its H0 fits run in lockstep, so one launch per tree level
+serves every gene of the batch; a few batches run at once on host threads.
+Small genes are bound by launch processing, not by the GPU's throughput,
+which the batches amortise.
 
 The synthetic genome follows the 2,000-gene scan workload: Yule trees of
 12-64 taxa with one random foreground branch, codon counts lognormal with
@@ -105,4 +111,90 @@ def scan(genes, workers=8, precision="fp64", gradient="adjoint", device=0, e=1e-
                 for i in order}
         for i, f in futs.items():
             out[i] = f.result()
+    return out, time.perf_counter() - t0
+
+
+def _pad(n, p=128):
+    return (n + p - 1) // p * p
+
+
+def batch_groups(genes, batch_genes=64, max_waste=0.25):
+    """Gene indices in batches: largest pattern count first; a batch takes
+    genes while their padded pattern count is within `max_waste` of its
+    first gene's (the batch pads every gene to that count)."""
+    order = sorted(range(len(genes)), key=lambda i: -genes[i].data.npatt)
+    groups, cur = [], []
+    for i in order:
+        if cur and (len(cur) >= batch_genes or
+                    _pad(genes[i].data.npatt) < (1 - max_waste) * _pad(genes[cur[0]].data.npatt)):
+            groups.append(cur)
+            cur = []
+        cur.append(i)
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
+    """H1 and H0 fits and the LRT of a list of genes on one batch engine
+    (lockstep fits).  Same starting points and optimiser as ``fit_gene``;
+    a gene whose H0 maximum lies above its H1 one is fitted again under H1
+    from the H0 estimates on its own engine.  Returns the per-gene dicts
+    of ``fit_gene`` (seconds = the batch's wall time)."""
+    t0 = time.perf_counter()
+    n = len(genes)
+    b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision=precision,
+                               device=device)
+    b.use_graph(graph)
+    try:
+        starts = []
+        for g in genes:
+            s = codeml.default_start(g.tree, 2.0, 1.5, False)
+            s[:g.tree.nbranch] = 0.1
+            starts.append(s)
+        x1, f1, i1 = b.fit(starts, e=e)
+        par1 = [b.last_params(i) for i in range(n)]
+        b.set_model(-1, True, 1.0)
+        x0, f0, i0 = b.fit([x[:-1] for x in x1], e=e)
+    finally:
+        b.close()
+    wall = time.perf_counter() - t0
+    out = []
+    for i, g in enumerate(genes):
+        h1 = codeml.Fit("H1", -f1[i], x1[i], *par1[i][:5], par1[i][7:11].copy(),
+                        x1[i][:g.tree.nbranch].copy(), len(x1[i]), i1[i]["iterations"],
+                        i1[i]["evaluations"], i1[i]["converged"], wall, {})
+        h0 = codeml.Fit("H0", -f0[i], x0[i], 0, 0, 0, 0, 1.0, None, None, len(x0[i]),
+                        i0[i]["iterations"], i0[i]["evaluations"], i0[i]["converged"], wall, {})
+        if h0.lnL > h1.lnL + 1e-6:
+            h1b = codeml.fit_model_a(g.tree, g.data, False, 1.5, start=np.append(h0.x, 1.0),
+                                     precision=precision, e=e, device=device)
+            h1b.iterations += h1.iterations
+            h1b.evaluations += h1.evaluations
+            h1 = h1b if h1b.lnL > h1.lnL else h1
+        r = codeml.lrt(h0, h1)
+        out.append({"gene": g.name, "ns": g.tree.ns, "npatt": g.data.npatt,
+                    "lnL1": h1.lnL, "lnL0": h0.lnL, "stat": r["stat"], "p": r["p_mixture"],
+                    "w2": h1.w2, "x1": h1.x.tolist(),
+                    "params1": [h1.kappa, h1.p0, h1.p1, h1.w0, h1.w2],
+                    "iterations": h1.iterations + h0.iterations,
+                    "evaluations": h1.evaluations + h0.evaluations,
+                    "converged": h1.converged and h0.converged, "seconds": wall})
+    return out
+
+
+def scan_batched(genes, batch_genes=64, concurrent=2, precision="fp64", device=0, e=1e-8,
+                 graph=False, max_waste=0.25):
+    """Fit every gene (H1, H0, LRT) in batches of up to `batch_genes` genes
+    of similar size, `concurrent` batches at a time.  Returns (results in
+    input order, wall seconds)."""
+    groups = batch_groups(genes, batch_genes, max_waste)
+    out = [None] * len(genes)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=max(1, concurrent)) as ex:
+        futs = [(grp, ex.submit(fit_batch, [genes[i] for i in grp], precision, device, e, graph))
+                for grp in groups]
+        for grp, f in futs:
+            for i, r in zip(grp, f.result()):
+                out[i] = r
     return out, time.perf_counter() - t0

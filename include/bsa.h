@@ -211,6 +211,52 @@ typedef int (*bsa_objective)(const double *x, double *f, double *grad, void *ctx
 int  bsa_debug_minimize(int n, bsa_objective fn, void *ctx, double *x, const double *lo,
                         const double *hi, const bsa_fit_opts *opts, double *fmin, int *info3);
 
+/* ------------------------------------------------------------------------
+ * Batch engine: many genes, each with its own tree, alignment and model
+ * options, evaluated together on one GPU (the genome scan: the reference
+ * runs codeml once per gene, one process each, codeml.c:357; each gene's
+ * likelihood is the one bsa_lnl computes).  The genes' jobs share one set
+ * of device buffers, so one evaluation of all of them costs one launch per
+ * tree level of the deepest tree; patterns are padded to the largest gene's
+ * count, so genes of similar size batch best.  Arrays of per-gene pointers
+ * are indexed by gene (creation order).  No counterpart in the reference.
+ * ------------------------------------------------------------------------ */
+typedef struct bsa_batch bsa_batch;
+
+/* all problems share device, precision, kernel and tile */
+int  bsa_batch_create(const bsa_problem *const *probs, int ngene, bsa_batch **out);
+int  bsa_batch_destroy(bsa_batch *batch);
+int  bsa_batch_size(const bsa_batch *batch);                    /* genes, or -1 */
+/* model options of one gene, or of every gene with gene < 0 */
+int  bsa_batch_set_model(bsa_batch *batch, int gene, const bsa_model_opts *opts);
+int  bsa_batch_num_params(const bsa_batch *batch, int gene);     /* np, or -1 */
+int  bsa_batch_default_bounds(const bsa_batch *batch, int gene, double *lo, double *hi);
+/* -lnL of every gene at x[gene] (np of that gene values each) */
+int  bsa_batch_lnl(bsa_batch *batch, const double *const *x, double *neg_lnl);
+/* -lnL and d(-lnL)/dx of every gene (adjoint sweep, as bsa_lnl_grad) */
+int  bsa_batch_lnl_grad(bsa_batch *batch, const double *const *x, double *neg_lnl,
+                        double *const *grad);
+/* ML fits of every gene from x[gene] (bsa_fit's optimiser, adjoint
+ * gradient only), run in lockstep: each evaluation of the batch serves
+ * every gene still being fitted.  lo/hi (either, or any entry, may be NULL:
+ * codeml's bounds); info[3*gene .. 3*gene+2] as bsa_fit's info (may be
+ * NULL); every gene's last evaluation is left at its estimates. */
+int  bsa_batch_fit(bsa_batch *batch, double *const *x, const double *const *lo,
+                   const double *const *hi, const bsa_fit_opts *opts, double *neg_lnl,
+                   int *info);
+/* per-gene outputs of the last evaluation, as bsa_class_lnl / bsa_last_params */
+int  bsa_batch_class_lnl(bsa_batch *batch, int gene, double *fhK, double *site_lnf);
+int  bsa_batch_last_params(const bsa_batch *batch, int gene, double *out11);
+/* as bsa_timing / bsa_work / bsa_set_graph, for the whole batch */
+int  bsa_batch_timing(bsa_batch *batch, double *out17, int reset);
+int  bsa_batch_work(bsa_batch *batch, double *out10);
+int  bsa_batch_set_graph(bsa_batch *batch, int enable);
+/* diagnostic, host only: plan a batch of the given genes without a device
+ * and check every index of its schedules against the buffers they index;
+ * out8 = {genes, padded patterns, partial slots, pruning launches, pruning
+ * jobs, adjoint jobs, eigen-basis jobs, branch-matrix chunks} */
+int  bsa_debug_forest_plan(const bsa_problem *const *probs, int ngene, double *out8);
+
 const char *bsa_last_error(void);
 
 /* diagnostic: one 16x16x4 matrix-core product in FP64 and in FP32 of

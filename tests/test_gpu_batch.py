@@ -1,0 +1,171 @@
+"""The batch engine on the GPU: a forest of genes of different sizes (3 to
+24 taxa, 20 to 400 codons, with and without gaps, each with its own
+foreground branch and model options) gives every gene the likelihood,
+per-class values and gradient of that gene's own single-gene engine (FP64:
+the same kernels on the same data, so lnL bitwise and gradients to the
+rounding of the branch-matrix chunk sums); the likelihood of one gene
+against the oracle; FP32 within its error budget; graph replay equal to
+eager launches; lockstep fits equal to one-gene fits; the batched genome
+scan equal to the per-gene scan."""
+import numpy as np
+import pytest
+
+from cucodeml_amd import codeml, engine, formats, scan, simulate
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make_gene(rng, ns, ncodon, gap_frac=0.0):
+    tree = simulate.yule_tree(ns, 0.1, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    p = simulate.uniform_params(rng)
+    seqs, _ = simulate.simulate_model_a(tree, ncodon, p["kappa"], p["w0"], p["w2"], p["p0"],
+                                        p["p1"], pi, rng, gap_frac=gap_frac)
+    data = formats.load_codon_data(tree.names, seqs)
+    x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"],
+                             p["w2"])
+    return tree, data, x
+
+
+@pytest.fixture(scope="module")
+def genes():
+    rng = np.random.default_rng(77)
+    spec = [(6, 60, 0.0), (24, 400, 0.1), (3, 20, 0.0), (11, 150, 0.0), (17, 90, 0.05),
+            (8, 300, 0.0), (5, 45, 0.2)]
+    return [make_gene(rng, ns, nc, gf) for ns, nc, gf in spec]
+
+
+def singles(genes, precision="fp64", fixed=()):
+    out = []
+    for i, (tree, data, x) in enumerate(genes):
+        e = engine.BranchSiteEngine(tree, data, precision=precision)
+        try:
+            xi = x
+            if i in fixed:
+                e.set_model(True, 1.0)
+                xi = x[:-1]
+            f, g = e.lnl_grad(xi)
+            fh, site = e.fhK()
+            out.append((f, g, fh, site))
+        finally:
+            e.close()
+    return out
+
+
+def test_batch_equals_single_engines_fp64(genes):
+    say(f"case: batch of {len(genes)} genes fp64 vs single engines (genes 2, 5 under H0)")
+    fixed = (2, 5)
+    want = singles(genes, fixed=fixed)
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in genes])
+    try:
+        for i in fixed:
+            b.set_model(i, True, 1.0)
+        xs = [x[:-1] if i in fixed else x for i, (_, _, x) in enumerate(genes)]
+        f = b.lnl(xs)
+        fg, gs = b.lnl_grad(xs)
+        for i, (fw, gw, fhw, sw) in enumerate(want):
+            fh, site = b.fhK(i)
+            rg = np.abs(gs[i] - gw).max() / np.abs(gw).max()
+            say(f"  gene {i}: ns {genes[i][0].ns} npatt {genes[i][1].npatt} -lnL {f[i]:.10f} "
+                f"single {fw:.10f} grad rel {rg:.1e}")
+            assert f[i] == fw and fg[i] == fw
+            assert np.array_equal(fh, fhw) and np.array_equal(site, sw)
+            assert rg <= 1e-12
+        t = b.timing()
+        say(f"  launches per evaluation: {t['n_launch'] / t['n_eval']:.0f} pruning")
+    finally:
+        b.close()
+
+
+def test_batch_gene_matches_oracle(genes):
+    say("case: batch gene 3 vs the oracle")
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in genes])
+    try:
+        f = b.lnl([x for _, _, x in genes])
+        par = b.last_params(3)
+    finally:
+        b.close()
+    tree, data, x = genes[3]
+    tree.set_branch_vector(x[:tree.nbranch])
+    want, _, _ = oracle.lnl(tree, data, *par[:5])
+    say(f"  lnL {-f[3]:.10f} oracle {want:.10f}")
+    assert abs(-f[3] - want) <= 1e-9 * abs(want)
+
+
+def test_batch_fp32_within_budget(genes):
+    say("case: batch fp32 vs single fp64")
+    want = singles(genes)
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in genes], precision="fp32")
+    try:
+        f, gs = b.lnl_grad([x for _, _, x in genes])
+    finally:
+        b.close()
+    for i, (fw, gw, _, _) in enumerate(want):
+        rg = np.linalg.norm(gs[i] - gw) / np.linalg.norm(gw)
+        say(f"  gene {i}: |d lnL| {abs(f[i] - fw):.2e} grad rel {rg:.1e}")
+        assert abs(f[i] - fw) <= 1e-5 * max(1.0, genes[i][1].npatt / 1e3)
+        assert rg <= 1e-4
+
+
+def test_batch_graph_equals_eager(genes):
+    say("case: batch graph replay vs eager")
+    xs = [x for _, _, x in genes]
+    rng = np.random.default_rng(5)
+    pts = [[x * (1 + 0.01 * rng.standard_normal(len(x))) for x in xs] for _ in range(3)]
+    res = {}
+    for graph in (False, True):
+        b = engine.BranchSiteBatch([(t, d) for t, d, _ in genes])
+        b.use_graph(graph)
+        try:
+            res[graph] = [b.lnl_grad(p) for p in pts]
+        finally:
+            b.close()
+    for (fa, ga), (fb, gb) in zip(res[False], res[True]):
+        assert np.array_equal(fa, fb)
+        for a, c in zip(ga, gb):
+            assert np.array_equal(a, c)
+
+
+def test_lockstep_fits_equal_single_fits(genes):
+    say("case: lockstep batch fits vs single-engine fits")
+    sub = genes[:5]
+    starts = []
+    for tree, _, _ in sub:
+        s = codeml.default_start(tree, 2.0, 1.5, False)
+        s[:tree.nbranch] = 0.1
+        starts.append(s)
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in sub])
+    try:
+        xb, fb, ib = b.fit(starts)
+    finally:
+        b.close()
+    for i, (tree, data, _) in enumerate(sub):
+        e = engine.BranchSiteEngine(tree, data)
+        try:
+            xs, fs, is_ = e.fit(starts[i])
+        finally:
+            e.close()
+        say(f"  gene {i}: batch -lnL {fb[i]:.8f} ({ib[i]['iterations']} it), single "
+            f"{fs:.8f} ({is_['iterations']} it)")
+        assert ib[i]["converged"]
+        assert abs(fb[i] - fs) <= 1e-6 * abs(fs)
+        assert np.abs(xb[i] - xs).max() <= 1e-3 * max(1.0, np.abs(xs).max())
+
+
+def test_batched_scan_equals_scan():
+    say("case: batched scan vs per-gene scan, 10 genes")
+    genes = scan.simulate_genes(10, seed=9, median=120, clip=(50, 300), taxa=(6, 14))
+    a, wa = scan.scan_batched(genes, batch_genes=4, concurrent=2)
+    c, wc = scan.scan(genes, workers=4)
+    say(f"  wall {wa:.2f} s batched vs {wc:.2f} s per gene")
+    for r, s in zip(a, c):
+        say(f"  {r['gene']}: lnL1 {r['lnL1']:.6f} / {s['lnL1']:.6f}  lnL0 {r['lnL0']:.6f} / "
+            f"{s['lnL0']:.6f}")
+        assert r["converged"]
+        assert abs(r["lnL1"] - s["lnL1"]) <= 1e-5 and abs(r["lnL0"] - s["lnL0"]) <= 1e-5
+        assert abs(r["stat"] - s["stat"]) <= 2e-5
