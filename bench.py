@@ -143,20 +143,30 @@ def measured_traffic(kernel):
     return None
 
 
-def genome_scan(ngenes, workers, precision):
+def genome_scan(ngenes, workers, precision, batch_genes, concurrent):
     """genes/hour of the genome scan (H1 + H0 fits and LRT per gene) on a
-    sample of the 2,000-gene workload."""
+    sample of the 2,000-gene workload: genes of similar size batched into
+    forests (batch_genes > 0; `concurrent` forests at a time) or one engine
+    per gene on `workers` threads (batch_genes = 0)."""
     from cucodeml_amd import scan
     genes = scan.simulate_genes(ngenes)
-    scan.scan(genes[:min(2, ngenes)], workers=1, precision=precision)   # warm-up
-    res, wall = scan.scan(genes, workers=workers, precision=precision)
+    if batch_genes > 0:
+        scan.scan_batched(genes[:min(4, ngenes)], batch_genes=4, concurrent=1,
+                          precision=precision)   # warm-up
+        res, wall = scan.scan_batched(genes, batch_genes=batch_genes, concurrent=concurrent,
+                                      precision=precision)
+        how = {"mode": "batched", "batch_genes": batch_genes, "concurrent_batches": concurrent}
+    else:
+        scan.scan(genes[:min(2, ngenes)], workers=1, precision=precision)   # warm-up
+        res, wall = scan.scan(genes, workers=workers, precision=precision)
+        how = {"mode": "engine per gene", "workers": workers}
     return {"value": ngenes / wall * 3600.0, "unit": "genes/hour",
             "sample": (f"first {ngenes} genes of the seeded 2,000-gene scan workload (Yule "
                        f"trees of {scan.SCAN_TAXA[0]}-{scan.SCAN_TAXA[1]} taxa, lognormal codon "
                        f"counts median {scan.SCAN_MEDIAN}, sigma {scan.SCAN_SIGMA}, clipped to "
                        f"{list(scan.SCAN_CLIP)})"),
             "per_gene": "H1 fit + H0 fit (bounded BFGS, adjoint gradient) + LRT",
-            "precision": precision, "workers": workers, "wall_s": wall,
+            "precision": precision, **how, "wall_s": wall,
             "converged": int(sum(r["converged"] for r in res)),
             "mean_evaluations": float(np.mean([r["evaluations"] for r in res]))}
 
@@ -177,9 +187,14 @@ def main():
                     help="steps of the FP64 forward-difference path timed beside (0: skip)")
     ap.add_argument("--cpu-sample-codons", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scan-genes", type=int, default=32,
+    ap.add_argument("--scan-genes", type=int, default=256,
                     help="genes of the genome-scan sample timed beside (0: skip)")
-    ap.add_argument("--scan-workers", type=int, default=8)
+    ap.add_argument("--scan-batch", type=int, default=256,
+                    help="genes per batch engine in the scan (0: one engine per gene)")
+    ap.add_argument("--scan-concurrent", type=int, default=3,
+                    help="batches fitted at once in the scan")
+    ap.add_argument("--scan-workers", type=int, default=8,
+                    help="engines at once when the scan runs one engine per gene")
     ap.add_argument("--no-beb", dest="beb", action="store_false",
                     help="skip timing one Bayes empirical Bayes analysis beside")
     args = ap.parse_args()
@@ -275,7 +290,8 @@ def main():
         out["speedup_vs_fd64_gpu"] = value * fd_el
         out["lnL_error_vs_fp64"] = abs(f - f64)
     if args.scan_genes > 0:
-        out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, "fp64")
+        out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, "fp64",
+                                         args.scan_batch, args.scan_concurrent)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tree, seqs, p, args.codons, data.npatt, npar,
                                            args.cpu_sample_codons)

@@ -3,9 +3,10 @@
 // (bsa_forest.h), so one evaluation of all of them costs one launch per tree
 // level of the deepest tree rather than one per level of every gene; the
 // host model set-up (three eigen systems per gene) and the chain rule run on
-// a pool of host threads.  bsa_batch_fit runs the genes' ML fits in
-// lockstep: one optimiser thread per gene, and an evaluation of the whole
-// forest whenever every gene still being fitted has asked for one.
+// a pool of host threads.  bsa_batch_fit runs one optimiser thread per gene
+// and serves their requests in rounds (continuous batching): each round
+// evaluates the genes whose points are staged, on the device, while the
+// other genes' threads prepare theirs.
 #include "bsa_fit.h"
 #include "bsa_forest.h"
 
@@ -60,43 +61,68 @@ int parallel_for(int n, Fn fn)
     return BSA_OK;
 }
 
-// Lockstep evaluation for the fit threads: each thread stages its gene and
-// arrives; the thread that completes the set of genes still being fitted
-// runs the forest's evaluation and releases the others.  A thread whose fit
-// has ended leaves, which may complete the set itself.  Genes that have
-// left are evaluated again at their last staged point (their results are
-// not read).
-struct Lockstep {
+// Continuous batching of the fits' evaluation requests.  A fit thread stages
+// its gene's point (host model set-up) and queues the gene; the server
+// thread runs one round over every gene queued, whenever the device is
+// free, while further requests queue for the next round.  A round launches
+// the jobs of its genes only, so a gene costs device time only for the
+// points its own fit asks for, and a gene's result does not depend on which
+// other genes share its round.
+struct RoundServer {
     std::mutex mu;
-    std::condition_variable cv;
+    std::condition_variable cv_server;
+    std::unique_ptr<std::condition_variable[]> cv_gene;
     Forest *f = nullptr;
-    int active = 0, arrived = 0, rc = BSA_OK;
-    std::string err;
-    unsigned long generation = 0;
+    std::vector<int> queue;          // staged genes waiting for a round
+    std::vector<char> ready;         // per gene: the results of its request are in
+    std::vector<int> grc;            // per gene: status of its last round
+    std::vector<std::string> gerr;
+    int fitting = 0;                 // fits still running
+    long rounds = 0;
 
-    void run_locked()
+    explicit RoundServer(Forest *fo) : cv_gene(new std::condition_variable[fo->G]), f(fo)
     {
-        rc = forest_run(*f, true);
-        if (rc) err = last_error_text();
-        arrived = 0;
-        generation++;
-        cv.notify_all();
+        ready.assign(f->G, 0);
+        grc.assign(f->G, BSA_OK);
+        gerr.assign(f->G, std::string());
+        fitting = f->G;
     }
-    int arrive()
+    int request(int g)
     {
         std::unique_lock<std::mutex> lk(mu);
-        const unsigned long gen = generation;
-        if (++arrived == active)
-            run_locked();
-        else
-            cv.wait(lk, [&] { return generation != gen; });
-        return rc ? fail(rc, "%s", err.c_str()) : BSA_OK;
+        ready[g] = 0;
+        queue.push_back(g);
+        cv_server.notify_one();
+        cv_gene[g].wait(lk, [&] { return ready[g] != 0; });
+        return grc[g] ? fail(grc[g], "%s", gerr[g].c_str()) : BSA_OK;
     }
-    void leave()
+    void finished()
     {
         std::unique_lock<std::mutex> lk(mu);
-        active--;
-        if (active > 0 && arrived == active) run_locked();
+        fitting--;
+        cv_server.notify_one();
+    }
+    void serve()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_server.wait(lk, [&] { return !queue.empty() || fitting == 0; });
+            if (queue.empty()) return;   // every fit has finished
+            std::vector<int> gl;
+            gl.swap(queue);
+            lk.unlock();
+            std::sort(gl.begin(), gl.end());
+            const int rc = forest_run_round(*f, gl, true);
+            const std::string err = rc ? last_error_text() : std::string();
+            lk.lock();
+            rounds++;
+            for (int g : gl) {
+                grc[g] = rc;
+                gerr[g] = err;
+                ready[g] = 1;
+                cv_gene[g].notify_one();
+            }
+        }
     }
 };
 
@@ -228,6 +254,41 @@ extern "C" int bsa_batch_lnl_grad(bsa_batch *b, const double *const *x, double *
     return batch_eval(b, x, neg_lnl, grad);
 }
 
+extern "C" int bsa_batch_eval_genes(bsa_batch *b, const int *genes, int n, const double *const *x,
+                                    double *neg_lnl, double *const *grad)
+{
+    if (!b || !genes || n <= 0 || !x || !neg_lnl) return fail(BSA_EINVAL, "bad argument");
+    const int G = (int)b->genes.size();
+    std::vector<int> gl(genes, genes + n);
+    for (int i = 0; i < n; i++) {
+        if (gl[i] < 0 || gl[i] >= G || (i > 0 && gl[i] <= gl[i - 1]))
+            return fail(BSA_EINVAL, "genes must be ascending indices below %d", G);
+        if (!x[i] || (grad && !grad[i])) return fail(BSA_EINVAL, "null array for entry %d", i);
+    }
+    const bool adjoint = grad != nullptr;
+    Forest &f = b->f;
+    HIP_SHARED;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(f.dev));
+    int rc = adjoint ? forest_setup_adjoint(f) : BSA_OK;
+    if (rc) return rc;
+    rc = parallel_for(n, [&](int i) { return gene_stage_x(*b->genes[gl[i]], f, x[i], adjoint); });
+    if (rc) return rc;
+    if ((rc = forest_run_round(f, gl, adjoint))) return rc;
+    for (int i = 0; i < n; i++) neg_lnl[i] = -gene_collect(*b->genes[gl[i]], f);
+    if (adjoint)
+        parallel_for(n, [&](int i) {
+            gene_chain_rule(*b->genes[gl[i]], f, x[i], grad[i]);
+            return BSA_OK;
+        });
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    f.timing[6] = ms;
+    f.timing_sum[6] += ms;
+    f.n_eval += 1;
+    return BSA_OK;
+}
+
 extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const *lo,
                              const double *const *hi, const bsa_fit_opts *opts, double *neg_lnl,
                              int *info)
@@ -250,16 +311,12 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
             if (!(blo[i][k] <= bhi[i][k])) return fail(BSA_EINVAL, "gene %d: bounds with lo > hi", i);
     }
     Forest &f = b->f;
-    int rc = forest_prepare_graph(f, true);
-    if (!rc) rc = forest_prepare_graph(f, false);   // the closing evaluation
-    if (rc) return rc;
     HIP_SHARED;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(f.dev));
-    if ((rc = forest_setup_adjoint(f))) return rc;
-    Lockstep ls;
-    ls.f = &f;
-    ls.active = G;
+    int rc = forest_setup_adjoint(f);
+    if (rc) return rc;
+    RoundServer srv(&f);
     std::vector<FitResult> res(G);
     std::vector<int> rcs(G, BSA_OK);
     std::vector<std::string> errs(G);
@@ -269,7 +326,7 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
         ObjFn fg = [&](const double *xx, double *fv, double *gv) {
             int r = gene_stage_x(g, f, xx, true);
             if (r) return r;
-            if ((r = ls.arrive())) return r;
+            if ((r = srv.request(i))) return r;
             *fv = -gene_collect(g, f);
             gene_chain_rule(g, f, xx, gv);
             return BSA_OK;
@@ -279,20 +336,24 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
             rcs[i] = r;
             errs[i] = last_error_text();
         }
-        ls.leave();
+        srv.finished();
     };
     {
         std::vector<std::thread> th;
         th.reserve(G);
         for (int i = 0; i < G; i++) th.emplace_back(fit_one, i);
+        srv.serve();   // HIP work on this thread only
         for (auto &t : th) t.join();
     }
+    f.n_eval += (double)srv.rounds;
     for (int i = 0; i < G; i++)
         if (rcs[i]) return fail(rcs[i], "gene %d: %s", i, errs[i].c_str());
     // leave every gene's last evaluation at its estimates
     rc = parallel_for(G, [&](int i) { return gene_stage_x(*b->genes[i], f, x[i], false); });
     if (rc) return rc;
-    if ((rc = forest_run(f, false))) return rc;
+    std::vector<int> all(G);
+    for (int i = 0; i < G; i++) all[i] = i;
+    if ((rc = forest_run_round(f, all, false))) return rc;
     for (int i = 0; i < G; i++) {
         neg_lnl[i] = -gene_collect(*b->genes[i], f);
         if (info) {

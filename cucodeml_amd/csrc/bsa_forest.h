@@ -36,8 +36,6 @@ const char *last_error_text();
 extern std::shared_mutex g_hip_mu;
 #define HIP_SHARED std::shared_lock<std::shared_mutex> hip_shared_lock_(::bsa::g_hip_mu)
 
-constexpr int kUVRSet = 64 * 64 * 2 + 64;   // U, V (64 x 64 each), root (64)
-
 struct Gene {
     int ns = 0, nnode = 0, root = 0, nbranch = 0, npatt = 0, ncodes = 0, namb = 0;
     std::vector<int> father, label, ibranch, height;
@@ -71,6 +69,9 @@ struct Gene {
     int gi = 0;                               // gene index
     int tip_base = 0, int_base = 0, slot_base = 0, pj_base = 0, ej_base = 0;
     int amb_base = 0, mask_base = 0;
+    // its jobs in the forest's arrays, per launch: pruning level l, adjoint
+    // depth d (interior children, then tips), [first, end)
+    std::vector<std::array<int, 2>> prune_rng, adj_int_rng, adj_tip_rng;
 
     // model cache and last evaluation
     ModelA model{};
@@ -107,7 +108,7 @@ struct Forest {
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // lnL only, with gradient
-    // model block: 3 G U/V/root sets, then 8 class-proportion words per gene
+    // model blocks of the genes (kGeneBlock words each)
     double *d_uvr = nullptr;
     PmatJob *d_pjobs = nullptr;
     PruneJob *d_prune = nullptr;
@@ -123,6 +124,9 @@ struct Forest {
     double *h_uvr = nullptr;                       // same layout as d_uvr
     double *h_out = nullptr;                       // [G][kRootSums]
     PmatJob *h_pjobs = nullptr;
+    // job lists of a round (forest_run_round): pinned host and device copies
+    int *h_sel = nullptr, *d_sel = nullptr;
+    size_t sel_cap = 0;
 
     // pmat, prune, root, child adjoints, branch matrices, adjoint finish, wall
     double timing[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -146,9 +150,7 @@ struct Forest {
     double *d_mask64 = nullptr;
     double *h_adjout = nullptr;                    // dtau[nej], then Z[3 G][4096]
 
-    double *h_fk() const { return h_uvr + (size_t)3 * G * kUVRSet; }
-    double *d_fk() const { return d_uvr + (size_t)3 * G * kUVRSet; }
-    size_t model_block() const { return (size_t)3 * G * kUVRSet + (size_t)8 * G; }
+    size_t model_block() const { return (size_t)G * kGeneBlock; }
 };
 
 // renumber the genes' jobs into one forest (host only)
@@ -176,6 +178,11 @@ int gene_stage_x(Gene &g, Forest &f, const double *x, bool adjoint);
 // device work of one evaluation of every gene (inputs staged), results
 // copied back; then gene_collect reads gene g's lnL
 int forest_run(Forest &f, bool adjoint);
+// the same for the listed genes only (ascending gene indices, inputs
+// staged): their jobs alone are launched and only their inputs and outputs
+// cross PCIe; other genes' staging buffers may be written meanwhile.  Eager
+// launches (no graph).
+int forest_run_round(Forest &f, const std::vector<int> &genes, bool adjoint);
 double gene_collect(Gene &g, const Forest &f);
 // d(-lnL)/dx of gene g from the adjoint outputs of the last forest_run
 void gene_chain_rule(const Gene &g, const Forest &f, const double *x, double *grad);

@@ -395,7 +395,7 @@ void forest_free(Forest &f)
     void *dev_ptrs[] = {f.d_uvr, f.d_pjobs, f.d_prune, f.d_rargs, f.d_PT, f.d_AT, f.d_L, f.d_E,
                         f.d_z, f.d_mask_amb, f.d_pi, f.d_fpatt, f.d_fh, f.d_site, f.d_partial,
                         f.d_out, f.d_adj, f.d_ejobs, f.d_set_off, f.d_set_list, f.d_A, f.d_C,
-                        f.d_EA, f.d_Mpart, f.d_W, f.d_G, f.d_dtau, f.d_Z, f.d_mask64};
+                        f.d_EA, f.d_Mpart, f.d_W, f.d_G, f.d_dtau, f.d_Z, f.d_mask64, f.d_sel};
     for (void *p : dev_ptrs)
         if (p) hipFree(p);
     if (f.h_uvr) hipHostFree(f.h_uvr);
@@ -403,6 +403,7 @@ void forest_free(Forest &f)
     if (f.h_pjobs) hipHostFree(f.h_pjobs);
     if (f.h_ejobs) hipHostFree(f.h_ejobs);
     if (f.h_adjout) hipHostFree(f.h_adjout);
+    if (f.h_sel) hipHostFree(f.h_sel);
     for (auto &v : f.ev)
         if (v) hipEventDestroy(v);
     for (auto &gr : f.graph)
@@ -471,10 +472,13 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
     }
     f.prune.clear();
     f.level_off.assign(1, 0);
+    for (Gene *gp : genes) gp->prune_rng.assign(nlev, {0, 0});
     for (size_t l = 0; l < nlev; l++) {
         for (Gene *gp : genes) {
-            const Gene &g = *gp;
+            Gene &g = *gp;
             if (l + 1 >= g.level_off.size()) continue;
+            g.prune_rng[l][0] = (int)f.prune.size();
+            g.prune_rng[l][1] = (int)f.prune.size() + g.level_off[l + 1] - g.level_off[l];
             for (int i = g.level_off[l]; i < g.level_off[l + 1]; i++) {
                 PruneJob j = g.prune[i];
                 j.out_slot = gslot(g, j.out_slot);
@@ -581,6 +585,10 @@ void forest_plan_adjoint(Forest &f)
     // per depth: the interior-child jobs of every gene, then the tip-child
     // jobs; the branch-matrix row of a job is its position
     std::vector<std::vector<int>> gidx(f.G);   // gene-local job -> forest job
+    for (Gene *gp : f.genes) {
+        gp->adj_int_rng.assign(maxd, {0, 0});
+        gp->adj_tip_rng.assign(maxd, {0, 0});
+    }
     f.adj.clear();
     f.adj_off.assign(1, 0);
     f.adj_nint.clear();
@@ -588,11 +596,14 @@ void forest_plan_adjoint(Forest &f)
         int nint = 0;
         for (int pass = 0; pass < 2; pass++)
             for (Gene *gp : f.genes) {
-                const Gene &g = *gp;
+                Gene &g = *gp;
                 if (d + 1 >= g.adj_off.size()) continue;
                 const int a = g.adj_off[d], b = g.adj_off[d + 1], ni = g.adj_nint[d];
                 const int lo = pass == 0 ? a : a + ni, hi = pass == 0 ? a + ni : b;
                 if (pass == 0) nint += ni;
+                std::array<int, 2> &rng = pass == 0 ? g.adj_int_rng[d] : g.adj_tip_rng[d];
+                rng[0] = (int)f.adj.size();
+                rng[1] = (int)f.adj.size() + (hi - lo);
                 std::vector<int> &gx = gidx[g.gi];
                 gx.resize(g.adj.size(), -1);
                 for (int i = lo; i < hi; i++) {
@@ -788,46 +799,95 @@ int forest_build(Forest &f, const std::vector<Gene *> &genes, int device, int pr
 
 // ---------------------------------------------------------------- evaluation
 
+// Job lists of one round (device pointers into Forest::d_sel and counts);
+// a null list means every job of that launch, in order.
+struct RoundSel {
+    const int *pj = nullptr;
+    int npj = 0;
+    std::vector<const int *> lev;       // per pruning level
+    std::vector<int> nlev;
+    const int *genes = nullptr;
+    int ngenes = 0;
+    std::vector<const int *> adl;       // per adjoint depth, interior children
+    std::vector<int> nadl;
+    const int *adm = nullptr;           // every adjoint job (branch matrices)
+    int nadm = 0;
+    const int *ej = nullptr;
+    int nej = 0;
+    const int *sets = nullptr;
+    int nsets = 0;
+};
+
 template <typename T>
-static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events)
+static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
+                          const RoundSel *rs = nullptr)
 {
 #define REC(i) \
     if (events) HIPCHK(hipEventRecord(f.ev[i], s))
     REC(0);
-    launch_pmat(s, f.d_uvr, f.d_pjobs, (int)f.pjobs.size(), (double *)f.d_PT, f.d_mask_amb,
-                f.ncodes, (double *)f.d_AT);
+    launch_pmat(s, f.d_uvr, f.d_pjobs, rs ? rs->pj : nullptr, rs ? rs->npj : (int)f.pjobs.size(),
+                (double *)f.d_PT, f.d_mask_amb, f.ncodes, (double *)f.d_AT);
     HIPCHK(hipGetLastError());
     REC(1);
     for (size_t l = 0; l + 1 < f.level_off.size(); l++) {
         const int a = f.level_off[l], b = f.level_off[l + 1];
-        launch_prune<T>(s, f.kernel, f.tile, f.d_prune + a, b - a, f.npad, f.d_z,
+        launch_prune<T>(s, f.kernel, f.tile, rs ? f.d_prune : f.d_prune + a,
+                        rs ? rs->lev[l] : nullptr, rs ? rs->nlev[l] : b - a, f.npad, f.d_z,
                         (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes, (T *)f.d_L,
                         f.d_E, adjoint ? (T *)f.d_C : nullptr);
     }
     HIPCHK(hipGetLastError());
     REC(2);
-    launch_root<T>(s, f.d_rargs + (adjoint ? f.G : 0), f.G, (const T *)f.d_L, f.d_E, f.d_pi,
-                   f.d_fk(), f.d_fpatt, f.d_fh, f.d_site, f.d_partial, f.nroot_blk, f.d_out,
-                   (T *)f.d_A, f.d_EA);
+    launch_root<T>(s, f.d_rargs + (adjoint ? f.G : 0), rs ? rs->genes : nullptr,
+                   rs ? rs->ngenes : f.G, (const T *)f.d_L, f.d_E, f.d_pi, f.d_uvr, f.d_fpatt,
+                   f.d_fh, f.d_site, f.d_partial, f.nroot_blk, f.d_out, (T *)f.d_A, f.d_EA);
     HIPCHK(hipGetLastError());
     REC(3);
     if (adjoint) {
         for (size_t d = 0; d + 1 < f.adj_off.size(); d++)
-            launch_adjoint_level<T>(s, f.d_adj + f.adj_off[d], f.adj_nint[d], f.npad, f.d_z,
-                                    (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
-                                    f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA);
+            launch_adjoint_level<T>(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
+                                    rs ? rs->adl[d] : nullptr, rs ? rs->nadl[d] : f.adj_nint[d],
+                                    f.npad, f.d_z, (const double *)f.d_PT, (const double *)f.d_AT,
+                                    f.ncodes, f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA);
         REC(6);
-        launch_adjoint_m<T>(s, f.d_adj, (int)f.adj.size(), f.npad, f.nchunk, f.tpc, f.d_z,
-                            (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes, f.d_mask64,
-                            (const T *)f.d_L, f.d_E, (const T *)f.d_C, (const T *)f.d_A, f.d_EA,
-                            f.d_Mpart);
+        launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
+                            f.npad, f.nchunk, f.tpc, f.d_z, (const double *)f.d_PT,
+                            (const double *)f.d_AT, f.ncodes, f.d_mask64, (const T *)f.d_L, f.d_E,
+                            (const T *)f.d_C, (const T *)f.d_A, f.d_EA, f.d_Mpart);
         REC(4);
-        launch_adjoint_finish(s, f.d_Mpart, f.nchunk, f.d_ejobs, (int)f.ejobs.size(), 3 * f.G,
-                              f.d_set_off, f.d_set_list, f.d_uvr, f.d_G, f.d_dtau, f.d_W, f.d_Z);
+        launch_adjoint_finish(s, f.d_Mpart, f.nchunk, f.d_ejobs, rs ? rs->ej : nullptr,
+                              rs ? rs->nej : (int)f.ejobs.size(), rs ? rs->sets : nullptr,
+                              rs ? rs->nsets : 3 * f.G, f.d_set_off, f.d_set_list, f.d_uvr, f.d_G,
+                              f.d_dtau, f.d_W, f.d_Z);
         HIPCHK(hipGetLastError());
         REC(5);
     }
 #undef REC
+    return BSA_OK;
+}
+
+// device times of the phases of the last evaluation from the stream's events
+static int read_timing(Forest &f, bool adjoint, int nadj_launch)
+{
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, f.ev[0], f.ev[1]));
+    f.timing[0] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, f.ev[1], f.ev[2]));
+    f.timing[1] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, f.ev[2], f.ev[3]));
+    f.timing[2] = ms;
+    f.timing[3] = f.timing[4] = f.timing[5] = 0;
+    if (adjoint) {
+        HIPCHK(hipEventElapsedTime(&ms, f.ev[3], f.ev[6]));
+        f.timing[3] = ms;
+        HIPCHK(hipEventElapsedTime(&ms, f.ev[6], f.ev[4]));
+        f.timing[4] = ms;
+        HIPCHK(hipEventElapsedTime(&ms, f.ev[4], f.ev[5]));
+        f.timing[5] = ms;
+        f.n_adj_launch += nadj_launch;
+    }
+    for (int i = 0; i < 6; i++) f.timing_sum[i] += f.timing[i];
+    f.n_launch += (double)(f.level_off.size() - 1);
     return BSA_OK;
 }
 
@@ -864,36 +924,163 @@ static int run_device(Forest &f, bool adjoint)
     HIPCHK(hipMemcpyAsync(f.h_out, f.d_out, sizeof(double) * kRootSums * f.G, hipMemcpyDeviceToHost,
                           s));
     HIPCHK(hipStreamSynchronize(s));
-    float ms;
     if (f.use_graph) {
         // one device time for the whole evaluation (out[0] of bsa_timing)
+        float ms;
         HIPCHK(hipEventElapsedTime(&ms, f.ev[0], f.ev[5]));
         for (int i = 0; i < 6; i++) f.timing[i] = 0;
         f.timing[0] = ms;
         for (int i = 0; i < 6; i++) f.timing_sum[i] += f.timing[i];
         return BSA_OK;
     }
-    HIPCHK(hipEventElapsedTime(&ms, f.ev[0], f.ev[1]));
-    f.timing[0] = ms;
-    HIPCHK(hipEventElapsedTime(&ms, f.ev[1], f.ev[2]));
-    f.timing[1] = ms;
-    HIPCHK(hipEventElapsedTime(&ms, f.ev[2], f.ev[3]));
-    f.timing[2] = ms;
-    f.timing[3] = f.timing[4] = f.timing[5] = 0;
-    if (adjoint) {
-        HIPCHK(hipEventElapsedTime(&ms, f.ev[3], f.ev[6]));
-        f.timing[3] = ms;
-        HIPCHK(hipEventElapsedTime(&ms, f.ev[6], f.ev[4]));
-        f.timing[4] = ms;
-        HIPCHK(hipEventElapsedTime(&ms, f.ev[4], f.ev[5]));
-        f.timing[5] = ms;
-        int nl = 1;   // the branch-matrix launch, plus one per depth with interior children
-        for (int v : f.adj_nint) nl += v > 0;
-        f.n_adj_launch += nl;
+    int nl = 1;   // the branch-matrix launch, plus one per depth with interior children
+    for (int v : f.adj_nint) nl += v > 0;
+    return read_timing(f, adjoint, nl);
+}
+
+// copies of the per-gene ranges [base(g), base(g) + len(g)) of the listed
+// genes (ascending), runs of consecutive genes merged into one copy; all
+// byte offsets and lengths are multiples of `unit`
+template <class Base, class Len>
+static int copy_runs(hipStream_t s, void *dst, const void *src, size_t unit,
+                     const std::vector<int> &genes, Base base, Len len, hipMemcpyKind kind)
+{
+    size_t i = 0;
+    while (i < genes.size()) {
+        size_t j = i;
+        while (j + 1 < genes.size() && genes[j + 1] == genes[j] + 1) j++;
+        const size_t a = base(genes[i]), b = base(genes[j]) + len(genes[j]);
+        if (b > a)
+            HIPCHK(hipMemcpyAsync((char *)dst + a * unit, (const char *)src + a * unit,
+                                  (b - a) * unit, kind, s));
+        i = j + 1;
     }
-    for (int i = 0; i < 6; i++) f.timing_sum[i] += f.timing[i];
-    f.n_launch += (double)(f.level_off.size() - 1);
     return BSA_OK;
+}
+
+template <typename T>
+static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
+{
+    hipStream_t s = f.stream;
+    const size_t nlev = f.level_off.size() - 1, nd = adjoint ? f.adj_off.size() - 1 : 0;
+    // job lists: P(t), pruning levels, genes, child adjoints per depth,
+    // branch matrices, eigen-basis jobs, eigen sets
+    size_t cap = f.pjobs.size() + f.prune.size() + f.G + 2 * f.adj.size() + f.ejobs.size() +
+                 3 * (size_t)f.G;
+    if (cap > f.sel_cap) {
+        if (f.h_sel) HIPCHK(hipHostFree(f.h_sel));
+        if (f.d_sel) HIPCHK(hipFree(f.d_sel));
+        f.h_sel = nullptr;
+        f.d_sel = nullptr;
+        f.sel_cap = 0;
+        HIPCHK(hipHostMalloc((void **)&f.h_sel, sizeof(int) * cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&f.d_sel, sizeof(int) * cap));
+        f.sel_cap = cap;
+    }
+    int *h = f.h_sel;
+    size_t n = 0;
+    auto put_range = [&](int a, int b) {
+        for (int i = a; i < b; i++) h[n++] = i;
+    };
+    RoundSel rs;
+    std::vector<size_t> off;   // start of each list in h_sel, in launch order
+    off.push_back(n);
+    for (int g : gl) {
+        const Gene &ge = *f.genes[g];
+        put_range(ge.pj_base, ge.pj_base + (int)ge.pjobs.size());
+    }
+    rs.npj = (int)(n - off.back());
+    for (size_t l = 0; l < nlev; l++) {
+        off.push_back(n);
+        for (int g : gl) put_range(f.genes[g]->prune_rng[l][0], f.genes[g]->prune_rng[l][1]);
+        rs.nlev.push_back((int)(n - off.back()));
+    }
+    off.push_back(n);
+    for (int g : gl) h[n++] = g;
+    rs.ngenes = (int)gl.size();
+    if (adjoint) {
+        for (size_t d = 0; d < nd; d++) {
+            off.push_back(n);
+            for (int g : gl) put_range(f.genes[g]->adj_int_rng[d][0], f.genes[g]->adj_int_rng[d][1]);
+            rs.nadl.push_back((int)(n - off.back()));
+        }
+        off.push_back(n);
+        for (size_t d = 0; d < nd; d++)
+            for (int g : gl) {
+                const Gene &ge = *f.genes[g];
+                put_range(ge.adj_int_rng[d][0], ge.adj_int_rng[d][1]);
+                put_range(ge.adj_tip_rng[d][0], ge.adj_tip_rng[d][1]);
+            }
+        rs.nadm = (int)(n - off.back());
+        off.push_back(n);
+        for (int g : gl) {
+            const Gene &ge = *f.genes[g];
+            put_range(ge.ej_base, ge.ej_base + (int)ge.ejobs.size());
+        }
+        rs.nej = (int)(n - off.back());
+        off.push_back(n);
+        for (int g : gl)
+            for (int k = 0; k < 3; k++) h[n++] = 3 * g + k;
+        rs.nsets = 3 * (int)gl.size();
+    }
+    if (n > f.sel_cap) return fail(BSA_EINVAL, "round job lists exceed their buffer");
+    HIPCHK(hipMemcpyAsync(f.d_sel, h, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    size_t k = 0;
+    rs.pj = f.d_sel + off[k++];
+    for (size_t l = 0; l < nlev; l++) rs.lev.push_back(f.d_sel + off[k++]);
+    rs.genes = f.d_sel + off[k++];
+    if (adjoint) {
+        for (size_t d = 0; d < nd; d++) rs.adl.push_back(f.d_sel + off[k++]);
+        rs.adm = f.d_sel + off[k++];
+        rs.ej = f.d_sel + off[k++];
+        rs.sets = f.d_sel + off[k++];
+    }
+    // the listed genes' inputs
+    const std::vector<Gene *> &G = f.genes;
+    int rc = copy_runs(s, f.d_uvr, f.h_uvr, sizeof(double), gl,
+                       [](int g) { return (size_t)g * kGeneBlock; },
+                       [](int) { return (size_t)kGeneBlock; }, hipMemcpyHostToDevice);
+    if (!rc)
+        rc = copy_runs(s, f.d_pjobs, f.h_pjobs, sizeof(PmatJob), gl,
+                       [&](int g) { return (size_t)G[g]->pj_base; },
+                       [&](int g) { return G[g]->pjobs.size(); }, hipMemcpyHostToDevice);
+    if (!rc && adjoint)
+        rc = copy_runs(s, f.d_ejobs, f.h_ejobs, sizeof(EigJob), gl,
+                       [&](int g) { return (size_t)G[g]->ej_base; },
+                       [&](int g) { return G[g]->ejobs.size(); }, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    if ((rc = enqueue_device<T>(f, s, adjoint, true, &rs))) return rc;
+    // their outputs
+    if (adjoint) {
+        rc = copy_runs(s, f.h_adjout, f.d_dtau, sizeof(double), gl,
+                       [&](int g) { return (size_t)G[g]->ej_base; },
+                       [&](int g) { return G[g]->ejobs.size(); }, hipMemcpyDeviceToHost);
+        if (!rc)
+            rc = copy_runs(s, f.h_adjout + f.ejobs.size(), f.d_Z, sizeof(double), gl,
+                           [](int g) { return (size_t)g * 3 * 4096; },
+                           [](int) { return (size_t)3 * 4096; }, hipMemcpyDeviceToHost);
+        if (rc) return rc;
+    }
+    rc = copy_runs(s, f.h_out, f.d_out, sizeof(double), gl,
+                   [](int g) { return (size_t)g * kRootSums; }, [](int) { return (size_t)kRootSums; },
+                   hipMemcpyDeviceToHost);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    int nl = 1;
+    for (int v : rs.nadl) nl += v > 0;
+    return read_timing(f, adjoint, nl);
+}
+
+int forest_run_round(Forest &f, const std::vector<int> &genes, bool adjoint)
+{
+    for (size_t i = 0; i < genes.size(); i++)
+        if (genes[i] < 0 || genes[i] >= f.G || (i > 0 && genes[i] <= genes[i - 1]))
+            return fail(BSA_EINVAL, "round genes must be ascending gene indices");
+    if (genes.empty()) return BSA_OK;
+    if (adjoint && !f.adj_ready) return fail(BSA_EINVAL, "adjoint not set up");
+    HIPCHK(hipSetDevice(f.dev));
+    return f.precision == BSA_FP32 ? run_round<float>(f, genes, adjoint)
+                                   : run_round<double>(f, genes, adjoint);
 }
 
 int forest_run(Forest &f, bool adjoint)
@@ -949,7 +1136,7 @@ int gene_stage(Gene &g, Forest &f, const double *x, double kappa, double p0, dou
         p0 + p1 > 1 + 1e-12)
         return fail(BSA_EINVAL, "parameters out of range: kappa=%g p0=%g p1=%g w0=%g w2=%g", kappa,
                     p0, p1, w0, w2);
-    double *uvr = f.h_uvr + (size_t)3 * g.gi * kUVRSet;
+    double *uvr = f.h_uvr + (size_t)g.gi * kGeneBlock;
     const bool same = g.have_eigen && kappa == g.cache_kappa && w0 == g.cache_w0 &&
                       w2 == g.cache_w2;
     if (same) {
@@ -957,7 +1144,7 @@ int gene_stage(Gene &g, Forest &f, const double *x, double kappa, double p0, dou
         if (!setup_model_a(kappa, p0, p1, w0, w2, g.pi.data(), &g.model, false))
             return fail(BSA_ENUMERIC, "model set-up failed");
     } else {
-        if (!setup_model_a(kappa, p0, p1, w0, w2, g.pi.data(), &g.model, true)) {
+        if (!setup_model_a(kappa, p0, p1, w0, w2, g.pi.data(), &g.model, true, f.G == 1)) {
             g.have_eigen = false;
             return fail(BSA_ENUMERIC, "eigen decomposition failed");
         }
@@ -981,7 +1168,7 @@ int gene_stage(Gene &g, Forest &f, const double *x, double kappa, double p0, dou
         g.model.Qfactor[0] = qf_fixed[0];
         g.model.Qfactor[1] = qf_fixed[1];
     }
-    double *fk = f.h_fk() + (size_t)8 * g.gi;
+    double *fk = uvr + 3 * kUVRSet;
     for (int c = 0; c < 4; c++) fk[c] = g.model.freqK[c];
     PmatJob *pj = f.h_pjobs + g.pj_base;
     for (size_t i = 0; i < g.pjobs.size(); i++) {

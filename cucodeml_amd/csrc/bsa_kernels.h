@@ -7,6 +7,21 @@ namespace bsa {
 constexpr int kPadP = 128;      // patterns are padded to a multiple of this
 constexpr int kMaxCh = 8;       // children per node handled by one job
 
+// Model block of one gene in device memory: its three eigen sets (U and V
+// 64 x 64 each, then the 64 roots; sets for omega = w0, 1, w2), then 8 words
+// whose first four are the class proportions.  Gene g's block starts at
+// g * kGeneBlock; eigen set s of gene g is forest set 3 g + s.
+constexpr int kUVRSet = 64 * 64 * 2 + 64;
+constexpr int kGeneBlock = 3 * kUVRSet + 8;
+__host__ __device__ inline size_t uvr_offset(int set)
+{
+    return (size_t)(set / 3) * kGeneBlock + (size_t)(set % 3) * kUVRSet;
+}
+
+// Launchers take an optional list `sel` of the jobs (or genes, or eigen
+// sets) to run: block i runs job sel[i]; with sel = nullptr, block i runs
+// job i.
+
 // One (node, site class) job of the pruning pass: multiply the child
 // contributions together, rescale per pattern by a power of two, store.
 struct PruneJob {
@@ -77,46 +92,49 @@ constexpr int kWParts = 8;              // partial sums per eigen set in the W r
 
 // adjoints of one depth's interior children (jobs with v_tip < 0 only)
 template <typename T>
-void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad,
+void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
                           const unsigned char *z, const double *PT, const double *TT, int ncodes,
                           const int *E, const T *Cb, T *A, int *EA);
 
 // branch-matrix chunk partials of all jobs (after the sweep)
 template <typename T>
-void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk, int tpc,
-                      const unsigned char *z, const double *PT, const double *TT, int ncodes,
-                      const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
-                      const int *EA, double *Mpart);
+void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
+                      int nchunk, int tpc, const unsigned char *z, const double *PT,
+                      const double *TT, int ncodes, const double *mask64, const T *L, const int *E,
+                      const T *Cb, const T *A, const int *EA, double *Mpart);
 
 // for every EigJob sums the chunk partials of its branch matrix M, forms
 // A = U^T M V^T, d lnL/d tau and G = A o X(tau); then per eigen set
 // W = sum G (kWParts partials in Wpart[kWParts][nsets][64*64]) and
-// Z = V^T W U^T.  Outputs dtau[nej] and Z[nsets][64*64] (3 sets per gene).
+// Z = V^T W U^T.  Outputs dtau[job] and Z[set][64*64] (3 sets per gene) for
+// the selected jobs (esel) and sets (ssel).
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
-                           int nej, int nsets, const int *set_off, const int *set_list,
-                           const double *uvr, double *G, double *dtau, double *Wpart, double *Z);
+                           const int *esel, int nej, const int *ssel, int nsets,
+                           const int *set_off, const int *set_list, const double *uvr, double *G,
+                           double *dtau, double *Wpart, double *Z);
 
 // launchers; T = storage type of the conditional likelihoods (double or float)
-void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
-                 const unsigned char *mask_amb, int ncodes, double *TT);
+void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, const int *sel, int njobs,
+                 double *PT, const unsigned char *mask_amb, int ncodes, double *TT);
 
 // kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64)
 // Cb: contribution buffer (interior children's P I per class) or nullptr
 template <typename T>
-void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
-                  const unsigned char *z, const double *PT, const double *TT, int ncodes, T *L,
-                  int *E, T *Cb);
+void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, const int *sel,
+                  int njobs, int npad, const unsigned char *z, const double *PT, const double *TT,
+                  int ncodes, T *L, int *E, T *Cb);
 
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with
 // seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA).  args:
-// device array of ngene RootArgs; gene g reads pi[g*64], fk[g*8] (its four
-// class proportions, in device memory so that a captured graph reads the
-// current ones), fpatt[g*npad], writes fh[g*4*npad], site[g*npad],
-// partial[g*nblk*2*kRootSums] and out[g*kRootSums]
+// device array of RootArgs per gene; gene g (gsel[i] for block row i, or i)
+// reads pi[g*64], its class proportions in its model block (device memory,
+// so that a captured graph reads the current ones), fpatt[g*npad], writes
+// fh[g*4*npad], site[g*npad], partial[g*nblk*2*kRootSums] and
+// out[g*kRootSums]
 template <typename T>
-void launch_root(hipStream_t s, const RootArgs *args, int ngene, const T *L, const int *E,
-                 const double *pi, const double *fk, const double *fpatt, double *fh, double *site,
-                 double *partial, int nblk, double *out, T *A, int *EA);
+void launch_root(hipStream_t s, const RootArgs *args, const int *gsel, int ngene, const T *L,
+                 const int *E, const double *pi, const double *uvr, const double *fpatt, double *fh,
+                 double *site, double *partial, int nblk, double *out, T *A, int *EA);
 
 // raw MFMA lane-layout probe: one wave, writes each lane's accumulator
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,

@@ -32,7 +32,6 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kUVR = 64 * 64 * 2 + 64;   // per eigen set: U 64x64, V 64x64, root 64
 constexpr double kLn2 = 0.69314718055994530942;
 constexpr double kLogFloor = -690.77552789821368;   // log(1e-300)
 constexpr int kRootPerWave = 4;    // patterns per wave in k_root (16 per block)
@@ -44,12 +43,13 @@ constexpr int kRootPerWave = 4;    // patterns per wave in k_root (16 per block)
 // wave's child-state index is uniform, so V reads are broadcasts.
 __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
                                               const PmatJob *__restrict__ jobs,
+                                              const int *__restrict__ sel,
                                               double *__restrict__ PT)
 {
     __shared__ double Ut[64 * 64];   // Ut[m][j] = U[j][m]
     __shared__ double Ve[64 * 64];   // Ve[m][k] = exp(root_m t) V[m][k]
-    const PmatJob jb = jobs[blockIdx.x];
-    const double *U = uvr + (size_t)jb.set * kUVR;
+    const PmatJob jb = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
+    const double *U = uvr + uvr_offset(jb.set);
     const double *V = U + 4096;
     const double *R = U + 8192;
     double *out = PT + (size_t)jb.pm * 4096;
@@ -87,11 +87,12 @@ __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
 // code c allows of P_{j,k} (row c of P itself for a sense codon), so that a
 // tip's contribution to parent state j is one load, whatever the code.
 __global__ __launch_bounds__(256) void k_tiptab(const PmatJob *__restrict__ jobs,
+                                                const int *__restrict__ sel,
                                                 const double *__restrict__ PT,
                                                 const unsigned char *__restrict__ mask_amb,
                                                 int ncodes, double *__restrict__ TT)
 {
-    const PmatJob jb = jobs[blockIdx.x];
+    const PmatJob jb = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
     if (jb.tip < 0) return;
     const double *P = PT + (size_t)jb.pm * 4096;
     double *out = TT + (size_t)jb.pm * ncodes * 64;
@@ -206,7 +207,8 @@ __device__ inline d4 mfma64(double a, double b, d4 c)
 // from HBM.  Accumulator element i of column block b of lane l is pattern
 // (row group base) + l/16 + 4i, parent state 16b + l%16.
 template <typename T, int RG>
-__global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const PruneJob *__restrict__ jobs, int npad,
+__global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const PruneJob *__restrict__ jobs,
+                                                    const int *__restrict__ sel, int npad,
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
                                                     const double *__restrict__ TT, int ncodes,
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
                                                     T *__restrict__ Cb)
 {
     __shared__ double sP[64 * 64];
-    const PruneJob &jb = jobs[blockIdx.y];
+    const PruneJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
     const int h0 = blockIdx.x * (64 * RG);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
     d4 acc[RG][4];
@@ -324,7 +326,8 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
 // Same job on vector FMA, tile of 64 patterns: thread = parent state j (lane)
 // x 16 patterns (wave); identity part added after the product as above.
 template <typename T>
-__global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__ jobs, int npad,
+__global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__ jobs,
+                                                    const int *__restrict__ sel, int npad,
                                                     const unsigned char *__restrict__ z,
                                                     const double *__restrict__ PT,
                                                     const double *__restrict__ TT, int ncodes,
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(256) void k_prune_valu(const PruneJob *__restrict__
 {
     __shared__ double sP[64 * 64];
     __shared__ double sA[64 * 64];
-    const PruneJob &jb = jobs[blockIdx.y];
+    const PruneJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
     const int h0 = blockIdx.x * 64;
     const int tid = threadIdx.x, j = tid & 63, pg = tid >> 6;
     const int hb = h0 + 16 * pg;
@@ -421,20 +424,21 @@ __device__ inline void neumaier(double &s, double &c, double x)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_root(const RootArgs *__restrict__ args,
+                                              const int *__restrict__ gsel,
                                               const T *__restrict__ L,
                                               const int *__restrict__ E,
                                               const double *__restrict__ pi,
-                                              const double *__restrict__ fk,
+                                              const double *__restrict__ uvr,
                                               const double *__restrict__ fpatt,
                                               double *__restrict__ fh, double *__restrict__ site,
                                               double *__restrict__ partial, T *__restrict__ A,
                                               int *__restrict__ EA)
 {
     __shared__ double ss[4][kRootSums], sc[4][kRootSums];
-    const int gi = blockIdx.y;
+    const int gi = gsel ? gsel[blockIdx.y] : blockIdx.y;
     const RootArgs a = args[gi];
     pi += (size_t)gi * 64;
-    fk += (size_t)gi * 8;
+    const double *fk = uvr + (size_t)gi * kGeneBlock + 3 * kUVRSet;
     fpatt += (size_t)gi * a.npad;
     fh += (size_t)gi * 4 * a.npad;
     site += (size_t)gi * a.npad;
@@ -515,12 +519,14 @@ __global__ __launch_bounds__(256) void k_root(const RootArgs *__restrict__ args,
 // One wave per reduced quantity: lane l sums blocks l, l+64, ...
 // (compensated), then a fixed butterfly combines the 64 (sum, compensation)
 // pairs; the order never depends on timing, so results are bitwise repeatable.
-__global__ __launch_bounds__(64) void k_final(const double *__restrict__ partial, int nblk,
+__global__ __launch_bounds__(64) void k_final(const int *__restrict__ gsel,
+                                              const double *__restrict__ partial, int nblk,
                                               double *__restrict__ out)
 {
     const int lane = threadIdx.x, q = blockIdx.x;
-    partial += (size_t)blockIdx.y * nblk * 2 * kRootSums;
-    out += (size_t)blockIdx.y * kRootSums;
+    const int gi = gsel ? gsel[blockIdx.y] : blockIdx.y;
+    partial += (size_t)gi * nblk * 2 * kRootSums;
+    out += (size_t)gi * kRootSums;
     double S = 0, Cc = 0;
     for (int i = lane; i < nblk; i += 64) {
         neumaier(S, Cc, partial[(size_t)i * 2 * kRootSums + 2 * q]);
@@ -605,7 +611,8 @@ __device__ inline int x_sum16(const AdjJob &jb, int h, int j0, int npad, const u
 // 16g..16g+15) straight from HBM/L2, and P_v (identity included) is staged in
 // LDS once per block.
 template <typename T>
-__global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ jobs, int npad,
+__global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ jobs,
+                                                  const int *__restrict__ sel, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ PT,
                                                   const double *__restrict__ TT, int ncodes,
@@ -614,7 +621,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
                                                   int *__restrict__ EA, int tpb)
 {
     __shared__ double sP[64 * kLdsStride];
-    const AdjJob &jb = jobs[blockIdx.y];
+    const AdjJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
     {
         const double *P = PT + (size_t)jb.v_pmat * 4096;
@@ -674,7 +681,8 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
 // waves' column strips are disjoint, so each chunk writes its partial with no
 // further reduction (chunk partials are summed later in a fixed order).
 template <typename T>
-__global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs, int npad,
+__global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs,
+                                                  const int *__restrict__ sel, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ PT,
                                                   const double *__restrict__ TT, int ncodes,
@@ -687,7 +695,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
                                                   double *__restrict__ Mpart, int nchunk, int tpc)
 {
     __shared__ double sX[64 * kLdsStride];
-    const AdjJob &jb = jobs[blockIdx.y];
+    const AdjJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
     const int chunk = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
     const int ntile = npad / 64;
@@ -741,6 +749,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
 // (tau e^{lambda tau} on ties), so that for any rate-matrix parameter theta
 // d lnL / d theta = <G, V dQ/dtheta U> (derivative of U e^{Lambda tau} V).
 __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jobs,
+                                                   const int *__restrict__ sel,
                                                    const double *__restrict__ Mpart, int nchunk,
                                                    const double *__restrict__ uvr,
                                                    double *__restrict__ G, double *__restrict__ dtau)
@@ -749,8 +758,9 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     __shared__ double sU[64 * kLdsStride];
     __shared__ double sE[64];
     __shared__ double sred[4];
-    const EigJob jb = jobs[blockIdx.x];
-    const double *U = uvr + (size_t)jb.set * (64 * 64 * 2 + 64);
+    const int ji = sel ? sel[blockIdx.x] : blockIdx.x;
+    const EigJob jb = jobs[ji];
+    const double *U = uvr + uvr_offset(jb.set);
     const double *V = U + 4096;
     const double *R = U + 8192;
     const int tid = threadIdx.x;
@@ -791,7 +801,7 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     }
     const double lb = R[col], eb = sE[col];
     double dsum = 0;
-    double *Go = G + (size_t)blockIdx.x * 4096;
+    double *Go = G + (size_t)ji * 4096;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int a = r0 + 4 * q;
@@ -812,7 +822,7 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
     if ((tid & 63) == 0) sred[tid >> 6] = dsum;
     __syncthreads();
-    if (tid == 0) dtau[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
+    if (tid == 0) dtau[ji] = sred[0] + sred[1] + sred[2] + sred[3];
 }
 
 // Per eigen set: partial sums of G over its (branch, class) jobs; part p
@@ -820,29 +830,32 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
 __global__ __launch_bounds__(256) void k_wreduce(const double *__restrict__ G,
                                                  const int *__restrict__ set_off,
                                                  const int *__restrict__ set_list,
+                                                 const int *__restrict__ ssel,
                                                  double *__restrict__ Wpart)
 {
-    const int set = blockIdx.x, q = blockIdx.y * 256 + threadIdx.x, p = blockIdx.z;
+    const int set = ssel ? ssel[blockIdx.x] : blockIdx.x;
+    const int q = blockIdx.y * 256 + threadIdx.x, p = blockIdx.z;
     double s = 0;
     for (int i = set_off[set] + p; i < set_off[set + 1]; i += kWParts)
         s += G[(size_t)set_list[i] * 4096 + q];
-    Wpart[((size_t)p * gridDim.x + set) * 4096 + q] = s;
+    Wpart[((size_t)p * gridDim.x + blockIdx.x) * 4096 + q] = s;
 }
 
 // Per eigen set: W = sum of its parts (fixed order), Z = V^T W U^T (so that
 // <W, V dQ U> = <Z, dQ>).
 __global__ __launch_bounds__(256) void k_zform(const double *__restrict__ Wpart,
+                                               const int *__restrict__ ssel,
                                                const double *__restrict__ uvr,
                                                double *__restrict__ Z)
 {
     __shared__ double sW[64 * kLdsStride];
     __shared__ double sB[64 * kLdsStride];
-    const int set = blockIdx.x, tid = threadIdx.x;
-    const double *U = uvr + (size_t)set * (64 * 64 * 2 + 64);
+    const int set = ssel ? ssel[blockIdx.x] : blockIdx.x, pos = blockIdx.x, tid = threadIdx.x;
+    const double *U = uvr + uvr_offset(set);
     const double *V = U + 4096;
     for (int q = tid; q < 4096; q += 256) {
         double w = 0;
-        for (int p = 0; p < kWParts; p++) w += Wpart[((size_t)p * gridDim.x + set) * 4096 + q];
+        for (int p = 0; p < kWParts; p++) w += Wpart[((size_t)p * gridDim.x + pos) * 4096 + q];
         sW[(q >> 6) * kLdsStride + (q & 63)] = w;
         sB[(q >> 6) * kLdsStride + (q & 63)] = V[q];
     }
@@ -896,45 +909,46 @@ __global__ void k_mfma_probe(const double *A, const double *B, double *D, const 
 
 // ------------------------------------------------------------------ launchers
 
-void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, int njobs, double *PT,
-                 const unsigned char *mask_amb, int ncodes, double *TT)
+void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, const int *sel, int njobs,
+                 double *PT, const unsigned char *mask_amb, int ncodes, double *TT)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), 0, s, uvr, jobs, PT);
-    hipLaunchKernelGGL(k_tiptab, dim3(njobs), dim3(256), 0, s, jobs, (const double *)PT, mask_amb,
-                       ncodes, TT);
+    hipLaunchKernelGGL(k_pmat, dim3(njobs), dim3(256), 0, s, uvr, jobs, sel, PT);
+    hipLaunchKernelGGL(k_tiptab, dim3(njobs), dim3(256), 0, s, jobs, sel, (const double *)PT,
+                       mask_amb, ncodes, TT);
 }
 
 template <typename T>
-void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, int njobs, int npad,
-                  const unsigned char *z, const double *PT, const double *TT, int ncodes, T *L,
-                  int *E, T *Cb)
+void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, const int *sel,
+                  int njobs, int npad, const unsigned char *z, const double *PT, const double *TT,
+                  int ncodes, T *L, int *E, T *Cb)
 {
     if (njobs <= 0) return;
     if (kernel == 0 && tile == 128)
         hipLaunchKernelGGL((k_prune_mfma<T, 2>), dim3(npad / 128, njobs), dim3(256), 0, s, jobs,
-                           npad, z, PT, TT, ncodes, L, E, Cb);
+                           sel, npad, z, PT, TT, ncodes, L, E, Cb);
     else if (kernel == 0)
         hipLaunchKernelGGL((k_prune_mfma<T, 1>), dim3(npad / 64, njobs), dim3(256), 0, s, jobs,
-                           npad, z, PT, TT, ncodes, L, E, Cb);
+                           sel, npad, z, PT, TT, ncodes, L, E, Cb);
     else
-        hipLaunchKernelGGL(k_prune_valu<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, npad, z,
-                           PT, TT, ncodes, L, E, Cb);
+        hipLaunchKernelGGL(k_prune_valu<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, sel,
+                           npad, z, PT, TT, ncodes, L, E, Cb);
 }
 
 template <typename T>
-void launch_root(hipStream_t s, const RootArgs *args, int ngene, const T *L, const int *E,
-                 const double *pi, const double *fk, const double *fpatt, double *fh, double *site,
-                 double *partial, int nblk, double *out, T *A, int *EA)
+void launch_root(hipStream_t s, const RootArgs *args, const int *gsel, int ngene, const T *L,
+                 const int *E, const double *pi, const double *uvr, const double *fpatt, double *fh,
+                 double *site, double *partial, int nblk, double *out, T *A, int *EA)
 {
-    hipLaunchKernelGGL(k_root<T>, dim3(nblk, ngene), dim3(256), 0, s, args, L, E, pi, fk, fpatt, fh,
-                       site, partial, A, EA);
-    hipLaunchKernelGGL(k_final, dim3(kRootSums, ngene), dim3(64), 0, s, (const double *)partial,
-                       nblk, out);
+    if (ngene <= 0) return;
+    hipLaunchKernelGGL(k_root<T>, dim3(nblk, ngene), dim3(256), 0, s, args, gsel, L, E, pi, uvr,
+                       fpatt, fh, site, partial, A, EA);
+    hipLaunchKernelGGL(k_final, dim3(kRootSums, ngene), dim3(64), 0, s, gsel,
+                       (const double *)partial, nblk, out);
 }
 
 template <typename T>
-void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad,
+void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
                           const unsigned char *z, const double *PT, const double *TT, int ncodes,
                           const int *E, const T *Cb, T *A, int *EA)
 {
@@ -943,41 +957,44 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, int njobs, int npad
     const int ntile = npad / 64;
     const int tpb = std::max(1, std::min(8, njobs * ntile / 2048));
     hipLaunchKernelGGL(k_adj_a<T>, dim3((ntile + tpb - 1) / tpb, njobs), dim3(256), 0, s, jobs,
-                       npad, z, PT, TT, ncodes, E, Cb, A, EA, tpb);
+                       sel, npad, z, PT, TT, ncodes, E, Cb, A, EA, tpb);
 }
 
 template <typename T>
-void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, int njobs, int npad, int nchunk, int tpc,
-                      const unsigned char *z, const double *PT, const double *TT, int ncodes,
-                      const double *mask64, const T *L, const int *E, const T *Cb, const T *A,
-                      const int *EA, double *Mpart)
+void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
+                      int nchunk, int tpc, const unsigned char *z, const double *PT,
+                      const double *TT, int ncodes, const double *mask64, const T *L, const int *E,
+                      const T *Cb, const T *A, const int *EA, double *Mpart)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, npad, z, PT, TT,
-                       ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk, tpc);
+    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel, npad, z, PT,
+                       TT, ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk, tpc);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
-                           int nej, int nsets, const int *set_off, const int *set_list,
-                           const double *uvr, double *G, double *dtau, double *Wpart, double *Z)
+                           const int *esel, int nej, const int *ssel, int nsets,
+                           const int *set_off, const int *set_list, const double *uvr, double *G,
+                           double *dtau, double *Wpart, double *Z)
 {
-    hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, Mpart, nchunk, uvr, G, dtau);
+    if (nej <= 0 || nsets <= 0) return;
+    hipLaunchKernelGGL(k_eigbranch, dim3(nej), dim3(256), 0, s, ejobs, esel, Mpart, nchunk, uvr, G,
+                       dtau);
     hipLaunchKernelGGL(k_wreduce, dim3(nsets, 16, kWParts), dim3(256), 0, s, (const double *)G,
-                       set_off, set_list, Wpart);
-    hipLaunchKernelGGL(k_zform, dim3(nsets), dim3(256), 0, s, (const double *)Wpart, uvr, Z);
+                       set_off, set_list, ssel, Wpart);
+    hipLaunchKernelGGL(k_zform, dim3(nsets), dim3(256), 0, s, (const double *)Wpart, ssel, uvr, Z);
 }
 
-template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, int, int,
+template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, const int *, int, int,
                                            const unsigned char *, const double *, const double *,
                                            int, const int *, const double *, double *, int *);
-template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, int, int,
+template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, const int *, int, int,
                                           const unsigned char *, const double *, const double *,
                                           int, const int *, const float *, float *, int *);
-template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, int, int, int, int,
-                                       const unsigned char *, const double *, const double *, int,
-                                       const double *, const double *, const int *, const double *,
-                                       const double *, const int *, double *);
-template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, int, int, int, int,
+template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, const int *, int, int, int,
+                                       int, const unsigned char *, const double *, const double *,
+                                       int, const double *, const double *, const int *,
+                                       const double *, const double *, const int *, double *);
+template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, const int *, int, int, int, int,
                                       const unsigned char *, const double *, const double *, int,
                                       const double *, const float *, const int *, const float *,
                                       const float *, const int *, double *);
@@ -988,18 +1005,18 @@ void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, A, B, D64, Af, Bf, D32);
 }
 
-template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, int, int,
+template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, const int *, int, int,
                                    const unsigned char *, const double *, const double *, int,
                                    double *, int *, double *);
-template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, int, int,
+template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, const int *, int, int,
                                   const unsigned char *, const double *, const double *, int,
                                   float *, int *, float *);
-template void launch_root<double>(hipStream_t, const RootArgs *, int, const double *, const int *,
-                                  const double *, const double *, const double *, double *,
-                                  double *, double *, int, double *, double *, int *);
-template void launch_root<float>(hipStream_t, const RootArgs *, int, const float *, const int *,
-                                 const double *, const double *, const double *, double *,
-                                 double *, double *, int, double *, float *, int *);
+template void launch_root<double>(hipStream_t, const RootArgs *, const int *, int, const double *,
+                                  const int *, const double *, const double *, const double *,
+                                  double *, double *, double *, int, double *, double *, int *);
+template void launch_root<float>(hipStream_t, const RootArgs *, const int *, int, const float *,
+                                 const int *, const double *, const double *, const double *,
+                                 double *, double *, double *, int, double *, float *, int *);
 
 // ------------------------------------------------------------------ Bayes empirical Bayes
 

@@ -294,7 +294,7 @@ bool eigen_rev(const double *Q, const double *pi, double *root, double *U, doubl
 // (p0 w0 + p1)/(p0+p1), of foreground branches at p0 w0 + p1 + (1-p0-p1) w2;
 // eigen systems for omega = w0, 1, w2.
 bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, const double *pi,
-                   ModelA *m, bool need_eigen)
+                   ModelA *m, bool need_eigen, bool threads)
 {
     double t = p0 + p1;
     if (!(t > 1e-100)) return false;
@@ -318,10 +318,14 @@ bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, con
         build_q(kappa, w[s], pi, Q);
         ok[s] = eigen_rev(Q, pi, m->root[s], m->U[s], m->V[s]);
     };
-    std::thread t1(one, 1), t2(one, 2);
-    one(0);
-    t1.join();
-    t2.join();
+    if (threads) {
+        std::thread t1(one, 1), t2(one, 2);
+        one(0);
+        t1.join();
+        t2.join();
+    } else {
+        for (int s = 0; s < 3; s++) one(s);
+    }
     return ok[0] && ok[1] && ok[2];
 }
 

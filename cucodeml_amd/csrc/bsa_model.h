@@ -28,8 +28,10 @@ struct ModelA {
 double build_q(double kappa, double omega, const double *pi, double *Q);
 double mean_rate(double kappa, double omega, const double *pi);
 bool eigen_rev(const double *Q, const double *pi, double *root, double *U, double *V);
+// threads: the three eigen systems on three threads (off where the caller
+// already runs one gene per thread)
 bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, const double *pi,
-                   ModelA *m, bool need_eigen);
+                   ModelA *m, bool need_eigen, bool threads = true);
 
 // Derivatives of lnL with respect to the natural model parameters, given
 // the adjoint outputs: Z[3][64*64] (per eigen set, Z = V^T W U^T, so that

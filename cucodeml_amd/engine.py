@@ -26,7 +26,8 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_model_a_setup", "bsa_default_bounds", "bsa_fit", "bsa_debug_minimize",
             "bsa_beb", "bsa_set_graph", "bsa_batch_create", "bsa_batch_destroy",
             "bsa_batch_size", "bsa_batch_set_model", "bsa_batch_num_params",
-            "bsa_batch_default_bounds", "bsa_batch_lnl", "bsa_batch_lnl_grad", "bsa_batch_fit",
+            "bsa_batch_default_bounds", "bsa_batch_lnl", "bsa_batch_lnl_grad",
+            "bsa_batch_eval_genes", "bsa_batch_fit",
             "bsa_batch_class_lnl", "bsa_batch_last_params", "bsa_batch_timing", "bsa_batch_work",
             "bsa_batch_set_graph", "bsa_debug_forest_plan"]
 
@@ -102,6 +103,7 @@ def lib():
         L.bsa_batch_lnl.argtypes = [C.c_void_p, dpp, dp]
         L.bsa_batch_lnl_grad.argtypes = [C.c_void_p, dpp, dp, dpp]
         L.bsa_batch_fit.argtypes = [C.c_void_p, dpp, dpp, dpp, C.POINTER(_FitOpts), dp, ip]
+        L.bsa_batch_eval_genes.argtypes = [C.c_void_p, ip, C.c_int, dpp, dp, C.c_void_p]
         L.bsa_batch_class_lnl.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.bsa_batch_last_params.argtypes = [C.c_void_p, C.c_int, dp]
         L.bsa_batch_timing.argtypes = [C.c_void_p, dp, C.c_int]
@@ -373,14 +375,27 @@ class BranchSiteBatch:
         _check(lib().bsa_batch_lnl_grad(self._h, _ptrs(xs), _dptr(out), _ptrs(gs)))
         return out, gs
 
+    def eval_genes(self, genes, xs, grad=True):
+        """-lnL (and gradients) of the listed genes only (ascending indices)
+        at xs[k] for genes[k]; the other genes keep their last results."""
+        genes = np.ascontiguousarray(genes, dtype=np.int32)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in xs]
+        out = np.zeros(len(genes))
+        gs = [np.zeros(len(x)) for x in xs] if grad else None
+        _check(lib().bsa_batch_eval_genes(self._h, genes.ctypes.data_as(C.POINTER(C.c_int)),
+                                          len(genes), _ptrs(xs), _dptr(out),
+                                          C.cast(_ptrs(gs), C.c_void_p) if grad else None))
+        return (out, gs) if grad else out
+
     def default_bounds(self, gene):
         lo, hi = np.zeros(self.np[gene]), np.zeros(self.np[gene])
         _check(lib().bsa_batch_default_bounds(self._h, gene, _dptr(lo), _dptr(hi)))
         return lo, hi
 
     def fit(self, x0s, e=1e-8, max_iter=2000, max_step=5.0):
-        """Lockstep ML fits of every gene from x0s[gene] (bounded BFGS,
-        adjoint gradient, codeml's bounds).  Returns (list of x, -lnL per
+        """ML fits of every gene from x0s[gene] (bounded BFGS, adjoint
+        gradient, codeml's bounds), one optimiser thread per gene, served in
+        rounds of the genes whose points are ready.  Returns (list of x, -lnL per
         gene, list of info dicts)."""
         xs = [np.array(x, dtype=np.float64) for x in self._xs(x0s)]
         o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)

@@ -5,7 +5,8 @@ per-class values and gradient of that gene's own single-gene engine (FP64:
 the same kernels on the same data, so lnL bitwise and gradients to the
 rounding of the branch-matrix chunk sums); the likelihood of one gene
 against the oracle; FP32 within its error budget; graph replay equal to
-eager launches; lockstep fits equal to one-gene fits; the batched genome
+eager launches; evaluations of gene subsets equal to whole-batch ones;
+batch fits (continuous batching) equal to one-gene fits; the batched genome
 scan equal to the per-gene scan."""
 import numpy as np
 import pytest
@@ -131,8 +132,31 @@ def test_batch_graph_equals_eager(genes):
             assert np.array_equal(a, c)
 
 
-def test_lockstep_fits_equal_single_fits(genes):
-    say("case: lockstep batch fits vs single-engine fits")
+def test_gene_subsets_equal_full_evaluation(genes):
+    say("case: evaluations of gene subsets vs the whole batch")
+    xs = [x for _, _, x in genes]
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in genes])
+    try:
+        f, gs = b.lnl_grad(xs)
+        fh6, _ = b.fhK(6)
+        for sub in ([1], [0, 2, 3], [4, 5]):
+            xs2 = [xs[i] * 1.01 for i in sub]
+            fs, gss = b.eval_genes(sub, xs2)
+            fw, gw = b.lnl_grad([xs[i] * 1.01 if i in sub else xs[i] for i in range(len(xs))])
+            for k, i in enumerate(sub):
+                assert fs[k] == fw[i]
+                assert np.abs(gss[k] - gw[i]).max() <= 1e-12 * np.abs(gw[i]).max()
+        b.lnl_grad(xs)
+        fs = b.eval_genes([2, 5], [xs[2] * 1.02, xs[5] * 1.02], grad=False)
+        fh6b, _ = b.fhK(6)
+        assert np.array_equal(fh6, fh6b)   # gene 6 not in the subset: kept
+        assert fs[0] != f[2]
+    finally:
+        b.close()
+
+
+def test_batch_fits_equal_single_fits(genes):
+    say("case: batch fits (rounds of ready genes) vs single-engine fits")
     sub = genes[:5]
     starts = []
     for tree, _, _ in sub:
