@@ -17,6 +17,8 @@ FLAGS = [
     # hipcc -O3 forms packed FP32 ops by itself; they are kept off on purpose
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument", "-pthread",
+    # host code (the model set-up's eigen systems) with AVX2 and FMA
+    "-Xarch_host", "-mavx2", "-Xarch_host", "-mfma",
 ]
 
 

@@ -1,9 +1,13 @@
 // HIP kernels for gfx950 (MI355X) of the branch-site model A likelihood:
 //   k_pmat        batched P(t) - I = U diag(exp(root t)) V - I, every (branch, eigen set)
-//   k_amb         per-tip rows for ambiguous codons (sums of P columns)
+//   k_tiptab      per-tip rows of every code (sense codons and ambiguity codes)
 //   k_prune_mfma  one tree level of Felsenstein pruning on the FP64 matrix cores
 //   k_prune_valu  the same on plain vector FMA
 //   k_root        per-pattern class log-likelihoods, mixture and weighted sum
+//   k_adj_a       adjoints of one depth's interior children (preorder sweep)
+//   k_adj_m       branch matrices of every (branch, class) job
+//   k_eigbranch, k_wreduce, k_zform   eigen-basis derivatives and their sums
+//   k_beb_*       Bayes empirical Bayes over the 10^4-point grid
 //
 // Arithmetic is FP64 throughout; T (double or float) is only the storage type
 // of the conditional likelihoods.  Layouts:
