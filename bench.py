@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--gradient", choices=["adjoint", "fd"], default="adjoint")
     ap.add_argument("--kernel", choices=["mfma", "valu"], default="mfma")
     ap.add_argument("--tile", type=int, default=0, help="patterns per pruning block (64/128)")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--fd-steps", type=int, default=1,
                     help="steps of the FP64 forward-difference path timed beside (0: skip)")
     ap.add_argument("--cpu-sample-codons", type=int, default=1000)
@@ -215,6 +217,9 @@ def main():
     xs = [x * (1.0 + 1e-7 * (k + 1)) for k in range(args.warmup + args.steps)]
     for k in range(args.warmup):
         step(xs[k])
+    # launch shapes picked by the engine's autotuner (untimed, like warm-up)
+    shapes = eng.tune(xs[max(0, args.warmup - 1)]) \
+        if args.tune and args.gradient == "adjoint" else None
     eng.timing(reset=True)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -238,7 +243,8 @@ def main():
         rls += [(tm["sum_adjoint"], adj_rl), (tm["sum_branch"], br_rl)]
     rl = max(rls, key=lambda r: r[0])[1]
     default_workload = (args.taxa, args.codons, args.seed, args.precision, args.gradient,
-                        args.kernel, args.tile) == (128, 10000, 2, "fp32", "adjoint", "mfma", 0)
+                        args.kernel, args.tile, args.tune) == (128, 10000, 2, "fp32", "adjoint",
+                                                               "mfma", 0, True)
     tr = measured_traffic(rl["kernel"]) if default_workload else None
     if tr:
         rl["traffic"], rl["traffic_source"] = tr
@@ -260,6 +266,7 @@ def main():
                                f"({data.npatt} patterns), {args.precision} partials, "
                                f"{args.gradient} gradient of {npar} parameters",
                    "taxa": args.taxa, "codons": args.codons, "patterns": data.npatt,
+                   "launch_shapes": shapes,
                    "np": npar, "gradient": args.gradient, "kernel": args.kernel,
                    "tile": args.tile or 64},
         "roofline": rl,

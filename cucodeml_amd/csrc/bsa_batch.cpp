@@ -369,6 +369,21 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
     return BSA_OK;
 }
 
+extern "C" int bsa_batch_tune(bsa_batch *b, const double *const *x, int reps, double *out6)
+{
+    if (!b || !x) return fail(BSA_EINVAL, "null argument");
+    const int G = (int)b->genes.size();
+    std::vector<std::vector<double>> gs(G);
+    std::vector<double *> gp(G);
+    std::vector<double> negl(G);
+    for (int i = 0; i < G; i++) {
+        gs[i].resize(gene_num_params(*b->genes[i]));
+        gp[i] = gs[i].data();
+    }
+    return forest_tune(b->f, [&]() { return batch_eval(b, x, negl.data(), gp.data()); }, reps,
+                       out6);
+}
+
 extern "C" int bsa_batch_class_lnl(bsa_batch *b, int gene, double *fhK, double *site)
 {
     if (int rc = check_gene(b, gene)) return rc;

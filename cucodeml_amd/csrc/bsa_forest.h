@@ -13,6 +13,7 @@
 
 #include <array>
 #include <cmath>
+#include <functional>
 #include <shared_mutex>
 #include <vector>
 
@@ -141,6 +142,10 @@ struct Forest {
     std::vector<EigJob> ejobs;
     std::vector<int> set_off, set_list;
     int nchunk = 0, tpc = 0, nmrow = 0;
+    // launch shapes the tuner may set: child-adjoint tiles per block (0: by
+    // launch size), branch-matrix tiles per chunk (0: by gene size); chunks
+    // the branch-matrix partials have room for
+    int tpb_set = 0, tpc_set = 0, mpart_chunks = 0;
     AdjJob *d_adj = nullptr;
     EigJob *d_ejobs = nullptr, *h_ejobs = nullptr;
     int *d_set_off = nullptr, *d_set_list = nullptr;
@@ -186,5 +191,16 @@ int forest_run_round(Forest &f, const std::vector<int> &genes, bool adjoint);
 double gene_collect(Gene &g, const Forest &f);
 // d(-lnL)/dx of gene g from the adjoint outputs of the last forest_run
 void gene_chain_rule(const Gene &g, const Forest &f, const double *x, double *grad);
+
+// Launch shapes: pruning tile (64 or 128 patterns), child-adjoint tiles per
+// block (0 = by launch size, else 1..8), branch-matrix tiles per chunk (0 = by
+// gene size, else 1..kAdjTilesPerChunk).  Captured graphs are dropped.
+int forest_set_shapes(Forest &f, int tile, int tpb, int tpc);
+// Autotuner: for each shape parameter in turn (pruning tile, child-adjoint
+// tiles per block, branch-matrix tiles per chunk) every candidate is timed
+// over `reps` calls of eval (one evaluation with gradient each) by the
+// device time of the evaluation, and the fastest is kept.  out[6] = {tile,
+// tpb, tpc, chunks, best ms, ms of the shapes it started from}.
+int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *out6);
 
 }  // namespace bsa

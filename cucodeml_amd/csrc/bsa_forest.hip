@@ -654,7 +654,8 @@ void forest_plan_adjoint(Forest &f)
     const int ntile = f.npad / 64;
     // tiles per branch-matrix block: up to kAdjTilesPerChunk, fewer when the
     // genes are small, so that the launch still has about 1024 blocks
-    f.tpc = std::max(1, std::min(kAdjTilesPerChunk, ntile * (int)f.adj.size() / 1024));
+    f.tpc = f.tpc_set > 0 ? f.tpc_set
+                          : std::max(1, std::min(kAdjTilesPerChunk, ntile * (int)f.adj.size() / 1024));
     f.nchunk = (ntile + f.tpc - 1) / f.tpc;
     for (int k = 6; k < 10; k++) {
         f.work[k] = 0;
@@ -687,6 +688,7 @@ int forest_setup_adjoint(Forest &f)
     HIPCHK(hipMalloc(&f.d_EA, sizeof(int) * (size_t)f.npad * nasl));
     HIPCHK(hipMalloc(&f.d_Mpart, sizeof(double) * 4096 * (size_t)f.nmrow * f.nchunk));
     HIPCHK(hipMemset(f.d_Mpart, 0, sizeof(double) * 4096 * (size_t)f.nmrow * f.nchunk));
+    f.mpart_chunks = f.nchunk;
     HIPCHK(hipMalloc(&f.d_W, sizeof(double) * 4096 * 3 * G * kWParts));
     HIPCHK(hipMalloc(&f.d_G, sizeof(double) * 4096 * nej));
     HIPCHK(hipMalloc(&f.d_dtau, sizeof(double) * nej));
@@ -848,7 +850,8 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
             launch_adjoint_level<T>(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
                                     rs ? rs->adl[d] : nullptr, rs ? rs->nadl[d] : f.adj_nint[d],
                                     f.npad, f.d_z, (const double *)f.d_PT, (const double *)f.d_AT,
-                                    f.ncodes, f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA);
+                                    f.ncodes, f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA,
+                                    f.tpb_set);
         REC(6);
         launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
                             f.npad, f.nchunk, f.tpc, f.d_z, (const double *)f.d_PT,
@@ -1229,6 +1232,99 @@ void gene_chain_rule(const Gene &g, const Forest &f, const double *x, double *gr
     }
     grad[nb + 3] = -mg.w0;
     if (!g.opts.fix_omega) grad[nb + 4] = -mg.w2;
+}
+
+}  // namespace bsa
+
+namespace bsa {
+
+int forest_set_shapes(Forest &f, int tile, int tpb, int tpc)
+{
+    HIP_SHARED;
+    if (tile != 64 && tile != 128) return fail(BSA_EINVAL, "tile %d: 64 or 128 patterns per block", tile);
+    if (tile == 128 && f.kernel != BSA_KERNEL_MFMA)
+        return fail(BSA_EINVAL, "128-pattern tiles need the MFMA kernel");
+    if (tpb < 0 || tpb > 8) return fail(BSA_EINVAL, "tiles per child-adjoint block %d: 0..8", tpb);
+    if (tpc < 0 || tpc > kAdjTilesPerChunk)
+        return fail(BSA_EINVAL, "tiles per branch-matrix chunk %d: 0..%d", tpc, kAdjTilesPerChunk);
+    for (auto &gr : f.graph)
+        if (gr) {
+            HIPCHK(hipGraphExecDestroy(gr));
+            gr = nullptr;
+        }
+    f.tile = tile;
+    f.tpb_set = tpb;
+    f.tpc_set = tpc;
+    if (!f.adj_ready) return BSA_OK;   // the chunking is set when the adjoint is
+    const int ntile = f.npad / 64;
+    f.tpc = tpc > 0 ? tpc
+                    : std::max(1, std::min(kAdjTilesPerChunk, ntile * (int)f.adj.size() / 1024));
+    f.nchunk = (ntile + f.tpc - 1) / f.tpc;
+    if (f.nchunk > f.mpart_chunks) {
+        HIPCHK(hipSetDevice(f.dev));
+        HIPCHK(hipStreamSynchronize(f.stream));
+        HIPCHK(hipFree(f.d_Mpart));
+        f.d_Mpart = nullptr;
+        f.mpart_chunks = 0;
+        HIPCHK(hipMalloc(&f.d_Mpart, sizeof(double) * 4096 * (size_t)f.nmrow * f.nchunk));
+        f.mpart_chunks = f.nchunk;
+    }
+    return forest_check(f);
+}
+
+int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *out)
+{
+    if (reps < 1) reps = 3;
+    auto device_ms = [&]() {
+        double t = 0;
+        for (int i = 0; i < 6; i++) t += f.timing[i];
+        return t;
+    };
+    auto measure = [&](double *ms) {
+        double best = 1e300;
+        for (int r = 0; r < reps; r++) {
+            const int rc = eval();
+            if (rc) return rc;
+            best = std::min(best, device_ms());
+        }
+        *ms = best;
+        return BSA_OK;
+    };
+    int cur[3] = {f.tile, f.tpb_set, f.tpc_set};
+    int rc = eval();   // first call: adjoint set-up, graph capture
+    if (rc) return rc;
+    double best;
+    if ((rc = measure(&best))) return rc;
+    const double start = best;
+    const std::vector<int> cand[3] = {
+        f.kernel == BSA_KERNEL_MFMA ? std::vector<int>{64, 128} : std::vector<int>{64},
+        {0, 1, 2, 4, 8},
+        {0, 4, 8, 16, 32}};
+    for (int k = 0; k < 3; k++) {
+        int keep = cur[k];
+        for (int v : cand[k]) {
+            if (v == keep) continue;
+            int shp[3] = {cur[0], cur[1], cur[2]};
+            shp[k] = v;
+            if ((rc = forest_set_shapes(f, shp[0], shp[1], shp[2]))) return rc;
+            double ms;
+            if ((rc = measure(&ms))) return rc;
+            if (ms < best) {
+                best = ms;
+                cur[k] = v;
+            }
+        }
+        if ((rc = forest_set_shapes(f, cur[0], cur[1], cur[2]))) return rc;
+    }
+    if (out) {
+        out[0] = f.tile;
+        out[1] = f.tpb_set;
+        out[2] = f.tpc;
+        out[3] = f.nchunk;
+        out[4] = best;
+        out[5] = start;
+    }
+    return BSA_OK;
 }
 
 }  // namespace bsa

@@ -90,11 +90,12 @@ struct EigJob {
 constexpr int kAdjTilesPerChunk = 32;   // most 64-pattern tiles per branch-matrix block
 constexpr int kWParts = 8;              // partial sums per eigen set in the W reduction
 
-// adjoints of one depth's interior children (jobs with v_tip < 0 only)
+// adjoints of one depth's interior children (jobs with v_tip < 0 only);
+// tpb: 64-pattern tiles per block, 0 for a choice by the launch's size
 template <typename T>
 void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
                           const unsigned char *z, const double *PT, const double *TT, int ncodes,
-                          const int *E, const T *Cb, T *A, int *EA);
+                          const int *E, const T *Cb, T *A, int *EA, int tpb);
 
 // branch-matrix chunk partials of all jobs (after the sweep)
 template <typename T>

@@ -954,12 +954,12 @@ void launch_root(hipStream_t s, const RootArgs *args, const int *gsel, int ngene
 template <typename T>
 void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
                           const unsigned char *z, const double *PT, const double *TT, int ncodes,
-                          const int *E, const T *Cb, T *A, int *EA)
+                          const int *E, const T *Cb, T *A, int *EA, int tpb)
 {
     if (njobs <= 0) return;
     // several tiles per block (P_v staged once) when the level has blocks to spare
     const int ntile = npad / 64;
-    const int tpb = std::max(1, std::min(8, njobs * ntile / 2048));
+    if (tpb <= 0) tpb = std::max(1, std::min(8, njobs * ntile / 2048));
     hipLaunchKernelGGL(k_adj_a<T>, dim3((ntile + tpb - 1) / tpb, njobs), dim3(256), 0, s, jobs,
                        sel, npad, z, PT, TT, ncodes, E, Cb, A, EA, tpb);
 }
@@ -990,10 +990,10 @@ void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const
 
 template void launch_adjoint_level<double>(hipStream_t, const AdjJob *, const int *, int, int,
                                            const unsigned char *, const double *, const double *,
-                                           int, const int *, const double *, double *, int *);
+                                           int, const int *, const double *, double *, int *, int);
 template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, const int *, int, int,
                                           const unsigned char *, const double *, const double *,
-                                          int, const int *, const float *, float *, int *);
+                                          int, const int *, const float *, float *, int *, int);
 template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, const int *, int, int, int,
                                        int, const unsigned char *, const double *, const double *,
                                        int, const double *, const double *, const int *,

@@ -29,7 +29,8 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_batch_default_bounds", "bsa_batch_lnl", "bsa_batch_lnl_grad",
             "bsa_batch_eval_genes", "bsa_batch_fit",
             "bsa_batch_class_lnl", "bsa_batch_last_params", "bsa_batch_timing", "bsa_batch_work",
-            "bsa_batch_set_graph", "bsa_debug_forest_plan"]
+            "bsa_batch_set_graph", "bsa_debug_forest_plan", "bsa_set_shapes", "bsa_tune",
+            "bsa_batch_tune"]
 
 
 class BsaError(RuntimeError):
@@ -88,6 +89,8 @@ def lib():
         L.bsa_default_bounds.argtypes = [C.c_void_p, dp, dp]
         L.bsa_beb.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp, dp, C.c_void_p]
         L.bsa_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.bsa_set_shapes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.bsa_tune.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, dp]
         L.bsa_fit.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, C.POINTER(_FitOpts), dp,
                               C.POINTER(C.c_int)]
         L.bsa_debug_minimize.argtypes = [C.c_int, _OBJECTIVE, C.c_void_p, dp, dp, dp,
@@ -109,6 +112,7 @@ def lib():
         L.bsa_batch_timing.argtypes = [C.c_void_p, dp, C.c_int]
         L.bsa_batch_work.argtypes = [C.c_void_p, dp]
         L.bsa_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.bsa_batch_tune.argtypes = [C.c_void_p, dpp, C.c_int, dp]
         L.bsa_debug_forest_plan.argtypes = [pp, C.c_int, dp]
         _lib = L
     return _lib
@@ -178,6 +182,18 @@ class BranchSiteEngine:
     def use_graph(self, enable=True):
         """Replay each evaluation's device work as a captured HIP graph."""
         _check(lib().bsa_set_graph(self._h, int(bool(enable))))
+
+    def set_shapes(self, tile=64, tpb=0, tpc=0):
+        """Launch shapes (bsa_set_shapes)."""
+        _check(lib().bsa_set_shapes(self._h, tile, tpb, tpc))
+
+    def tune(self, x, reps=3):
+        """Autotune the launch shapes at x (bsa_tune); returns the choice."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        o = np.zeros(6)
+        _check(lib().bsa_tune(self._h, _dptr(x), len(x), reps, _dptr(o)))
+        return {"tile": int(o[0]), "tpb": int(o[1]), "tpc": int(o[2]), "chunks": int(o[3]),
+                "ms": float(o[4]), "ms_before": float(o[5])}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -374,6 +390,14 @@ class BranchSiteBatch:
         gs = [np.zeros(len(x)) for x in xs]
         _check(lib().bsa_batch_lnl_grad(self._h, _ptrs(xs), _dptr(out), _ptrs(gs)))
         return out, gs
+
+    def tune(self, xs, reps=3):
+        """Autotune the batch's launch shapes at xs (bsa_batch_tune)."""
+        xs = self._xs(xs)
+        o = np.zeros(6)
+        _check(lib().bsa_batch_tune(self._h, _ptrs(xs), reps, _dptr(o)))
+        return {"tile": int(o[0]), "tpb": int(o[1]), "tpc": int(o[2]), "chunks": int(o[3]),
+                "ms": float(o[4]), "ms_before": float(o[5])}
 
     def eval_genes(self, genes, xs, grad=True):
         """-lnL (and gradients) of the listed genes only (ascending indices)

@@ -175,6 +175,22 @@ int  bsa_model_a_setup(double kappa, double p0, double p1, double w0, double w2,
  * the branch matrices */
 int  bsa_work(bsa_engine *eng, double *out10);
 
+/* launch shapes: pruning tile (64 or 128 patterns per block), child-adjoint
+ * tiles per block (0: chosen by each launch's size, else 1..8), branch-matrix
+ * tiles per chunk (0: chosen by the gene's size, else 1..32).  Results do not
+ * depend on the first two; the chunking sets the order in which a branch
+ * matrix is summed (gradients move in the 16th digit).  No counterpart in
+ * the reference. */
+int  bsa_set_shapes(bsa_engine *eng, int tile, int tpb, int tpc);
+
+/* autotuner: times each candidate of each launch shape in turn (pruning
+ * tile, child-adjoint tiles per block, branch-matrix tiles per chunk) by
+ * the device time of `reps` lnL + gradient evaluations at x, and keeps the
+ * fastest.  out6 (may be NULL) = {tile, tpb, tpc, chunks, best ms, ms of the
+ * shapes it started from}.  The engine's last evaluation is at x.  No
+ * counterpart in the reference. */
+int  bsa_tune(bsa_engine *eng, const double *x, int np, int reps, double *out6);
+
 /* codeml's parameter bounds for model A in the engine's x layout
  * (SetxBound codeml.c:1786-1828, SetxBoundTimes treesub.c:3872-3884):
  * branch lengths [4e-6, 50], kappa [1e-4, 999], transformed p's [-99, 99]
@@ -250,6 +266,8 @@ int  bsa_batch_eval_genes(bsa_batch *batch, const int *genes, int n, const doubl
 int  bsa_batch_fit(bsa_batch *batch, double *const *x, const double *const *lo,
                    const double *const *hi, const bsa_fit_opts *opts, double *neg_lnl,
                    int *info);
+/* bsa_tune for a batch, at x[gene] for every gene */
+int  bsa_batch_tune(bsa_batch *batch, const double *const *x, int reps, double *out6);
 /* per-gene outputs of the last evaluation, as bsa_class_lnl / bsa_last_params */
 int  bsa_batch_class_lnl(bsa_batch *batch, int gene, double *fhK, double *site_lnf);
 int  bsa_batch_last_params(const bsa_batch *batch, int gene, double *out11);
