@@ -213,7 +213,9 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
                     s2 = s;
                     g = c * e[i];
                     h = c * p;
-                    r = std::hypot(p, e[i]);
+                    // p and e[i] are of the order of the matrix norm here, so
+                    // the plain square root cannot overflow (hypot is ~3x slower)
+                    r = std::sqrt(p * p + e[i] * e[i]);
                     e[i + 1] = s * r;
                     s = e[i] / r;
                     c = p / r;

@@ -99,10 +99,20 @@ def test_adjoint_full_size_directional():
     e64 = engine.BranchSiteEngine(tree, data)
     f, g = e64.lnl_grad(x)
     d = rng.standard_normal(len(x)) * (np.abs(x) + 0.1)
-    eps = 1e-6
-    dd = (e64.lnl(x + eps * d) - e64.lnl(x - eps * d)) / (2 * eps)
-    say(f"  g.d {g @ d:.8e}  fd {dd:.8e}  timing {e64.timing()}")
-    assert abs(g @ d - dd) < 1e-5 * abs(dd) + 1e-3
+    # Rounding of lnL (~5e5, ~1e-15 relative per evaluation) gives a
+    # difference-quotient noise of ~|f| 1e-14 / eps, so the step cannot be
+    # small; the h^2 truncation term of central differences (~3e-2 at
+    # h = 1e-4 along this d) is removed by Richardson extrapolation over
+    # h and 2h, which leaves ~1e-4 of noise.
+    eps = 1e-4
+
+    def cd(h):
+        return (e64.lnl(x + h * d) - e64.lnl(x - h * d)) / (2 * h)
+
+    d1, d2 = cd(eps), cd(2 * eps)
+    dd = (4 * d1 - d2) / 3
+    say(f"  g.d {g @ d:.8e}  fd(h) {d1:.8e}  fd(2h) {d2:.8e}  Richardson {dd:.8e}")
+    assert abs(g @ d - dd) < 1e-5 * abs(dd) + 3e-14 * abs(f) / eps
     e32 = engine.BranchSiteEngine(tree, data, precision="fp32")
     f32, g32 = e32.lnl_grad(x)
     rel = np.abs(g32 - g) / np.maximum(1.0, np.abs(g))
