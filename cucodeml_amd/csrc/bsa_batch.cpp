@@ -4,13 +4,14 @@
 // level of the deepest tree rather than one per level of every gene; the
 // host model set-up (three eigen systems per gene) and the chain rule run on
 // a pool of host threads.  bsa_batch_fit runs one optimiser thread per gene
-// and serves their requests in rounds (continuous batching): each round
-// evaluates the genes whose points are staged, on the device, while the
-// other genes' threads prepare theirs.
+// being fitted (at most 256 at once) and serves their requests in rounds
+// (continuous batching): each round evaluates the genes whose points are
+// staged, on the device, while the other genes' threads prepare theirs.
 #include "bsa_fit.h"
 #include "bsa_forest.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -28,6 +29,8 @@ struct bsa_batch {
 };
 
 namespace {
+
+constexpr int kMaxFitThreads = 256;   // optimiser threads of one batch fit
 
 int host_threads()
 {
@@ -339,9 +342,16 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
         srv.finished();
     };
     {
+        // at most kMaxFitThreads fits in flight; a worker takes the next gene
+        // when its fit ends
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            for (int i = next++; i < G; i = next++) fit_one(i);
+        };
+        const int nw = std::min(G, kMaxFitThreads);
         std::vector<std::thread> th;
-        th.reserve(G);
-        for (int i = 0; i < G; i++) th.emplace_back(fit_one, i);
+        th.reserve(nw);
+        for (int t = 0; t < nw; t++) th.emplace_back(worker);
         srv.serve();   // HIP work on this thread only
         for (auto &t : th) t.join();
     }

@@ -258,9 +258,11 @@ int  bsa_batch_lnl_grad(bsa_batch *batch, const double *const *x, double *neg_ln
 int  bsa_batch_eval_genes(bsa_batch *batch, const int *genes, int n, const double *const *x,
                           double *neg_lnl, double *const *grad);
 /* ML fits of every gene from x[gene] (bsa_fit's optimiser, adjoint
- * gradient only), one optimiser thread per gene; their requests are served
- * in rounds, each evaluating the genes whose points are ready (continuous
- * batching), so a gene costs device time only for its own points.  lo/hi (either, or any entry, may be NULL:
+ * gradient only), one optimiser thread per gene being fitted (at most 256
+ * at once, the next gene starting when a fit ends); their requests are
+ * served in rounds, each evaluating the genes whose points are ready
+ * (continuous batching), so a gene costs device time only for its own
+ * points.  lo/hi (either, or any entry, may be NULL:
  * codeml's bounds); info[3*gene .. 3*gene+2] as bsa_fit's info (may be
  * NULL); every gene's last evaluation is left at its estimates. */
 int  bsa_batch_fit(bsa_batch *batch, double *const *x, const double *const *lo,

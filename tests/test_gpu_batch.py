@@ -193,3 +193,30 @@ def test_batched_scan_equals_scan():
         assert r["converged"]
         assert abs(r["lnL1"] - s["lnL1"]) <= 1e-5 and abs(r["lnL0"] - s["lnL0"]) <= 1e-5
         assert abs(r["stat"] - s["stat"]) <= 2e-5
+
+
+def test_batch_fit_of_more_genes_than_fit_threads():
+    say("case: batch fit of 300 small genes (more than the 256 fit threads)")
+    rng = np.random.default_rng(123)
+    genes = [make_gene(rng, int(rng.integers(4, 7)), int(rng.integers(20, 60))) for _ in range(300)]
+    starts = []
+    for tree, _, _ in genes:
+        s = codeml.default_start(tree, 2.0, 1.5, False)
+        s[:tree.nbranch] = 0.1
+        starts.append(s)
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in genes])
+    try:
+        xb, fb, ib = b.fit(starts)
+        t = b.timing()
+    finally:
+        b.close()
+    say(f"  rounds {t['n_eval']:.0f}, evaluations {sum(i['evaluations'] for i in ib)}")
+    assert all(i["converged"] for i in ib)
+    for k in (0, 150, 299):
+        tree, data, _ = genes[k]
+        e = engine.BranchSiteEngine(tree, data)
+        try:
+            xs, fs, _ = e.fit(starts[k])
+        finally:
+            e.close()
+        assert abs(fb[k] - fs) <= 1e-6 * abs(fs)
