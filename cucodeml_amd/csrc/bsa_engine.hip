@@ -178,9 +178,14 @@ extern "C" int bsa_lnl_grad_fd(bsa_engine *e, const double *x, int np, int centr
 // class 2b (1, w2), class 1 (1, 1)); the 10^4-point grid over
 // (p0-p1 triangle, w0, w2) is integrated on the GPU.
 namespace {
+// device buffer released when it goes out of scope (release errors cannot
+// be reported from a destructor and are ignored)
 struct DevBuf {
     void *p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
 };
 }  // namespace
 
@@ -458,26 +463,25 @@ extern "C" int bsa_debug_mfma_probe(int device, const double *A, const double *B
 {
     HIP_SHARED;
     HIPCHK(hipSetDevice(device));
-    double *dA, *dB, *dD;
-    float *fA, *fB, *fD;
-    HIPCHK(hipMalloc(&dA, 64 * 8));
-    HIPCHK(hipMalloc(&dB, 64 * 8));
-    HIPCHK(hipMalloc(&dD, 256 * 8));
-    HIPCHK(hipMalloc(&fA, 64 * 4));
-    HIPCHK(hipMalloc(&fB, 64 * 4));
-    HIPCHK(hipMalloc(&fD, 256 * 4));
+    DevBuf bA, bB, bD, cA, cB, cD;
+    HIPCHK(hipMalloc(&bA.p, 64 * 8));
+    HIPCHK(hipMalloc(&bB.p, 64 * 8));
+    HIPCHK(hipMalloc(&bD.p, 256 * 8));
+    HIPCHK(hipMalloc(&cA.p, 64 * 4));
+    HIPCHK(hipMalloc(&cB.p, 64 * 4));
+    HIPCHK(hipMalloc(&cD.p, 256 * 4));
     float hA[64], hB[64], hD[256];
     for (int i = 0; i < 64; i++) { hA[i] = (float)A[i]; hB[i] = (float)B[i]; }
-    HIPCHK(hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(fA, hA, 64 * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(fB, hB, 64 * 4, hipMemcpyHostToDevice));
-    launch_mfma_probe(nullptr, dA, dB, dD, fA, fB, fD);
+    HIPCHK(hipMemcpy(bA.p, A, 64 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(bB.p, B, 64 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(cA.p, hA, 64 * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(cB.p, hB, 64 * 4, hipMemcpyHostToDevice));
+    launch_mfma_probe(nullptr, (const double *)bA.p, (const double *)bB.p, (double *)bD.p,
+                      (const float *)cA.p, (const float *)cB.p, (float *)cD.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(D64, dD, 256 * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hD, fD, 256 * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(D64, bD.p, 256 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hD, cD.p, 256 * 4, hipMemcpyDeviceToHost));
     for (int i = 0; i < 256; i++) D32[i] = hD[i];
-    hipFree(dA); hipFree(dB); hipFree(dD); hipFree(fA); hipFree(fB); hipFree(fD);
     return BSA_OK;
 }

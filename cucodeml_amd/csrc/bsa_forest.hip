@@ -391,24 +391,22 @@ static int gtip(const Gene &g, int t) { return t < 0 ? -1 : g.tip_base + t; }
 
 void forest_free(Forest &f)
 {
-    if (f.stream) hipSetDevice(f.dev);
+    // best-effort release: errors cannot be reported here and are ignored
+    if (f.stream) (void)hipSetDevice(f.dev);
     void *dev_ptrs[] = {f.d_uvr, f.d_pjobs, f.d_prune, f.d_rargs, f.d_PT, f.d_AT, f.d_L, f.d_E,
                         f.d_z, f.d_mask_amb, f.d_pi, f.d_fpatt, f.d_fh, f.d_site, f.d_partial,
                         f.d_out, f.d_adj, f.d_ejobs, f.d_set_off, f.d_set_list, f.d_A, f.d_C,
                         f.d_EA, f.d_Mpart, f.d_W, f.d_G, f.d_dtau, f.d_Z, f.d_mask64, f.d_sel};
     for (void *p : dev_ptrs)
-        if (p) hipFree(p);
-    if (f.h_uvr) hipHostFree(f.h_uvr);
-    if (f.h_out) hipHostFree(f.h_out);
-    if (f.h_pjobs) hipHostFree(f.h_pjobs);
-    if (f.h_ejobs) hipHostFree(f.h_ejobs);
-    if (f.h_adjout) hipHostFree(f.h_adjout);
-    if (f.h_sel) hipHostFree(f.h_sel);
+        if (p) (void)hipFree(p);
+    void *host_ptrs[] = {f.h_uvr, f.h_out, f.h_pjobs, f.h_ejobs, f.h_adjout, f.h_sel};
+    for (void *p : host_ptrs)
+        if (p) (void)hipHostFree(p);
     for (auto &v : f.ev)
-        if (v) hipEventDestroy(v);
+        if (v) (void)hipEventDestroy(v);
     for (auto &gr : f.graph)
-        if (gr) hipGraphExecDestroy(gr);
-    if (f.stream) hipStreamDestroy(f.stream);
+        if (gr) (void)hipGraphExecDestroy(gr);
+    if (f.stream) (void)hipStreamDestroy(f.stream);
     f = Forest();
 }
 
@@ -1104,13 +1102,13 @@ static int capture_graph(Forest &f, bool adjoint)
     const int rc = enqueue_device<T>(f, s, adjoint, false);
     const hipError_t ec = hipStreamEndCapture(s, &graph);
     if (rc) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
     if (ec != hipSuccess) return fail(BSA_EDEVICE, "graph capture: %s", hipGetErrorString(ec));
     hipGraphExec_t gr = nullptr;
     const hipError_t ei = hipGraphInstantiate(&gr, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return fail(BSA_EDEVICE, "graph instantiation: %s", hipGetErrorString(ei));
     f.graph[adjoint ? 1 : 0] = gr;
     return BSA_OK;

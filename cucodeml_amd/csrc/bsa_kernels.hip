@@ -564,8 +564,8 @@ __device__ inline double pow2i(int d)
 // consecutive states j0..j0+15 of pattern h (one lane), and its exponent.
 template <typename T>
 __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npad,
-                              const unsigned char *z, const double *PT, const double *TT, int ncodes,
-                              const int *E, const T *Cb, const T *A, const int *EA, double *x)
+                              const unsigned char *z, const double *TT, int ncodes, const int *E,
+                              const T *Cb, const T *A, const int *EA, double *x)
 {
     const int asl = jb.a_aslot[term];
     Store<T>::load16(A + ((size_t)asl * npad + h) * 64 + j0, x);
@@ -592,13 +592,13 @@ __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npa
 // largest; returns the exponent.
 template <typename T>
 __device__ inline int x_sum16(const AdjJob &jb, int h, int j0, int npad, const unsigned char *z,
-                              const double *PT, const double *TT, int ncodes, const int *E,
-                              const T *Cb, const T *A, const int *EA, double *x)
+                              const double *TT, int ncodes, const int *E, const T *Cb,
+                              const T *A, const int *EA, double *x)
 {
-    int ex = x_row16<T>(jb, 0, h, j0, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
+    int ex = x_row16<T>(jb, 0, h, j0, npad, z, TT, ncodes, E, Cb, A, EA, x);
     for (int term = 1; term < jb.nterm; term++) {
         double t[16];
-        const int et = x_row16<T>(jb, term, h, j0, npad, z, PT, TT, ncodes, E, Cb, A, EA, t);
+        const int et = x_row16<T>(jb, term, h, j0, npad, z, TT, ncodes, E, Cb, A, EA, t);
         const int em = ex > et ? ex : et;
         const double f0 = pow2i(ex - em), f1 = pow2i(et - em);
 #pragma unroll
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
         if (tile >= ntile) break;
         const int hw = tile * 64 + 16 * w;
         double x[16];
-        const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
+        const int ex = x_sum16<T>(jb, hw + r, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
         if (t == 0) __syncthreads();
         d4 Y[4];
 #pragma unroll
@@ -688,7 +688,6 @@ template <typename T>
 __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs,
                                                   const int *__restrict__ sel, int npad,
                                                   const unsigned char *__restrict__ z,
-                                                  const double *__restrict__ PT,
                                                   const double *__restrict__ TT, int ncodes,
                                                   const double *__restrict__ mask64,
                                                   const T *__restrict__ L,
@@ -713,7 +712,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
         if (tile >= ntile) break;
         const int h0 = tile * 64, hx = h0 + 16 * w + r;
         double x[16];
-        int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, PT, TT, ncodes, E, Cb, A, EA, x);
+        int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
         if (!vtip) ex += E[(size_t)jb.v_slot * npad + hx];
         const double sc = pow2i(ex);
         __syncthreads();   // the previous tile's reads of sX are done
@@ -971,8 +970,9 @@ void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, const int *sel, int njo
                       const T *Cb, const T *A, const int *EA, double *Mpart)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel, npad, z, PT,
-                       TT, ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk, tpc);
+    (void)PT;   // tip rows come from TT, interior rows from L
+    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel, npad, z, TT,
+                       ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk, tpc);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,

@@ -13,8 +13,9 @@ kernels of different genes overlap on the GPU; work is handed out
 largest-first (npatt x taxa), the LPT order, so the long genes do not end
 the scan alone.  ``scan_batched``: genes of similar pattern count grouped
 into batches (one forest of genes per batch, bsa_batch_*), each batch's H1
-fits and then its H0 fits run in lockstep, so one launch per tree level
-serves every gene of the batch; a few batches run at once on host threads.
+fits and then its H0 fits served in rounds of the genes whose points are
+ready, so one launch per tree level serves many genes; a few batches run
+at once on host threads.
 Small genes are bound by launch processing, not by the GPU's throughput,
 which the batches amortise.
 
@@ -137,7 +138,7 @@ def batch_groups(genes, batch_genes=64, max_waste=0.25):
 
 def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
     """H1 and H0 fits and the LRT of a list of genes on one batch engine
-    (lockstep fits).  Same starting points and optimiser as ``fit_gene``;
+    (bsa_batch_fit).  Same starting points and optimiser as ``fit_gene``;
     a gene whose H0 maximum lies above its H1 one is fitted again under H1
     from the H0 estimates on its own engine.  Returns the per-gene dicts
     of ``fit_gene`` (seconds = the batch's wall time)."""
