@@ -118,3 +118,59 @@ def test_adjoint_full_size_directional():
     rel = np.abs(g32 - g) / np.maximum(1.0, np.abs(g))
     say(f"  fp32 grad max rel err {rel.max():.3e}, lnL diff {f32 - f:.3e}")
     assert rel.max() < 1e-3
+
+
+def test_adjoint_full_size_every_component():
+    """128 taxa x 10,000 codons (the gradient-check workload): every one of
+    the 258 components of the adjoint gradient, FP64 and FP32 storage,
+    against Richardson-extrapolated central differences of the FP64
+    likelihood (steps h and 2h per component, h = 1e-4 (|x_i| + 1), at most
+    1 % of a branch length; the h^2 term cancels, rounding noise ~|f| 1e-14
+    / h)."""
+    say("case: adjoint full size, every component")
+    rng = np.random.default_rng(7)
+    tree = simulate.yule_tree(128, 0.05, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    p = simulate.uniform_params(rng)
+    seqs, _ = simulate.simulate_model_a(tree, 10000, p["kappa"], p["w0"], p["w2"], p["p0"],
+                                        p["p1"], pi, rng)
+    data = formats.load_codon_data(tree.names, seqs)
+    x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"], p["w2"])
+    nb = tree.nbranch
+    e64 = engine.BranchSiteEngine(tree, data)
+    try:
+        f, g64 = e64.lnl_grad(x)
+        fd = np.zeros(len(x))
+        hs = np.zeros(len(x))
+        for i in range(len(x)):
+            # branch steps stay well inside (0, t): 2h is at most 2 % of the
+            # branch, so the h^4 term left by the extrapolation is ~1e-8
+            h = 1e-4 * (abs(x[i]) + 1)
+            if i < nb:
+                h = min(h, 0.01 * x[i])
+            hs[i] = h
+
+            def cd(hh):
+                xp, xm = x.copy(), x.copy()
+                xp[i] += hh
+                xm[i] -= hh
+                return (e64.lnl(xp) - e64.lnl(xm)) / (2 * hh)
+            fd[i] = (4 * cd(h) - cd(2 * h)) / 3
+    finally:
+        e64.close()
+    e32 = engine.BranchSiteEngine(tree, data, precision="fp32")
+    try:
+        f32, g32 = e32.lnl_grad(x)
+    finally:
+        e32.close()
+    noise = 3e-14 * abs(f) / hs
+    tol64 = 5e-6 * np.maximum(1.0, np.abs(fd)) + noise
+    err64 = np.abs(g64 - fd)
+    err32 = np.abs(g32 - g64) / np.maximum(1.0, np.abs(g64))
+    worst = int(np.argmax(err64 / tol64))
+    say(f"  np {len(x)}: max |g64 - fd| {err64.max():.3e} (component {worst}, tol "
+        f"{tol64[worst]:.3e}); max rel |g32 - g64| {err32.max():.3e}; lnL fp32 - fp64 "
+        f"{f32 - f:.3e}")
+    assert np.all(err64 <= tol64)
+    assert err32.max() <= 1e-4
+    assert abs(f32 - f) <= 1e-4
