@@ -171,6 +171,81 @@ def genome_scan(ngenes, workers, precision, batch_genes, concurrent):
             "mean_evaluations": float(np.mean([r["evaluations"] for r in res]))}
 
 
+def large_fit(ns=512, ncodon=50000, gap=0.1, seed=4, e=1e-8):
+    """Full ML fit (H1) of the large-gene workload, 512 taxa x 50,000 codons
+    with 10 % gaps, from codeml's start (all branches 0.1): with FP64
+    partials; with FP32 partials finished on FP64 (as the driver does); and
+    with FP32 partials started at the FP64 estimates, which shows where FP32
+    partials put the same maximum.  The engine's bounded BFGS runs on a
+    Python objective that prints its progress, so that a long fit reports
+    as it goes."""
+    from cucodeml_amd import codeml
+    rng = np.random.default_rng(seed)
+    tree = simulate.yule_tree(ns, 0.02, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    p = simulate.uniform_params(rng)
+    seqs, _ = simulate.simulate_model_a(tree, ncodon, p["kappa"], p["w0"], p["w2"], p["p0"],
+                                        p["p1"], pi, rng, gap_frac=gap)
+    data = formats.load_codon_data(tree.names, seqs)
+    x0 = codeml.default_start(tree, 2.0, 1.5, False)
+    x0[:tree.nbranch] = 0.1
+
+    def fit(precision, x, label):
+        eng = engine.BranchSiteEngine(tree, data, precision=precision)
+        n = [0]
+        t0 = time.perf_counter()
+
+        def fun(v):
+            f, g = eng.lnl_grad(v)
+            n[0] += 1
+            if n[0] % 100 == 0:
+                print(f"large fit {label}: {n[0]} evaluations, -lnL {f:.6f}, "
+                      f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+            return f, g
+        try:
+            lo, hi = eng.default_bounds()
+            xs, f, info = engine.minimize(fun, x, lo, hi, e=e, max_iter=5000)
+            return xs, f, info, n[0], time.perf_counter() - t0
+        finally:
+            eng.close()
+
+    x64, f64, i64, n64, t64 = fit("fp64", x0, "fp64")
+    x32, f32, i32, n32, t32 = fit("fp32", x0, "fp32")
+    x32p, f32p, i32p, n32p, t32p = fit("fp64", x32, "fp32 finished on fp64")
+    # the FP32 fit started at the FP64 estimates: where FP32 partials put
+    # the same maximum
+    x32m, f32m, i32m, n32m, t32m = fit("fp32", x64, "fp32 from the fp64 estimates")
+    e64 = engine.BranchSiteEngine(tree, data, precision="fp64")
+    try:
+        f32m_64 = e64.lnl(x32m)
+    finally:
+        e64.close()
+    nb = tree.nbranch
+    par = lambda x: np.array([x[nb], *engine_ps(x, nb), x[nb + 3], x[nb + 4]])   # noqa: E731
+    names = ["kappa", "p0", "p1", "w0", "w2"]
+    return {"workload": f"{ns} taxa x {ncodon} codons ({data.npatt} patterns), {gap:.0%} gaps, "
+                        f"H1 from all branches 0.1", "e": e,
+            "fp64": {"neg_lnl": f64, "evaluations": n64, "iterations": i64["iterations"],
+                     "seconds": t64, "estimates": dict(zip(names, par(x64).tolist()))},
+            "fp32_then_fp64": {"neg_lnl_fp32": f32, "neg_lnl_after_fp64_finish": f32p,
+                               "evaluations": n32 + n32p,
+                               "iterations": i32["iterations"] + i32p["iterations"],
+                               "seconds": t32 + t32p, "fp32_part_seconds": t32,
+                               "estimates": dict(zip(names, par(x32p).tolist())),
+                               "delta_neg_lnl_vs_fp64": f32p - f64},
+            "fp32_at_the_fp64_maximum": {
+                "neg_lnl_fp32": f32m, "neg_lnl_fp64_at_its_estimates": f32m_64,
+                "delta_neg_lnl_vs_fp64": f32m_64 - f64, "evaluations": n32m,
+                "max_abs_param_diff": float(np.abs(par(x32m) - par(x64)).max()),
+                "max_abs_branch_diff": float(np.abs(x32m[:nb] - x64[:nb]).max())}}
+
+
+def engine_ps(x, nb):
+    """p0, p1 from codeml's logistic variables"""
+    a, b = np.exp(x[nb + 1]), np.exp(x[nb + 2])
+    return a / (1 + a + b), b / (1 + a + b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +260,11 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="patterns per pruning block (64/128)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the default launch shapes (no autotuning)")
+    ap.add_argument("--large-fit", action="store_true",
+                    help="also run the full H1 fit of 512 taxa x 50,000 codons, FP32 and FP64 "
+                         "(minutes; progress on stderr)")
+    ap.add_argument("--large-fit-e", type=float, default=1e-8,
+                    help="relative convergence tolerance of the large fit")
     ap.add_argument("--fd-steps", type=int, default=1,
                     help="steps of the FP64 forward-difference path timed beside (0: skip)")
     ap.add_argument("--cpu-sample-codons", type=int, default=1000)
@@ -299,6 +379,8 @@ def main():
     if args.scan_genes > 0:
         out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, "fp64",
                                          args.scan_batch, args.scan_concurrent)
+    if args.large_fit:
+        out["large_fit"] = large_fit(e=args.large_fit_e)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tree, seqs, p, args.codons, data.npatt, npar,
                                            args.cpu_sample_codons)
