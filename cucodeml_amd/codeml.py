@@ -2,8 +2,8 @@
 
 Reads a codeml control file (the subset the model-A path uses: seqfile,
 treefile, outfile, seqtype = 1, model = 2, NSsites = 2, CodonFreq = 2,
-kappa, fix_omega, omega, cleandata = 0; codeml.c:1430-1768), the PHYLIP
-codon alignment and the Newick tree with ``#1`` on the foreground branch,
+kappa, fix_omega, omega, cleandata = 0, ndata; codeml.c:1430-1768), the PHYLIP
+codon alignment(s) and the Newick tree with ``#1`` on the foreground branch,
 fits the model by maximum likelihood on the GPU (bsa_fit, in place of ming2
 at codeml.c:688) and writes the estimates in the layout of codeml's ``out``
 file (lnL line codeml.c:738-741, parameter vector 751, site-class table
@@ -151,6 +151,49 @@ def fit_model_a(tree, data, fix_omega=False, omega=1.0, start=None, precision="f
             eng.close()
 
 
+def fit_model_a_batch(problems, fix_omega=False, omega=1.0, starts=None, precision="fp64",
+                      e=1e-8, device=0, kappa=2.0, polish=True):
+    """``fit_model_a`` for several data sets at once, on one batch engine
+    (adjoint gradient; every data set has its own tree and alignment).
+    ``problems`` is a list of (tree, data); ``starts`` a list of starting
+    vectors or None entries.  Same optimiser, starting values and FP32
+    polish as the one-data-set fit, so each result equals that fit's."""
+    n = len(problems)
+    starts = [None] * n if starts is None else list(starts)
+    x0 = [default_start(t, kappa, omega if not fix_omega else 1.5, fix_omega) if s is None
+          else np.asarray(s, dtype=np.float64) for (t, _), s in zip(problems, starts)]
+    t0 = time.perf_counter()
+
+    def one_pass(prec, xs):
+        b = engine.BranchSiteBatch(problems, precision=prec, device=device,
+                                   fix_omega=fix_omega, omega_fix=omega)
+        try:
+            x, f, info = b.fit(xs, e=e)
+            return x, f, info, [b.last_params(i) for i in range(n)], b.np
+        finally:
+            b.close()
+
+    x, f, info, par, nps = one_pass(precision, x0)
+    extra = [{} for _ in range(n)]
+    if precision == "fp32" and polish:
+        extra = [{"fp32_lnL": -f[i], "fp32_iterations": info[i]["iterations"],
+                  "fp32_evaluations": info[i]["evaluations"]} for i in range(n)]
+        x2, f, info2, par, nps = one_pass("fp64", x)
+        info = [{"iterations": a["iterations"] + c["iterations"],
+                 "evaluations": a["evaluations"] + c["evaluations"],
+                 "converged": c["converged"]} for a, c in zip(info, info2)]
+        x = x2
+    sec = time.perf_counter() - t0
+    out = []
+    for i, (tree, _) in enumerate(problems):
+        nb = tree.nbranch
+        p = par[i]
+        out.append(Fit("H0" if fix_omega else "H1", -f[i], x[i], p[0], p[1], p[2], p[3], p[4],
+                       p[7:11].copy(), x[i][:nb].copy(), nps[i], info[i]["iterations"],
+                       info[i]["evaluations"], info[i]["converged"], sec, extra[i]))
+    return out
+
+
 def standard_errors(eng, x, small_diff=1e-6):
     """SEs of the parameters at x in codeml's printed form (p0, p1
     untransformed) from the Hessian approximation of Seo et al. 2004, as the
@@ -277,78 +320,131 @@ def read_in_codeml(path):
     return False, np.array(vals)
 
 
-def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
-    ctl = read_ctl(ctl_path)
-    check_ctl(ctl)
-    base = os.path.dirname(os.path.abspath(ctl_path))
-    names, seqs = formats.read_phylip(os.path.join(base, ctl["seqfile"]))
-    with open(os.path.join(base, ctl["treefile"])) as f:
+def read_tree_text(path):
+    """The first tree of a codeml tree file (which may start with "ns ntree")."""
+    with open(path) as f:
         txt = f.read()
-    # a treefile may start with "ns ntree"
     first = txt.lstrip().split("\n", 1)
     if len(first) == 2 and all(tok.isdigit() for tok in first[0].split()):
         txt = first[1]
-    tree = formats.parse_newick(txt, names)
-    if len(tree.sons[tree.root]) < 3:
-        raise ValueError("the tree is rooted; give it unrooted (root with 3 sons), as codeml "
-                         "uses with clock = 0")
-    if not np.any(tree.label == 1):
-        raise ValueError("no foreground branch (#1) in the tree")
-    data = formats.load_codon_data(names, seqs)
-    fix_omega = int(_num(ctl, "fix_omega", 0)) == 1
-    omega = _num(ctl, "omega", 1.0)
-    kappa = _num(ctl, "kappa", 2.0)
-    eng = engine.BranchSiteEngine(tree, data, precision=precision, device=device,
-                                  fix_omega=fix_omega, omega_fix=omega)
-    try:
-        start, fixed_run = None, False
-        inpath = os.path.join(base, "in.codeml")
-        if os.path.exists(inpath):
-            fixed, vals = read_in_codeml(inpath)
-            if fixed:
-                fixed_run = True
-                eng.set_model(fix_omega, omega, p_transformed=False)
-                negl = eng.lnl(vals[:eng.np])
-                par = eng.last_params()
-                nb = tree.nbranch
-                fit = Fit("H0" if fix_omega else "H1", -negl, vals[:eng.np].copy(), par[0],
-                          par[1], par[2], par[3], par[4], par[7:11].copy(), vals[:nb].copy(),
-                          eng.np, 0, 1, True, 0.0)
-                xe = vals[:eng.np].copy()
-                fit.x = engine.x_from_params(vals[:nb], par[0], par[1], par[2], par[3],
-                                             None if fix_omega else par[4])
-            else:
-                start = vals[:eng.np]
-        if not fixed_run:
-            fit = fit_model_a(tree, data, fix_omega, omega, start=start, gradient=gradient,
-                              kappa=kappa, eng=eng)
-            xe = fit.x
-        if int(_num(ctl, "getSE", 0)) >= 1:
-            fit.extra["se"] = standard_errors(eng, fit.x_untransformed(),
-                                              _num(ctl, "Small_Diff", 1e-6))
-            eng.set_model(fix_omega, omega, p_transformed=not fixed_run)
-        # site posteriors under H1, as the reference prints after the fit
-        # (NEB codeml.c:5079-5129, BEB 6617-6826)
-        if not fix_omega:
-            neb = eng.neb(xe)
-            beb = eng.beb(xe)
-            fit.extra["neb"], fit.extra["beb"] = neb, beb
-    finally:
-        eng.close()
+    return txt
+
+
+def load_problems(ctl, base):
+    """(tree, data, names) for each of the ``ndata`` alignments of the
+    sequence file, all on the tree of the tree file (codeml.c:315-343 reads
+    the data sets one after another from the open sequence file; Forestry
+    reads the same tree file for each)."""
+    ndata = max(1, int(_num(ctl, "ndata", 1)))
+    sets = formats.read_phylip_sets(os.path.join(base, ctl["seqfile"]), ndata)
+    txt = read_tree_text(os.path.join(base, ctl["treefile"]))
+    out = []
+    for names, seqs in sets:
+        tree = formats.parse_newick(txt, names)
+        if len(tree.sons[tree.root]) < 3:
+            raise ValueError("the tree is rooted; give it unrooted (root with 3 sons), as codeml "
+                             "uses with clock = 0")
+        if not np.any(tree.label == 1):
+            raise ValueError("no foreground branch (#1) in the tree")
+        out.append((tree, formats.load_codon_data(names, seqs), names))
+    return out
+
+
+def _fixed_fit(eng, tree, vals, fix_omega, omega):
+    """A Fit holding the likelihood at the fixed values of an in.codeml file
+    that starts with -1 (p0, p1 untransformed); no optimisation."""
+    eng.set_model(fix_omega, omega, p_transformed=False)
+    negl = eng.lnl(vals[:eng.np])
+    par = eng.last_params()
+    nb = tree.nbranch
+    fit = Fit("H0" if fix_omega else "H1", -negl, vals[:eng.np].copy(), par[0], par[1], par[2],
+              par[3], par[4], par[7:11].copy(), vals[:nb].copy(), eng.np, 0, 1, True, 0.0)
+    fit.x = engine.x_from_params(vals[:nb], par[0], par[1], par[2], par[3],
+                                 None if fix_omega else par[4])
+    return fit, vals[:eng.np].copy()
+
+
+def _report(tree, data, names, fit, fix_omega):
+    """(out text, rst text) of one data set."""
     text = format_out(tree, data, fit, names)
     if "se" in fit.extra:
         text += "SEs for parameters:\n" + "".join(f" {v:8.6f}" for v in fit.extra["se"]) + "\n"
+    rst = ""
     if not fix_omega:
         text += format_beb_out(fit.extra["beb"], data)
+        rst = (format_site_table("Naive Empirical Bayes (NEB) probabilities for 4 classes",
+                                 fit.extra["neb"], data, names) + "\n\n"
+               + format_site_table("Bayes Empirical Bayes (BEB) probabilities for 4 classes "
+                                   "(class)", fit.extra["beb"]["post"], data, names))
+    return text, rst
+
+
+def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
+    """Analyse the data sets of a control file and write ``out`` and ``rst``.
+    Returns the Fit, or with ``ndata`` > 1 the list of Fits, one per data
+    set, whose sections in ``out`` and ``rst`` open with "Data set k" as
+    codeml's do (codeml.c:316-320).  The ML fits of several data sets run
+    together on one batch engine (adjoint gradient)."""
+    ctl = read_ctl(ctl_path)
+    check_ctl(ctl)
+    base = os.path.dirname(os.path.abspath(ctl_path))
+    problems = load_problems(ctl, base)
+    fix_omega = int(_num(ctl, "fix_omega", 0)) == 1
+    omega = _num(ctl, "omega", 1.0)
+    kappa = _num(ctl, "kappa", 2.0)
+    start, fixed_vals = None, None
+    inpath = os.path.join(base, "in.codeml")
+    if os.path.exists(inpath):
+        fixed, vals = read_in_codeml(inpath)
+        if fixed:
+            fixed_vals = vals
+        else:
+            start = vals
+    fits = [None] * len(problems)
+    if len(problems) > 1 and fixed_vals is None and gradient == "adjoint":
+        nps = [t.nbranch + (4 if fix_omega else 5) for t, _, _ in problems]
+        fits = fit_model_a_batch([(t, d) for t, d, _ in problems], fix_omega, omega,
+                                 [None if start is None else start[:m] for m in nps],
+                                 precision=precision, device=device, kappa=kappa)
+    out_parts, rst_parts = [], []
+    for k, (tree, data, names) in enumerate(problems):
+        eng = engine.BranchSiteEngine(tree, data, precision=precision, device=device,
+                                      fix_omega=fix_omega, omega_fix=omega)
+        try:
+            fit, xe = fits[k], None
+            fixed_run = fixed_vals is not None
+            if fixed_run:
+                fit, xe = _fixed_fit(eng, tree, fixed_vals, fix_omega, omega)
+            elif fit is None:
+                fit = fit_model_a(tree, data, fix_omega, omega,
+                                  start=None if start is None else start[:eng.np],
+                                  gradient=gradient, kappa=kappa, eng=eng)
+            if xe is None:
+                xe = fit.x
+            if int(_num(ctl, "getSE", 0)) >= 1:
+                fit.extra["se"] = standard_errors(eng, fit.x_untransformed(),
+                                                  _num(ctl, "Small_Diff", 1e-6))
+                eng.set_model(fix_omega, omega, p_transformed=not fixed_run)
+            # site posteriors under H1, as the reference prints after the fit
+            # (NEB codeml.c:5079-5129, BEB 6617-6826)
+            if not fix_omega:
+                fit.extra["neb"] = eng.neb(xe)
+                fit.extra["beb"] = eng.beb(xe)
+        finally:
+            eng.close()
+        fits[k] = fit
+        text, rst = _report(tree, data, names, fit, fix_omega)
+        if len(problems) > 1:
+            text = f"\n\nData set {k + 1}\n" + text
+            rst = f"\t{k + 1}\n" + rst
+        out_parts.append(text)
+        rst_parts.append(rst)
+    if not fix_omega:
         with open(os.path.join(base, "rst"), "w") as f:
-            f.write(format_site_table("Naive Empirical Bayes (NEB) probabilities for 4 classes",
-                                      fit.extra["neb"], data, names))
-            f.write("\n\n")
-            f.write(format_site_table("Bayes Empirical Bayes (BEB) probabilities for 4 classes "
-                                      "(class)", fit.extra["beb"]["post"], data, names))
+            f.write("".join(rst_parts))
     with open(os.path.join(base, ctl.get("outfile", "out")), "w") as f:
-        f.write(text)
-    return fit
+        f.write("".join(out_parts))
+    return fits[0] if len(fits) == 1 else fits
 
 
 def main(argv=None):
@@ -358,9 +454,11 @@ def main(argv=None):
     ap.add_argument("--fd", action="store_true", help="forward-difference gradient (gradientB)")
     ap.add_argument("--device", type=int, default=0)
     a = ap.parse_args(argv)
-    fit = run(a.ctl, a.precision, "fd" if a.fd else "adjoint", a.device)
-    print(f"lnL = {fit.lnL:.6f}  np = {fit.np}  iterations = {fit.iterations}  "
-          f"time = {fit.seconds:.2f} s")
+    fits = run(a.ctl, a.precision, "fd" if a.fd else "adjoint", a.device)
+    for k, fit in enumerate(fits if isinstance(fits, list) else [fits]):
+        head = f"data set {k + 1}: " if isinstance(fits, list) else ""
+        print(f"{head}lnL = {fit.lnL:.6f}  np = {fit.np}  iterations = {fit.iterations}  "
+              f"time = {fit.seconds:.2f} s")
     return 0
 
 

@@ -183,15 +183,34 @@ def parse_newick(text, names):
 
 def read_phylip(path_or_text):
     """Read a sequential PHYLIP nucleotide alignment (names then sequences)."""
+    names, seqs = read_phylip_sets(path_or_text, 1)[0]
+    return names, seqs
+
+
+def read_phylip_sets(path_or_text, ndata):
+    """The first `ndata` alignments of a file holding several sequential
+    PHYLIP alignments one after another (codeml's ndata, codeml.c:315-343):
+    a list of (names, seqs)."""
     if "\n" in path_or_text:
         text = path_or_text
     else:
         with open(path_or_text) as f:
             text = f.read()
     lines = text.splitlines()
-    k = 0
-    while not lines[k].strip():
-        k += 1
+    out, k = [], 0
+    for i in range(ndata):
+        while k < len(lines) and not lines[k].strip():
+            k += 1
+        if k >= len(lines):
+            raise ValueError(f"the sequence file holds {i} data sets, ndata = {ndata}")
+        names, seqs, k = _read_phylip_at(lines, k)
+        out.append((names, seqs))
+    return out
+
+
+def _read_phylip_at(lines, k):
+    """One alignment starting at line k (its "ns ls" line); returns (names,
+    seqs, first line after it)."""
     ns, ls = (int(v) for v in lines[k].split()[:2])
     k += 1
     names, seqs = [], []
@@ -208,7 +227,7 @@ def read_phylip(path_or_text):
             k += 1
         names.append(name)
         seqs.append("".join(buf[:ls]).upper())
-    return names, seqs
+    return names, seqs, k
 
 
 def write_phylip(path, names, seqs):

@@ -73,3 +73,31 @@ def test_out_file_layout_round_trip():
     want = fit.x_untransformed()
     assert np.allclose(xr, np.round(want, 6), atol=1e-12)
     assert abs(xr[nb + 1] - 0.5) < 1e-12 and abs(xr[nb + 2] - 0.3) < 1e-12
+
+
+def test_load_problems_reads_ndata_sets_on_one_tree():
+    fx, tree, data = fixtures.load("cfg0")
+    lines = fx["phylip"].strip().splitlines()
+    head, rows = lines[0], [ln for ln in lines[1:] if ln.strip()]
+    # the second data set lists the same taxa in reverse order
+    second = head + "\n" + "\n".join(reversed(rows)) + "\n"
+    with tempfile.TemporaryDirectory() as d:
+        write(d, "seq.phy", fx["phylip"] + "\n" + second)
+        write(d, "tree.nwk", f"{fx['ns']} 1\n{fx['newick']}\n")
+        ctl = codeml.read_ctl(write(d, "codeml.ctl", fixtures.CTL.format(fix_omega=0, omega=1.5)
+                                    + "ndata = 2\n"))
+        probs = codeml.load_problems(ctl, d)
+        assert len(probs) == 2
+        ctl1 = dict(ctl, ndata="1")
+        assert len(codeml.load_problems(ctl1, d)) == 1
+        ctl3 = dict(ctl, ndata="3")
+        with pytest.raises(ValueError):
+            codeml.load_problems(ctl3, d)
+    (t1, d1, n1), (t2, d2, n2) = probs
+    assert n2 == list(reversed(n1))
+    assert d1.npatt == data.npatt and d2.npatt == data.npatt
+    # the tree numbers tips in the order of each alignment, so the two
+    # problems are the same up to that relabelling: same branch lengths
+    # as a multiset and the same compressed site counts
+    assert np.allclose(np.sort(t1.branch_vector()), np.sort(t2.branch_vector()))
+    assert np.array_equal(np.sort(d1.fpatt), np.sort(d2.fpatt))

@@ -72,6 +72,23 @@ def test_phylip_roundtrip(tmp_path):
     assert names == ["s1", "long name"] and seqs == ["AAATTT", "CCCGGG"]
 
 
+def test_phylip_several_data_sets(tmp_path):
+    # ndata alignments one after another in one file (codeml.c:315-343)
+    p = tmp_path / "m.phy"
+    text = ("2 6\ns1  AAATTT\ns2  CCCGGG\n\n\n"
+            "3 3\nb  TTT\na  GGG\nc  AAA\n"
+            "2 9\nx  AAA CCC\nGGG\ny  TTTTTTTTT\n")
+    p.write_text(text)
+    sets = formats.read_phylip_sets(str(p), 3)
+    assert sets[0] == (["s1", "s2"], ["AAATTT", "CCCGGG"])
+    assert sets[1] == (["b", "a", "c"], ["TTT", "GGG", "AAA"])
+    assert sets[2] == (["x", "y"], ["AAACCCGGG", "TTTTTTTTT"])
+    assert formats.read_phylip_sets(str(p), 1) == sets[:1]
+    assert formats.read_phylip(str(p)) == sets[0]
+    with pytest.raises(ValueError, match="3 data sets"):
+        formats.read_phylip_sets(str(p), 4)
+
+
 @pytest.mark.parametrize("ns", [4, 12, 64])
 def test_yule_tree_shape(ns):
     t = simulate.yule_tree(ns, 0.05, np.random.default_rng(ns))

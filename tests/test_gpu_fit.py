@@ -105,6 +105,50 @@ def test_driver_fixed_parameters_match_reference_lnl():
     assert abs(lnl - fx["fixed_lnL"]) <= 2e-6
 
 
+def test_driver_several_data_sets_equal_one_at_a_time():
+    # ndata = 3: the alignments follow one another in the sequence file and
+    # share the tree file (codeml.c:315-343); the driver fits them together
+    # on one batch engine.  Each data set's section of out must equal a run
+    # of the driver on that data set alone.
+    print("case: driver ndata = 3 (cfg0, a simulated set, cfg0 rows reversed), H1", flush=True)
+    from cucodeml_amd import simulate
+    fx, tree, _ = fixtures.load("cfg0")
+    rng = np.random.default_rng(31)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    seqs_b, _ = simulate.simulate_model_a(tree, 90, 2.5, 0.2, 3.0, 0.5, 0.3, pi, rng)
+    lines = fx["phylip"].strip().splitlines()
+    rev = lines[0] + "\n" + "\n".join(reversed([ln for ln in lines[1:] if ln.strip()])) + "\n"
+    texts = [fx["phylip"].strip() + "\n",
+             f"{len(tree.names)} {len(seqs_b[0])}\n"
+             + "".join(f"{n}  {q}\n" for n, q in zip(tree.names, seqs_b)), rev]
+    ctl = fixtures.CTL.format(fix_omega=0, omega=1.5)
+    single = []
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "tree.nwk"), "w").write(f"{fx['ns']} 1\n{fx['newick']}\n")
+        for t in texts:
+            open(os.path.join(d, "seq.phy"), "w").write(t)
+            open(os.path.join(d, "codeml.ctl"), "w").write(ctl)
+            single.append(codeml.run(os.path.join(d, "codeml.ctl")))
+        open(os.path.join(d, "seq.phy"), "w").write("\n".join(texts))
+        open(os.path.join(d, "codeml.ctl"), "w").write(ctl + "ndata = 3\n")
+        fits = codeml.run(os.path.join(d, "codeml.ctl"))
+        out = open(os.path.join(d, "out")).read()
+        rst = open(os.path.join(d, "rst")).read()
+    assert len(fits) == 3
+    sections = out.split("\n\nData set ")[1:]
+    assert [s.split("\n", 1)[0] for s in sections] == ["1", "2", "3"]
+    assert rst.count("Bayes Empirical Bayes (BEB) probabilities") == 3
+    for k, (f, g, sec) in enumerate(zip(fits, single, sections)):
+        lnl, npar, x = fixtures.parse_out(sec)
+        print(f"  data set {k + 1}: lnL {f.lnL:.6f} alone {g.lnL:.6f}; {f.iterations} / "
+              f"{g.iterations} iterations", flush=True)
+        assert f.converged
+        assert abs(f.lnL - g.lnL) <= 1e-6 * abs(g.lnL)
+        assert abs(lnl - round(f.lnL, 6)) < 1e-9 and npar == f.np
+    # cfg0 with its rows in reverse order is the same problem
+    assert abs(fits[0].lnL - fits[2].lnL) <= 1e-6 * abs(fits[0].lnL)
+
+
 def test_lrt_statistic():
     print("case: LRT cfg0", flush=True)
     fx, tree, data = fixtures.load("cfg0")
