@@ -153,6 +153,9 @@ struct Forest {
     int *d_EA = nullptr;
     double *d_Mpart = nullptr, *d_W = nullptr, *d_G = nullptr, *d_dtau = nullptr, *d_Z = nullptr;
     double *d_mask64 = nullptr;
+    // per tip row (ns_total x npad): the tip's patterns ordered by its code
+    // (the order the branch-matrix kernel visits a tip's patterns in)
+    int *d_tperm = nullptr;
     double *h_adjout = nullptr;                    // dtau[nej], then Z[3 G][4096]
 
     size_t model_block() const { return (size_t)G * kGeneBlock; }

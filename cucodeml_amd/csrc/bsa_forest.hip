@@ -346,9 +346,12 @@ void gene_plan_adjoint(Gene &g, int wbytes)
     // children only): one 61 x 61 contraction per pattern; bytes: the
     // parent's adjoint per term, the siblings' kept contributions (interior)
     // or codes (tips), exponents, the child's adjoint and exponent written.
-    // Branch matrices (every job): one 61 x 61 outer-product accumulation per
-    // pattern; bytes: the same X operands, plus the child's partials and
-    // exponent (interior) or code (tip).  The contributions C kept by the
+    // Branch matrices (every job): for an interior child one 61 x 61
+    // outer-product accumulation per pattern; for a tip, whose partials are
+    // code indicators, one row of X added into the column of the tip's codon
+    // per pattern.  Bytes: the same X operands, plus the child's partials and
+    // exponent (interior) or code and place in the tip's pattern order (tip).
+    // The contributions C kept by the
     // forward pass are the siblings' partials times P, so no contraction is
     // repeated for them.
     const double w = wbytes, np_ = g.npatt;
@@ -364,8 +367,8 @@ void gene_plan_adjoint(Gene &g, int wbytes)
             fa += 2.0 * kS * kS * np_;
             ba += np_ * (xbytes + kS * w + 4.0);
         }
-        fm += 2.0 * kS * kS * np_;
-        bm += np_ * (xbytes + (vint ? kS * w + 4.0 : 1.0));
+        fm += vint ? 2.0 * kS * kS * np_ : (double)kS * np_;
+        bm += np_ * (xbytes + (vint ? kS * w + 4.0 : 1.0 + 4.0));
     }
     g.work[6] = fa;
     g.work[7] = ba;
@@ -396,7 +399,8 @@ void forest_free(Forest &f)
     void *dev_ptrs[] = {f.d_uvr, f.d_pjobs, f.d_prune, f.d_rargs, f.d_PT, f.d_AT, f.d_L, f.d_E,
                         f.d_z, f.d_mask_amb, f.d_pi, f.d_fpatt, f.d_fh, f.d_site, f.d_partial,
                         f.d_out, f.d_adj, f.d_ejobs, f.d_set_off, f.d_set_list, f.d_A, f.d_C,
-                        f.d_EA, f.d_Mpart, f.d_W, f.d_G, f.d_dtau, f.d_Z, f.d_mask64, f.d_sel};
+                        f.d_EA, f.d_Mpart, f.d_W, f.d_G, f.d_dtau, f.d_Z, f.d_mask64, f.d_tperm,
+                        f.d_sel};
     for (void *p : dev_ptrs)
         if (p) (void)hipFree(p);
     void *host_ptrs[] = {f.h_uvr, f.h_out, f.h_pjobs, f.h_ejobs, f.h_adjout, f.h_sel};
@@ -699,6 +703,31 @@ int forest_setup_adjoint(Forest &f)
                 m64[(size_t)(gp->mask_base + c) * 64 + k] = gp->code_mask[(size_t)c * kS + k];
     HIPCHK(hipMalloc(&f.d_mask64, sizeof(double) * m64.size()));
     HIPCHK(hipMemcpy(f.d_mask64, m64.data(), sizeof(double) * m64.size(), hipMemcpyHostToDevice));
+    // each tip's patterns in the order of its code (counting sort, stable),
+    // the forest's padding patterns after them in place: a 64-pattern tile
+    // of a tip's branch matrix then holds one or a few codons.  An entry is
+    // the pattern (low 24 bits) and its code (top 8 bits).
+    if (f.npad > (1 << 24)) return fail(BSA_EINVAL, "more than 2^24 site patterns");
+    std::vector<int> tp((size_t)f.ns_total * f.npad);
+    for (Gene *gp : f.genes) {
+        const Gene &g = *gp;
+        for (int v = 0; v < g.ns; v++) {
+            int *row = tp.data() + (size_t)(g.tip_base + v) * f.npad;
+            const unsigned char *zv = g.z.data() + (size_t)v * g.npatt;
+            int start[257] = {0};
+            for (int h = 0; h < g.npatt; h++) start[zv[h] + 1]++;
+            for (int c = 0; c < 256; c++) start[c + 1] += start[c];
+            for (int h = 0; h < g.npatt; h++) row[start[zv[h]]++] = h;
+            for (int h = g.npatt; h < f.npad; h++) row[h] = h;
+            for (int i = 0; i < f.npad; i++) {
+                const int h = row[i];
+                const unsigned code = h < g.npatt ? zv[h] : 0u;
+                row[i] = (int)((code << 24) | (unsigned)h);
+            }
+        }
+    }
+    HIPCHK(hipMalloc(&f.d_tperm, sizeof(int) * tp.size()));
+    HIPCHK(hipMemcpy(f.d_tperm, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice));
     HIPCHK(hipHostMalloc((void **)&f.h_adjout, sizeof(double) * (nej + 3 * G * 4096),
                          hipHostMallocDefault));
     f.adj_ready = true;
@@ -853,8 +882,9 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
         REC(6);
         launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
                             f.npad, f.nchunk, f.tpc, f.d_z, (const double *)f.d_PT,
-                            (const double *)f.d_AT, f.ncodes, f.d_mask64, (const T *)f.d_L, f.d_E,
-                            (const T *)f.d_C, (const T *)f.d_A, f.d_EA, f.d_Mpart);
+                            (const double *)f.d_AT, f.ncodes, f.d_mask64, f.d_tperm,
+                            (const T *)f.d_L, f.d_E, (const T *)f.d_C, (const T *)f.d_A, f.d_EA,
+                            f.d_Mpart);
         REC(4);
         launch_adjoint_finish(s, f.d_Mpart, f.nchunk, f.d_ejobs, rs ? rs->ej : nullptr,
                               rs ? rs->nej : (int)f.ejobs.size(), rs ? rs->sets : nullptr,

@@ -684,12 +684,19 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
 // 16w..16w+15 of M and loads its B operand (I_v) straight from HBM/L2.  The
 // waves' column strips are disjoint, so each chunk writes its partial with no
 // further reduction (chunk partials are summed later in a fixed order).
+// For a tip v, I_v(h, .) is the indicator of the codons the tip's code allows
+// (one column for a resolved codon).  Its patterns are visited in the order of
+// tperm, sorted by that code, so a tile holds one or a few codons: there wave
+// w owns rows 16w..16w+15 of M instead, and multiplies only the 16-column
+// strips that the tile's codes reach (usually one of four).  tperm entries
+// hold the pattern in the low 24 bits and its code in the top 8.
 template <typename T>
 __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs,
                                                   const int *__restrict__ sel, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ TT, int ncodes,
                                                   const double *__restrict__ mask64,
+                                                  const int *__restrict__ tperm,
                                                   const T *__restrict__ L,
                                                   const int *__restrict__ E,
                                                   const T *__restrict__ Cb,
@@ -704,13 +711,24 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
     const int ntile = npad / 64;
     const bool vtip = jb.v_tip >= 0;
     const int col = 16 * w + r;
+    const int *perm = vtip ? tperm + (size_t)jb.v_tip * npad : nullptr;
+    // a tip's pattern for this lane's X row, loaded one tile ahead
+    int pnext = 0;
+    if (vtip && chunk * tpc < ntile) pnext = perm[chunk * tpc * 64 + 16 * w + r];
     d4 Macc[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) Macc[a] = d4{0, 0, 0, 0};
     for (int t = 0; t < tpc; t++) {
         const int tile = chunk * tpc + t;
         if (tile >= ntile) break;
-        const int h0 = tile * 64, hx = h0 + 16 * w + r;
+        // row i of the tile is pattern h0 + i, or for a tip the pattern in
+        // place h0 + i of the tip's order
+        const int h0 = tile * 64;
+        int hx = h0 + 16 * w + r;
+        if (vtip) {
+            hx = pnext & 0xFFFFFF;
+            if (t + 1 < tpc && tile + 1 < ntile) pnext = perm[h0 + 64 + 16 * w + r];
+        }
         double x[16];
         int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
         if (!vtip) ex += E[(size_t)jb.v_slot * npad + hx];
@@ -718,31 +736,54 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
         __syncthreads();   // the previous tile's reads of sX are done
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) sX[(16 * w + r) * kLdsStride + 16 * g + kk] = x[kk] * sc;
-        double bv[16];
         if (vtip) {
-            // indicator of the codons each code allows (identity rows for sense codons)
+            // the tip's codes in rows 4kk + g of the tile, and the column
+            // strips they reach (every strip for an ambiguous code)
             int code[16];
+            unsigned reach = 0;
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) code[kk] = z[(size_t)jb.v_tip * npad + h0 + 4 * kk + g];
+            for (int kk = 0; kk < 16; kk++) {
+                code[kk] = (unsigned)perm[h0 + 4 * kk + g] >> 24;
+                reach |= code[kk] < 61 ? 1u << (code[kk] >> 4) : 0xFu;
+            }
+            __syncthreads();
+            // wave w: rows 16w..16w+15 of M, one product per strip the tile reaches
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) bv[kk] = mask64[(size_t)(jb.v_mask + code[kk]) * 64 + col];
+            for (int sp = 0; sp < 4; sp++) {
+                if (!__any((reach >> sp) & 1u)) continue;   // the same for the whole wave
+                const int c = 16 * sp + r;
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const double b = code[kk] < 61
+                                         ? (code[kk] == c ? 1.0 : 0.0)
+                                         : mask64[(size_t)(jb.v_mask + code[kk]) * 64 + c];
+                    Macc[sp] = mfma64(sX[(4 * kk + g) * kLdsStride + 16 * w + r], b, Macc[sp]);
+                }
+            }
         } else {
+            double bv[16];
             const T *Lv = L + ((size_t)jb.v_slot * npad + h0 + g) * 64 + col;
 #pragma unroll
             for (int kk = 0; kk < 16; kk++) bv[kk] = Store<T>::get(Lv + (size_t)(4 * kk) * 64);
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++)
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+                    Macc[a] = mfma64(sX[(4 * kk + g) * kLdsStride + 16 * a + r], bv[kk], Macc[a]);
         }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 16; kk++)
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-                Macc[a] = mfma64(sX[(4 * kk + g) * kLdsStride + 16 * a + r], bv[kk], Macc[a]);
     }
+    // interior child: Macc[a] holds rows 16a.., columns 16w..; tip: rows
+    // 16w.., columns 16a..
     double *Mo = Mpart + ((size_t)jb.m_index * nchunk + chunk) * 4096;
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) Mo[(16 * a + g + 4 * i) * 64 + col] = Macc[a][i];
+        for (int i = 0; i < 4; i++) {
+            const int row = vtip ? 16 * w + g + 4 * i : 16 * a + g + 4 * i;
+            const int cl = vtip ? 16 * a + r : col;
+            Mo[row * 64 + cl] = Macc[a][i];
+        }
 }
 
 // Per (branch, class): M = sum of its chunk partials (fixed order), then
@@ -966,13 +1007,14 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int
 template <typename T>
 void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
                       int nchunk, int tpc, const unsigned char *z, const double *PT,
-                      const double *TT, int ncodes, const double *mask64, const T *L, const int *E,
-                      const T *Cb, const T *A, const int *EA, double *Mpart)
+                      const double *TT, int ncodes, const double *mask64, const int *tperm,
+                      const T *L, const int *E, const T *Cb, const T *A, const int *EA,
+                      double *Mpart)
 {
     if (njobs <= 0) return;
     (void)PT;   // tip rows come from TT, interior rows from L
     hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel, npad, z, TT,
-                       ncodes, mask64, L, E, Cb, A, EA, Mpart, nchunk, tpc);
+                       ncodes, mask64, tperm, L, E, Cb, A, EA, Mpart, nchunk, tpc);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
@@ -996,12 +1038,13 @@ template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, const int
                                           int, const int *, const float *, float *, int *, int);
 template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, const int *, int, int, int,
                                        int, const unsigned char *, const double *, const double *,
-                                       int, const double *, const double *, const int *,
-                                       const double *, const double *, const int *, double *);
+                                       int, const double *, const int *, const double *,
+                                       const int *, const double *, const double *, const int *,
+                                       double *);
 template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, const int *, int, int, int, int,
                                       const unsigned char *, const double *, const double *, int,
-                                      const double *, const float *, const int *, const float *,
-                                      const float *, const int *, double *);
+                                      const double *, const int *, const float *, const int *,
+                                      const float *, const float *, const int *, double *);
 
 void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *D64,
                        const float *Af, const float *Bf, float *D32)
