@@ -817,50 +817,62 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     }
     if (tid < 64) sE[tid] = exp(R[tid] * jb.tau);
     __syncthreads();
-    // Y = U^T M: Y[a][k] = sum_j U[j][a] M[j][k]; thread: column k = tid%64,
-    // rows a = tid/64 + 4q
-    const int col = tid & 63, r0 = tid >> 6;
-    double Y[16];
+    // Both products on the matrix cores (16x16x4 FP64 steps, operands from
+    // LDS): wave w owns rows 16w..16w+15 of the result, lane l = 16g + r
+    // holds result rows 16w + g + 4i, columns 16b + r (accumulator b, element
+    // i); the contraction index runs over 16g + s at step s.
+    const int lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    // Y = U^T M: Y[a][k] = sum_j U[j][a] M[j][k]
+    d4 acc[4];
 #pragma unroll
-    for (int q = 0; q < 16; q++) Y[q] = 0;
-    for (int j = 0; j < 64; j++) {
-        const double mjk = sM[j * kLdsStride + col];
+    for (int b = 0; b < 4; b++) acc[b] = d4{0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 16; q++) Y[q] = fma(sU[j * kLdsStride + r0 + 4 * q], mjk, Y[q]);
+    for (int st = 0; st < 16; st++) {
+        const int jj = 16 * g + st;
+        const double ua = sU[jj * kLdsStride + 16 * w + r];
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[b] = mfma64(ua, sM[jj * kLdsStride + 16 * b + r], acc[b]);
     }
     __syncthreads();
     // Y back into sM, V into sU
 #pragma unroll
-    for (int q = 0; q < 16; q++) sM[(r0 + 4 * q) * kLdsStride + col] = Y[q];
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sM[(16 * w + g + 4 * i) * kLdsStride + 16 * b + r] = acc[b][i];
     for (int q = tid; q < 4096; q += 256) sU[(q >> 6) * kLdsStride + (q & 63)] = V[q];
     __syncthreads();
-    // A[a][b] = sum_k Y[a][k] V[b][k]; thread: column b = tid%64, rows a = tid/64 + 4q
-    double Aab[16];
+    // A[a][c] = sum_k Y[a][k] V[c][k]
 #pragma unroll
-    for (int q = 0; q < 16; q++) Aab[q] = 0;
-    for (int k = 0; k < 64; k++) {
-        const double vbk = sU[col * kLdsStride + k];
+    for (int b = 0; b < 4; b++) acc[b] = d4{0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 16; q++) Aab[q] = fma(sM[(r0 + 4 * q) * kLdsStride + k], vbk, Aab[q]);
+    for (int st = 0; st < 16; st++) {
+        const int kk = 16 * g + st;
+        const double ya = sM[(16 * w + r) * kLdsStride + kk];
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[b] = mfma64(ya, sU[(16 * b + r) * kLdsStride + kk], acc[b]);
     }
-    const double lb = R[col], eb = sE[col];
     double dsum = 0;
     double *Go = G + (size_t)ji * 4096;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int a = r0 + 4 * q;
-        const double la = R[a], ea = sE[a];
-        double x;
-        const double dl = la - lb;
-        if (fabs(dl * jb.tau) < 1e-6) {
-            // e^{lb tau} (e^{dl tau} - 1)/dl = e^{lb tau} tau (1 + dl tau/2 + ...)
-            const double u = dl * jb.tau;
-            x = eb * jb.tau * (1.0 + u * (0.5 + u / 6.0));
-        } else {
-            x = (ea - eb) / dl;
+    for (int b = 0; b < 4; b++) {
+        const int c = 16 * b + r;
+        const double lb = R[c], eb = sE[c];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int a = 16 * w + g + 4 * i;
+            const double la = R[a], ea = sE[a];
+            double x;
+            const double dl = la - lb;
+            if (fabs(dl * jb.tau) < 1e-6) {
+                // e^{lb tau} (e^{dl tau} - 1)/dl = e^{lb tau} tau (1 + dl tau/2 + ...)
+                const double u = dl * jb.tau;
+                x = eb * jb.tau * (1.0 + u * (0.5 + u / 6.0));
+            } else {
+                x = (ea - eb) / dl;
+            }
+            Go[a * 64 + c] = acc[b][i] * x;
+            if (a == c) dsum += acc[b][i] * la * ea;
         }
-        Go[a * 64 + col] = Aab[q] * x;
-        if (a == col) dsum += Aab[q] * la * ea;
     }
     // reduce the diagonal terms
     for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
