@@ -20,7 +20,10 @@
 //     less than e |x| (the reference's H_end rule, tools.c:5724-5740),
 //     confirmed once more after restarting H from the scaled identity, or
 //     when the projected gradient vanishes or no descent step can be found
-//     along the scaled steepest descent.
+//     along the scaled steepest descent, or when f has fallen by less than
+//     e |f| over the last 50 iterations (a stall: on degenerate likelihood
+//     surfaces, e.g. p0 -> 1 where w2 and p1 no longer matter, x can keep
+//     drifting along a flat direction without an H_end stop).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -64,6 +67,10 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     double gamma = 1.0;
     int it = 0, good = 0, converged = 0;
     bool confirmed = false;
+    constexpr int kStall = 50;
+    std::vector<double> fhist;   // f after each accepted step
+    fhist.reserve(max_iter + 1);
+    fhist.push_back(f);
     for (; it < max_iter; it++) {
         // working set and projected-gradient test
         double pg = 0;
@@ -177,6 +184,13 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         std::memcpy(x, xt.data(), sizeof(double) * n);
         std::swap(g, gt);
         f = ft;
+        fhist.push_back(f);
+        if ((int)fhist.size() > kStall &&
+            fhist[fhist.size() - 1 - kStall] - f < e * std::max(1.0, std::fabs(f))) {
+            converged = 1;
+            it++;
+            break;
+        }
         good = small_step ? good + 1 : 0;
         if (good >= 2) {
             if (confirmed) {
