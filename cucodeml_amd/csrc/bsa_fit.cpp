@@ -21,9 +21,10 @@
 //     confirmed once more after restarting H from the scaled identity, or
 //     when the projected gradient vanishes or no descent step can be found
 //     along the scaled steepest descent, or when f has fallen by less than
-//     e |f| over the last 50 iterations (a stall: on degenerate likelihood
-//     surfaces, e.g. p0 -> 1 where w2 and p1 no longer matter, x can keep
-//     drifting along a flat direction without an H_end stop).
+//     1e-3 e |f| over the last 50 iterations (a stall: on degenerate
+//     likelihood surfaces, e.g. p0 -> 1 where w2 and p1 no longer matter, x
+//     can keep drifting along a flat direction without an H_end stop; the
+//     1e-3 keeps slow but real progress along a ridge going).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -186,7 +187,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         f = ft;
         fhist.push_back(f);
         if ((int)fhist.size() > kStall &&
-            fhist[fhist.size() - 1 - kStall] - f < e * std::max(1.0, std::fabs(f))) {
+            fhist[fhist.size() - 1 - kStall] - f < 1e-3 * e * std::max(1.0, std::fabs(f))) {
             converged = 1;
             it++;
             break;
