@@ -786,6 +786,43 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
         }
 }
 
+// R = S1^T X S2^T for 64 x 64 matrices on the matrix cores (16x16x4 FP64
+// steps, operands from LDS).  On entry sS holds S1 and sX holds X (row
+// stride kLdsStride); S2 (row-major, 64 x 64) is read from global memory
+// into sS once S1 is no longer needed, and S1^T X is kept in sX.  Wave w
+// owns result rows 16w..16w+15: on return acc[b][i] = R[16w + g + 4i][16b + r]
+// for lane 16g + r.  The contraction index runs over 16g + s at step s.
+__device__ inline void two_products(double *sS, double *sX, const double *__restrict__ S2,
+                                    d4 acc[4])
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int st = 0; st < 16; st++) {
+        const int jj = 16 * g + st;
+        const double ua = sS[jj * kLdsStride + 16 * w + r];
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[b] = mfma64(ua, sX[jj * kLdsStride + 16 * b + r], acc[b]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sX[(16 * w + g + 4 * i) * kLdsStride + 16 * b + r] = acc[b][i];
+    for (int q = tid; q < 4096; q += 256) sS[(q >> 6) * kLdsStride + (q & 63)] = S2[q];
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = d4{0, 0, 0, 0};
+#pragma unroll
+    for (int st = 0; st < 16; st++) {
+        const int kk = 16 * g + st;
+        const double ya = sX[(16 * w + r) * kLdsStride + kk];
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[b] = mfma64(ya, sS[(16 * b + r) * kLdsStride + kk], acc[b]);
+    }
+}
+
 // Per (branch, class): M = sum of its chunk partials (fixed order), then
 // A = U^T M V^T in the eigenbasis of its set;
 // d lnL / d tau = sum_a A_aa lambda_a e^{lambda_a tau} and
@@ -817,40 +854,10 @@ __global__ __launch_bounds__(256) void k_eigbranch(const EigJob *__restrict__ jo
     }
     if (tid < 64) sE[tid] = exp(R[tid] * jb.tau);
     __syncthreads();
-    // Both products on the matrix cores (16x16x4 FP64 steps, operands from
-    // LDS): wave w owns rows 16w..16w+15 of the result, lane l = 16g + r
-    // holds result rows 16w + g + 4i, columns 16b + r (accumulator b, element
-    // i); the contraction index runs over 16g + s at step s.
+    // A = U^T M V^T; lane 16g + r of wave w holds A[16w + g + 4i][16b + r]
     const int lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
-    // Y = U^T M: Y[a][k] = sum_j U[j][a] M[j][k]
     d4 acc[4];
-#pragma unroll
-    for (int b = 0; b < 4; b++) acc[b] = d4{0, 0, 0, 0};
-#pragma unroll
-    for (int st = 0; st < 16; st++) {
-        const int jj = 16 * g + st;
-        const double ua = sU[jj * kLdsStride + 16 * w + r];
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[b] = mfma64(ua, sM[jj * kLdsStride + 16 * b + r], acc[b]);
-    }
-    __syncthreads();
-    // Y back into sM, V into sU
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) sM[(16 * w + g + 4 * i) * kLdsStride + 16 * b + r] = acc[b][i];
-    for (int q = tid; q < 4096; q += 256) sU[(q >> 6) * kLdsStride + (q & 63)] = V[q];
-    __syncthreads();
-    // A[a][c] = sum_k Y[a][k] V[c][k]
-#pragma unroll
-    for (int b = 0; b < 4; b++) acc[b] = d4{0, 0, 0, 0};
-#pragma unroll
-    for (int st = 0; st < 16; st++) {
-        const int kk = 16 * g + st;
-        const double ya = sM[(16 * w + r) * kLdsStride + kk];
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[b] = mfma64(ya, sU[(16 * b + r) * kLdsStride + kk], acc[b]);
-    }
+    two_products(sU, sM, V, acc);
     double dsum = 0;
     double *Go = G + (size_t)ji * 4096;
 #pragma unroll
@@ -916,32 +923,15 @@ __global__ __launch_bounds__(256) void k_zform(const double *__restrict__ Wpart,
         sB[(q >> 6) * kLdsStride + (q & 63)] = V[q];
     }
     __syncthreads();
-    const int col = tid & 63, r0 = tid >> 6;
-    // T1 = V^T W: T1[m][b] = sum_a V[a][m] W[a][b]
-    double T1[16];
+    // Z = V^T W U^T; lane 16g + r of wave w holds Z[16w + g + 4i][16b + r]
+    const int lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    d4 acc[4];
+    two_products(sB, sW, U, acc);
 #pragma unroll
-    for (int q = 0; q < 16; q++) T1[q] = 0;
-    for (int a = 0; a < 64; a++) {
-        const double wab = sW[a * kLdsStride + col];
+    for (int b = 0; b < 4; b++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) T1[q] = fma(sB[a * kLdsStride + r0 + 4 * q], wab, T1[q]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; q++) sW[(r0 + 4 * q) * kLdsStride + col] = T1[q];
-    for (int q = tid; q < 4096; q += 256) sB[(q >> 6) * kLdsStride + (q & 63)] = U[q];
-    __syncthreads();
-    // Z[m][n] = sum_b T1[m][b] U[n][b]
-    double Zv[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) Zv[q] = 0;
-    for (int b = 0; b < 64; b++) {
-        const double unb = sB[col * kLdsStride + b];
-#pragma unroll
-        for (int q = 0; q < 16; q++) Zv[q] = fma(sW[(r0 + 4 * q) * kLdsStride + b], unb, Zv[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++) Z[(size_t)set * 4096 + (r0 + 4 * q) * 64 + col] = Zv[q];
+        for (int i = 0; i < 4; i++)
+            Z[(size_t)set * 4096 + (16 * w + g + 4 * i) * 64 + 16 * b + r] = acc[b][i];
 }
 
 // ------------------------------------------------------------------ probe
