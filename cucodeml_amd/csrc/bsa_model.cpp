@@ -312,22 +312,38 @@ bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, con
     m->Qfactor[0] = 1 / m->mr[0];
     m->Qfactor[1] = 1 / m->mr[1];
     if (!need_eigen) return true;
-    // the three eigen systems are independent: two on helper threads
+    // the eigen systems of the distinct omegas are independent: all but the
+    // first on helper threads.  A set whose omega equals another's (w2 = 1
+    // under H0, or w0 = 1) takes that set's system.
     const double w[3] = {w0, 1.0, w2};
+    int src[3] = {0, 1, 2};
+    if (w0 == 1.0) src[0] = 1;
+    if (w2 == 1.0) src[2] = 1;
+    else if (w2 == w0) src[2] = src[0];
     bool ok[3] = {false, false, false};
     auto one = [&](int s) {
         double Q[kS * kS];
         build_q(kappa, w[s], pi, Q);
         ok[s] = eigen_rev(Q, pi, m->root[s], m->U[s], m->V[s]);
     };
-    if (threads) {
-        std::thread t1(one, 1), t2(one, 2);
-        one(0);
-        t1.join();
-        t2.join();
+    int own[3], nown = 0;
+    for (int s = 0; s < 3; s++)
+        if (src[s] == s) own[nown++] = s;
+    if (threads && nown > 1) {
+        std::thread helpers[2];
+        for (int k = 1; k < nown; k++) helpers[k - 1] = std::thread(one, own[k]);
+        one(own[0]);
+        for (int k = 1; k < nown; k++) helpers[k - 1].join();
     } else {
-        for (int s = 0; s < 3; s++) one(s);
+        for (int k = 0; k < nown; k++) one(own[k]);
     }
+    for (int s = 0; s < 3; s++)
+        if (src[s] != s) {
+            std::memcpy(m->root[s], m->root[src[s]], sizeof m->root[s]);
+            std::memcpy(m->U[s], m->U[src[s]], sizeof m->U[s]);
+            std::memcpy(m->V[s], m->V[src[s]], sizeof m->V[s]);
+            ok[s] = ok[src[s]];
+        }
     return ok[0] && ok[1] && ok[2];
 }
 

@@ -144,3 +144,23 @@ def test_host_model_setup_matches_oracle():
     QQ = U[0] @ np.diag(R[0]) @ V[0]
     assert np.max(np.abs((QQ - Q)[np.ix_(keep, keep)])) < 1e-12
     assert U[0][5, 5] == 1 and V[0][30, 30] == 1 and R[0][5] == 0
+
+
+@pytest.mark.parametrize("w0,w2,same", [(0.2, 1.0, {2: 1}), (1.0, 3.0, {0: 1}),
+                                        (1.0, 1.0, {0: 1, 2: 1}), (0.4, 0.4, {2: 0})])
+def test_coinciding_omegas_share_one_eigen_system(w0, w2, same):
+    """w2 = 1 (H0) or w0 = 1 or w0 = w2: the set takes the other's system,
+    identical to computing it on its own (the same omega in a separate call)."""
+    rng = np.random.default_rng(3)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    _, _, R, U, V = engine.model_a_setup(2.2, 0.5, 0.3, w0, w2, pi)
+    for s, src in same.items():
+        assert np.array_equal(R[s], R[src]) and np.array_equal(U[s], U[src])
+        assert np.array_equal(V[s], V[src])
+    # each set equals the system of its omega computed where no sharing applies
+    w = (w0, 1.0, w2)
+    for s in range(3):
+        a, b = (w[s], 0.123) if s == 0 else (0.123, w[s])
+        _, _, Ri, Ui, Vi = engine.model_a_setup(2.2, 0.5, 0.3, a, b, pi)
+        k = 0 if s == 0 else (1 if w[s] == 1.0 else 2)
+        assert np.array_equal(R[s], Ri[k]) and np.array_equal(U[s], Ui[k])
