@@ -68,10 +68,11 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     double gamma = 1.0;
     int it = 0, good = 0, converged = 0;
     bool confirmed = false;
+    // f after the last kStall + 1 accepted steps (ring buffer)
     constexpr int kStall = 50;
-    std::vector<double> fhist;   // f after each accepted step
-    fhist.reserve(max_iter + 1);
-    fhist.push_back(f);
+    std::vector<double> fring(kStall + 1);
+    long nacc = 0;
+    fring[0] = f;
     for (; it < max_iter; it++) {
         // working set and projected-gradient test
         double pg = 0;
@@ -185,9 +186,10 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         std::memcpy(x, xt.data(), sizeof(double) * n);
         std::swap(g, gt);
         f = ft;
-        fhist.push_back(f);
-        if ((int)fhist.size() > kStall &&
-            fhist[fhist.size() - 1 - kStall] - f < 1e-3 * e * std::max(1.0, std::fabs(f))) {
+        nacc++;
+        fring[nacc % (kStall + 1)] = f;
+        if (nacc >= kStall &&
+            fring[(nacc - kStall) % (kStall + 1)] - f < 1e-3 * e * std::max(1.0, std::fabs(f))) {
             converged = 1;
             it++;
             break;
