@@ -311,7 +311,9 @@ def main():
     nev = tm["n_eval"]
     sto = "double" if args.precision == "fp64" else "float"
     prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
-                        f"k_prune_{args.kernel}<{sto}> (FP64 arithmetic, {sto} storage)")
+                        (f"k_prune_res/k_prune_mfma<{sto}> (pruning levels, FP64 arithmetic, "
+                         f"{sto} storage)") if args.kernel == "mfma" else
+                        f"k_prune_valu<{sto}> (FP64 arithmetic, {sto} storage)")
     rls = [(tm["sum_prune"], prune_rl)]
     adj_rl = br_rl = None
     if args.gradient == "adjoint":

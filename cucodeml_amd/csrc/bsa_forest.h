@@ -103,6 +103,7 @@ struct Forest {
     std::vector<PmatJob> pjobs;
     std::vector<PruneJob> prune;
     std::vector<int> level_off;
+    std::vector<int> level_maxint;            // most interior children of a job, per level
 
     // device
     hipStream_t stream = nullptr;

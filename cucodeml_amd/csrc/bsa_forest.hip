@@ -474,6 +474,7 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
     }
     f.prune.clear();
     f.level_off.assign(1, 0);
+    f.level_maxint.clear();
     for (Gene *gp : genes) gp->prune_rng.assign(nlev, {0, 0});
     for (size_t l = 0; l < nlev; l++) {
         for (Gene *gp : genes) {
@@ -493,6 +494,13 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
                 f.prune.push_back(j);
             }
         }
+        int maxint = 0;
+        for (int i = f.level_off.back(); i < (int)f.prune.size(); i++) {
+            int n = 0;
+            for (int k = 0; k < f.prune[i].nch; k++) n += f.prune[i].ch_tip[k] < 0;
+            maxint = std::max(maxint, n);
+        }
+        f.level_maxint.push_back(maxint);
         f.level_off.push_back((int)f.prune.size());
     }
     for (double &v : f.work) v = 0;
@@ -861,7 +869,8 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
     for (size_t l = 0; l + 1 < f.level_off.size(); l++) {
         const int a = f.level_off[l], b = f.level_off[l + 1];
         launch_prune<T>(s, f.kernel, f.tile, rs ? f.d_prune : f.d_prune + a,
-                        rs ? rs->lev[l] : nullptr, rs ? rs->nlev[l] : b - a, f.npad, f.d_z,
+                        rs ? rs->lev[l] : nullptr, rs ? rs->nlev[l] : b - a, f.level_maxint[l],
+                        f.npad, f.d_z,
                         (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes, (T *)f.d_L,
                         f.d_E, adjoint ? (T *)f.d_C : nullptr);
     }

@@ -122,11 +122,12 @@ void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, const in
                  double *PT, const unsigned char *mask_amb, int ncodes, double *TT);
 
 // kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64)
-// Cb: contribution buffer (interior children's P I per class) or nullptr
+// Cb: contribution buffer (interior children's P I per class) or nullptr;
+// maxint: the most interior children of any job in the launch
 template <typename T>
 void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, const int *sel,
-                  int njobs, int npad, const unsigned char *z, const double *PT, const double *TT,
-                  int ncodes, T *L, int *E, T *Cb);
+                  int njobs, int maxint, int npad, const unsigned char *z, const double *PT,
+                  const double *TT, int ncodes, T *L, int *E, T *Cb);
 
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with
 // seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA).  args:

@@ -2,6 +2,7 @@
 //   k_pmat        batched P(t) - I = U diag(exp(root t)) V - I, every (branch, eigen set)
 //   k_tiptab      per-tip rows of every code (sense codons and ambiguity codes)
 //   k_prune_mfma  one tree level of Felsenstein pruning on the FP64 matrix cores
+//   k_prune_res   the same with the children's P resident in LDS (big levels)
 //   k_prune_valu  the same on plain vector FMA
 //   k_root        per-pattern class log-likelihoods, mixture and weighted sum
 //   k_adj_a       adjoints of one depth's interior children (preorder sweep)
@@ -321,6 +322,128 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
                 rb = xorshift32(rb);
             }
             if (r == 0) E[(size_t)jb.out_slot * npad + h] = esum[q][i] + e;
+        }
+    }
+}
+
+// The same job with every interior child's P resident in LDS, for jobs with
+// at most two interior children: a block of 16 waves takes one (node, class)
+// job and tpb tiles of 256 patterns.  The P tiles are staged once; then each
+// wave runs its 16-pattern rows of every tile with no further barrier, so
+// the waves overlap one another's loads and products freely.  Per pattern
+// the arithmetic is that of k_prune_mfma, so results are the same bitwise.
+constexpr int kResWaves = 16;
+
+template <typename T>
+__global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_res(const PruneJob *__restrict__ jobs,
+                                                                const int *__restrict__ sel,
+                                                                int npad,
+                                                                const unsigned char *__restrict__ z,
+                                                                const double *__restrict__ PT,
+                                                                const double *__restrict__ TT,
+                                                                int ncodes, T *__restrict__ L,
+                                                                int *__restrict__ E,
+                                                                T *__restrict__ Cb, int tpb)
+{
+    __shared__ double sP[2][64 * 64];
+    const PruneJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    const int nch = jb.nch;
+    {
+        int k = 0;
+        for (int c = 0; c < nch && k < 2; c++) {
+            if (jb.ch_tip[c] >= 0) continue;
+            const double *P = PT + (size_t)jb.ch_pmat[c] * 4096;
+            for (int q = tid; q < 4096; q += 64 * kResWaves)
+                sP[k][q] = P[q] + ((q >> 6) == (q & 63) ? 1.0 : 0.0);
+            k++;
+        }
+    }
+    __syncthreads();
+    T *Lo = L + (size_t)jb.out_slot * npad * 64;
+    for (int t = 0; t < tpb; t++) {
+        const int hw = (blockIdx.x * tpb + t) * (16 * kResWaves) + 16 * w;
+        if (hw >= npad) break;   // the same for the whole wave
+        const int hb = hw + g;
+        d4 acc[4];
+        int esum[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            acc[b] = d4{1, 1, 1, 1};
+            esum[b] = 0;
+        }
+        int k = 0;
+        for (int c = 0; c < nch; c++) {
+            const int pm = jb.ch_pmat[c];
+            const int tip = jb.ch_tip[c];
+            if (tip >= 0) {
+                const unsigned char *zt = z + (size_t)tip * npad;
+                int code[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) code[i] = zt[hb + 4 * i];
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        acc[b][i] *= tip_value(TT, pm, ncodes, code[i], 16 * b + r);
+            } else {
+                const int slot = jb.ch_slot[c];
+                const T *Lc = L + (size_t)slot * npad * 64;
+                const int *Ec = E + (size_t)slot * npad;
+                double a[16];
+                Store<T>::load16(Lc + (size_t)(hw + r) * 64 + 16 * g, a);
+                d4 tmp[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) tmp[b] = d4{0, 0, 0, 0};
+                const double *Pk = sP[k];
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const double *prow = Pk + (16 * g + kk) * 64 + r;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) tmp[b] = mfma64(a[kk], prow[16 * b], tmp[b]);
+                }
+                if (Cb) {
+                    const int cs = jb.ch_cslot[c];
+                    T *Co = Cb + (size_t)cs * npad * 64;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        uint32_t rb = row_seed((uint32_t)cs ^ 0x3C6EF372u, (uint32_t)(hb + 4 * i),
+                                               (uint32_t)r);
+#pragma unroll
+                        for (int b = 0; b < 4; b++) {
+                            Store<T>::put(Co + (size_t)(hb + 4 * i) * 64 + 16 * b + r,
+                                          tmp[b][i], rb);
+                            rb = xorshift32(rb);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[b] *= tmp[b];
+#pragma unroll
+                for (int i = 0; i < 4; i++) esum[i] += Ec[hb + 4 * i];
+                k++;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double m = acc[0][i];
+#pragma unroll
+            for (int b = 1; b < 4; b++) m = acc[b][i] > m ? acc[b][i] : m;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                double o = __shfl_xor(m, off, 64);
+                m = o > m ? o : m;
+            }
+            int e = 0;
+            if (m > 0) frexp(m, &e);
+            const int h = hb + 4 * i;
+            uint32_t rb = row_seed((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)r);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                Store<T>::put(Lo + (size_t)h * 64 + 16 * b + r, ldexp(acc[b][i], -e), rb);
+                rb = xorshift32(rb);
+            }
+            if (r == 0) E[(size_t)jb.out_slot * npad + h] = esum[i] + e;
         }
     }
 }
@@ -966,10 +1089,19 @@ void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, const in
 
 template <typename T>
 void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, const int *sel,
-                  int njobs, int npad, const unsigned char *z, const double *PT, const double *TT,
-                  int ncodes, T *L, int *E, T *Cb)
+                  int njobs, int maxint, int npad, const unsigned char *z, const double *PT,
+                  const double *TT, int ncodes, T *L, int *E, T *Cb)
 {
     if (njobs <= 0) return;
+    // P resident in LDS where every job has at most two interior children
+    // and the launch has at least half as many blocks of 16 waves as CUs
+    // (below that the 4-wave blocks spread the work better)
+    const int nres = (npad + 16 * kResWaves - 1) / (16 * kResWaves);
+    if (kernel == 0 && tile == 64 && maxint <= 2 && (long)nres * njobs >= 128) {
+        hipLaunchKernelGGL(k_prune_res<T>, dim3(nres, njobs), dim3(64 * kResWaves), 0, s, jobs,
+                           sel, npad, z, PT, TT, ncodes, L, E, Cb, 1);
+        return;
+    }
     if (kernel == 0 && tile == 128)
         hipLaunchKernelGGL((k_prune_mfma<T, 2>), dim3(npad / 128, njobs), dim3(256), 0, s, jobs,
                            sel, npad, z, PT, TT, ncodes, L, E, Cb);
@@ -1055,10 +1187,10 @@ void launch_mfma_probe(hipStream_t s, const double *A, const double *B, double *
 }
 
 template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, const int *, int, int,
-                                   const unsigned char *, const double *, const double *, int,
+                                   int, const unsigned char *, const double *, const double *, int,
                                    double *, int *, double *);
 template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, const int *, int, int,
-                                  const unsigned char *, const double *, const double *, int,
+                                  int, const unsigned char *, const double *, const double *, int,
                                   float *, int *, float *);
 template void launch_root<double>(hipStream_t, const RootArgs *, const int *, int, const double *,
                                   const int *, const double *, const double *, const double *,
