@@ -4,10 +4,18 @@
 // likelihood evaluation and runs on the host in double precision.
 #include "bsa_model.h"
 
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <utility>
+
+#include <immintrin.h>
+#include <unistd.h>
 
 namespace bsa {
 
@@ -92,96 +100,132 @@ double mean_rate(double kappa, double omega, const double *pi)
     return mr;
 }
 
-// Householder reduction of the symmetric matrix in z (n x n, row-major) to
-// tridiagonal form (diagonal d, sub-diagonal e), accumulating the orthogonal
-// transform in z; then the implicit-shift QL iteration on the tridiagonal
-// matrix, rotating z into the eigenvectors (columns).  This is the classic
-// Householder + QL pair of algorithms (Wilkinson & Reinsch, Handbook for
-// Automatic Computation vol. II, procedures tred2 and tql2).
+// Symmetric eigen decomposition of the n x n (n <= 64) row-major matrix in
+// z: eigenvalues in d (descending), eigenvectors in the columns of z.
+//
+// 1. Householder tridiagonalisation (Golub & Van Loan, Matrix Computations,
+//    Algorithm 8.3.1) on full symmetric storage: step k reflects row k's
+//    entries right of the sub-diagonal to alpha e_1 and applies the
+//    reflector to the trailing block as a rank-2 update.  Every inner loop is
+//    a row AXPY (the symmetric product B v is a sum of rows of B), so all of
+//    them run on the vector units; the reflectors are kept in the rows they
+//    zeroed.
+// 2. Q = H_0 H_1 ... H_{n-3}, accumulated backwards so that step k only
+//    touches the trailing block (left products with H_k).
+// 3. QL iteration with implicit shifts on the tridiagonal matrix (Wilkinson
+//    & Reinsch, Handbook for Automatic Computation vol. II, procedure tql2),
+//    accumulating the rotations into Q.  A sweep's rotations depend only on
+//    d and e, so they are applied to the vectors while the next sweep's are
+//    computed.
+typedef double v4d __attribute__((vector_size(32)));
+
 static bool sym_eigen(int n, double *z, double *d, double *e)
 {
-    // The reduction reads and updates columns of the (symmetric) input and
-    // of the accumulated transform; it runs on the transposed view so that
-    // those are contiguous rows.  Afterwards row v of the buffer holds
-    // column v of the transform, which is the layout the QL rotations want.
-#define Z(i, j) z[(j) * n + (i)]
-    for (int j = 0; j < n; j++) d[j] = Z(n - 1, j);
-    for (int i = n - 1; i > 0; i--) {
-        double scale = 0, h = 0;
-        for (int k = 0; k < i; k++) scale += std::fabs(d[k]);
-        if (scale == 0) {
-            e[i] = d[i - 1];
-            for (int j = 0; j < i; j++) {
-                d[j] = Z(i - 1, j);
-                Z(i, j) = 0;
-                Z(j, i) = 0;
-            }
-        } else {
-            for (int k = 0; k < i; k++) {
-                d[k] /= scale;
-                h += d[k] * d[k];
-            }
-            double f = d[i - 1];
-            double g = std::sqrt(h);
-            if (f > 0) g = -g;
-            e[i] = scale * g;
-            h -= f * g;
-            d[i - 1] = f - g;
-            for (int j = 0; j < i; j++) e[j] = 0;
-            for (int j = 0; j < i; j++) {
-                f = d[j];
-                Z(j, i) = f;
-                g = e[j] + Z(j, j) * f;
-                for (int k = j + 1; k <= i - 1; k++) {
-                    g += Z(k, j) * d[k];
-                    e[k] += Z(k, j) * f;
-                }
-                e[j] = g;
-            }
-            f = 0;
-            for (int j = 0; j < i; j++) {
-                e[j] /= h;
-                f += e[j] * d[j];
-            }
-            double hh = f / (h + h);
-            for (int j = 0; j < i; j++) e[j] -= hh * d[j];
-            for (int j = 0; j < i; j++) {
-                f = d[j];
-                g = e[j];
-                for (int k = j; k <= i - 1; k++) Z(k, j) -= (f * e[k] + g * d[k]);
-                d[j] = Z(i - 1, j);
-                Z(i, j) = 0;
-            }
-        }
-        d[i] = h;
+    constexpr int ld = 64;
+    if (n < 1 || n > ld) return false;
+    if (n == 1) {
+        d[0] = z[0];
+        z[0] = 1;
+        return std::isfinite(d[0]);
     }
-    for (int i = 0; i < n - 1; i++) {
-        Z(n - 1, i) = Z(i, i);
-        Z(i, i) = 1;
-        double h = d[i + 1];
-        if (h != 0) {
-            for (int k = 0; k <= i; k++) d[k] = Z(k, i + 1) / h;
-            for (int j = 0; j <= i; j++) {
-                double g = 0;
-                for (int k = 0; k <= i; k++) g += Z(k, i + 1) * Z(k, j);
-                for (int k = 0; k <= i; k++) Z(k, j) -= g * d[k];
-            }
-        }
-        for (int k = 0; k <= i; k++) Z(k, i + 1) = 0;
-    }
-    for (int j = 0; j < n; j++) {
-        d[j] = Z(n - 1, j);
-        Z(n - 1, j) = 0;
-    }
-    Z(n - 1, n - 1) = 1;
-    e[0] = 0;
-#undef Z
-    // ZT(v, k) = component k of vector v
-#define ZT(v, k) z[(v) * n + (k)]
+    alignas(64) double A[ld * ld];
+    alignas(64) double Z[ld * ld];
+    alignas(64) double p[ld];
+    double beta[ld];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) A[i * ld + j] = z[i * n + j];
 
-    // QL with implicit shifts
-    for (int i = 1; i < n; i++) e[i - 1] = e[i];
+    // 1. tridiagonalisation; e[k] couples k and k + 1
+    for (int k = 0; k + 2 < n; k++) {
+        double *x = A + k * ld + k + 1;   // row k right of the diagonal (= column k below it)
+        const int L = n - k - 1;
+        d[k] = A[k * ld + k];
+        double ss = 0;
+        for (int i = 1; i < L; i++) ss += x[i] * x[i];
+        if (ss == 0) {   // nothing to annihilate
+            e[k] = x[0];
+            beta[k] = 0;
+            continue;
+        }
+        const double x0 = x[0];
+        double alpha = std::sqrt(x0 * x0 + ss);
+        if (x0 > 0) alpha = -alpha;
+        const double v0 = x0 - alpha;      // x0 and -alpha have the same sign: no cancellation
+        const double b = -1.0 / (alpha * v0);   // 2 / v^T v
+        x[0] = v0;                         // x is now the reflector v
+        beta[k] = b;
+        e[k] = alpha;
+        double *B = A + (k + 1) * ld + (k + 1);
+        for (int i = 0; i < L; i++) p[i] = 0;
+        for (int j = 0; j < L; j++) {   // p = b B v = b sum_j v_j B[j][.]
+            const double f = b * x[j];
+            const double *Bj = B + j * ld;
+            for (int i = 0; i < L; i++) p[i] += f * Bj[i];
+        }
+        double pv = 0;
+        for (int i = 0; i < L; i++) pv += p[i] * x[i];
+        const double K = 0.5 * b * pv;
+        for (int i = 0; i < L; i++) p[i] -= K * x[i];   // w = p - K v
+        for (int i = 0; i < L; i++) {   // B -= v w^T + w v^T
+            const double vi = x[i], wi = p[i];
+            double *Bi = B + i * ld;
+            for (int j = 0; j < L; j++) Bi[j] -= vi * p[j] + wi * x[j];
+        }
+    }
+    d[n - 2] = A[(n - 2) * ld + n - 2];
+    e[n - 2] = A[(n - 2) * ld + n - 1];
+    d[n - 1] = A[(n - 1) * ld + n - 1];
     e[n - 1] = 0;
+
+    // 2. Q, then its transpose in A (row v = column v of Q, zero-padded to a
+    // multiple of 4 components for the blocked rotations)
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Z[i * ld + j] = i == j ? 1.0 : 0.0;
+    for (int k = n - 3; k >= 0; k--) {
+        if (beta[k] == 0) continue;
+        const double *v = A + k * ld + k + 1;
+        const int L = n - k - 1;
+        double *S = Z + (k + 1) * ld + (k + 1);
+        for (int j = 0; j < L; j++) p[j] = 0;
+        for (int i = 0; i < L; i++) {   // p = v^T S
+            const double vi = v[i];
+            const double *Si = S + i * ld;
+            for (int j = 0; j < L; j++) p[j] += vi * Si[j];
+        }
+        for (int i = 0; i < L; i++) {   // S -= b v p^T
+            const double f = beta[k] * v[i];
+            double *Si = S + i * ld;
+            for (int j = 0; j < L; j++) Si[j] -= f * p[j];
+        }
+    }
+    const int nv = (n + 3) & ~3;
+    for (int v = 0; v < n; v++) {
+        for (int k = 0; k < n; k++) A[v * ld + k] = Z[k * ld + v];
+        for (int k = n; k < nv; k++) A[v * ld + k] = 0;
+    }
+#define ZT(v, k) A[(v) * ld + (k)]
+
+    // 3. QL with implicit shifts.  The rotations of one sweep are applied to
+    // the vectors while the next sweep's rotations are computed: that chain
+    // is bound by the latency of its square roots and divisions, the vector
+    // updates by throughput, so the two overlap.  Rotations still reach the
+    // vectors in the order the sweeps make them.
+    double cs[2][ld][2];
+    int cur = 0;           // the sweep being computed writes cs[cur]
+    int pj = -1, pl = 0;   // rotations pj, pj - 1, .., pl of cs[cur ^ 1] still to apply
+    auto apply_one = [&]() {
+        const double c = cs[cur ^ 1][pj][0], s = cs[cur ^ 1][pj][1];
+        double *zi = &ZT(pj, 0), *zi1 = &ZT(pj + 1, 0);
+        for (int k = 0; k < nv; k += 4) {
+            v4d x, y;
+            std::memcpy(&x, zi + k, sizeof(v4d));
+            std::memcpy(&y, zi1 + k, sizeof(v4d));
+            const v4d up = s * x + c * y, lo = c * x - s * y;
+            std::memcpy(zi1 + k, &up, sizeof(v4d));
+            std::memcpy(zi + k, &lo, sizeof(v4d));
+        }
+        pj--;
+    };
     double f = 0, tst1 = 0;
     const double eps = std::ldexp(1.0, -52);
     for (int l = 0; l < n; l++) {
@@ -196,63 +240,67 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
             do {
                 if (++iter > 60) return false;
                 double g = d[l];
-                double p = (d[l + 1] - g) / (2 * e[l]);
-                double r = std::hypot(p, 1.0);
-                if (p < 0) r = -r;
-                d[l] = e[l] / (p + r);
-                d[l + 1] = e[l] * (p + r);
+                double q = (d[l + 1] - g) / (2 * e[l]);
+                double r = std::hypot(q, 1.0);
+                if (q < 0) r = -r;
+                d[l] = e[l] / (q + r);
+                d[l + 1] = e[l] * (q + r);
                 double dl1 = d[l + 1];
                 double h = g - d[l];
                 for (int i = l + 2; i < n; i++) d[i] -= h;
                 f += h;
-                p = d[m];
+                q = d[m];
                 double c = 1, c2 = 1, c3 = 1, el1 = e[l + 1], s = 0, s2 = 0;
                 for (int i = m - 1; i >= l; i--) {
                     c3 = c2;
                     c2 = c;
                     s2 = s;
                     g = c * e[i];
-                    h = c * p;
-                    // p and e[i] are of the order of the matrix norm here, so
+                    h = c * q;
+                    // q and e[i] are of the order of the matrix norm here, so
                     // the plain square root cannot overflow (hypot is ~3x slower)
-                    r = std::sqrt(p * p + e[i] * e[i]);
+                    r = std::sqrt(q * q + e[i] * e[i]);
                     e[i + 1] = s * r;
                     s = e[i] / r;
-                    c = p / r;
-                    p = c * d[i] - s * g;
+                    c = q / r;
+                    q = c * d[i] - s * g;
                     d[i + 1] = h + s * (c * g + s * d[i]);
-                    double *zi = &ZT(i, 0), *zi1 = &ZT(i + 1, 0);
-                    for (int k = 0; k < n; k++) {
-                        const double a = zi[k], b = zi1[k];
-                        zi1[k] = s * a + c * b;
-                        zi[k] = c * a - s * b;
-                    }
+                    cs[cur][i][0] = c;
+                    cs[cur][i][1] = s;
+                    if (pj >= pl) apply_one();
                 }
-                p = -s * s2 * c3 * el1 * e[l] / dl1;
-                e[l] = s * p;
-                d[l] = c * p;
+                while (pj >= pl) apply_one();   // the rest of the previous sweep
+                pj = m - 1;                     // this sweep's rotations are next
+                pl = l;
+                cur ^= 1;
+                q = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * q;
+                d[l] = c * q;
             } while (std::fabs(e[l]) > eps * tst1);
         }
         d[l] = d[l] + f;
         e[l] = 0;
     }
-    // sort descending (eigenvalues and vector columns)
+    while (pj >= pl) apply_one();
+    // sort descending (eigenvalues and vectors)
     for (int i = 0; i < n - 1; i++) {
         int k = i;
-        double p = d[i];
+        double q = d[i];
         for (int j = i + 1; j < n; j++)
-            if (d[j] > p) { k = j; p = d[j]; }
+            if (d[j] > q) { k = j; q = d[j]; }
         if (k != i) {
             d[k] = d[i];
-            d[i] = p;
+            d[i] = q;
             for (int j = 0; j < n; j++) std::swap(ZT(i, j), ZT(k, j));
         }
     }
-    // back to eigenvectors in columns
-    for (int i = 0; i < n; i++)
-        for (int j = i + 1; j < n; j++) std::swap(ZT(i, j), ZT(j, i));
-    return true;
+    // eigenvectors back in the columns of z
+    for (int v = 0; v < n; v++)
+        for (int k = 0; k < n; k++) z[k * n + v] = ZT(v, k);
 #undef ZT
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(d[i])) return false;
+    return true;
 }
 
 // Q = U diag(root) V with UV = I, via the symmetric S = Pi^1/2 Q Pi^-1/2;
@@ -289,6 +337,84 @@ bool eigen_rev(const double *Q, const double *pi, double *root, double *U, doubl
         if (!std::isfinite(root[i])) return false;
     return true;
 }
+
+// Two helper threads for the eigen systems of one evaluation, shared by the
+// process.  After a task a helper spins for kSpin waiting for the next one
+// (an optimiser asks every few milliseconds, and handing a task to a
+// spinning thread takes about a microsecond, where starting a thread takes
+// tens), then sleeps on a condition variable until it is woken.  One caller
+// at a time: try_run returns false at once while another caller holds them.
+// The object is never destroyed, so helpers parked at process exit are
+// harmless.  A child made by fork() has no helper threads, so there
+// try_run always declines.
+class EigenHelpers {
+public:
+    static EigenHelpers &get()
+    {
+        static EigenHelpers *h = new EigenHelpers();
+        return *h;
+    }
+    // fn(0) on the caller, fn(1) .. fn(n - 1) (n <= 3) on the helpers
+    bool try_run(int n, const std::function<void(int)> &fn)
+    {
+        if (n < 1 || n > 3 || getpid() != pid_ || busy_.exchange(true, std::memory_order_acquire))
+            return false;
+        for (int k = 1; k < n; k++) {
+            Helper &h = helper_[k - 1];
+            h.fn = &fn;
+            h.arg = k;
+            {
+                std::lock_guard<std::mutex> lk(h.mu);
+                h.posted.store(h.posted.load(std::memory_order_relaxed) + 1,
+                               std::memory_order_release);
+            }
+            h.cv.notify_one();
+        }
+        fn(0);
+        for (int k = 1; k < n; k++) {
+            Helper &h = helper_[k - 1];
+            const unsigned long want = h.posted.load(std::memory_order_relaxed);
+            while (h.done.load(std::memory_order_acquire) != want) _mm_pause();
+        }
+        busy_.store(false, std::memory_order_release);
+        return true;
+    }
+
+private:
+    static constexpr std::chrono::microseconds kSpin{10000};
+    struct Helper {
+        std::atomic<unsigned long> posted{0}, done{0};
+        const std::function<void(int)> *fn = nullptr;
+        int arg = 0;
+        std::mutex mu;
+        std::condition_variable cv;
+    };
+    EigenHelpers() : pid_(getpid())
+    {
+        for (Helper &h : helper_) std::thread([&h] { serve(h); }).detach();
+    }
+    static void serve(Helper &h)
+    {
+        unsigned long seen = 0;
+        for (;;) {
+            const auto t0 = std::chrono::steady_clock::now();
+            int polls = 0;
+            while (h.posted.load(std::memory_order_acquire) == seen) {
+                _mm_pause();
+                if (++polls % 256 == 0 && std::chrono::steady_clock::now() - t0 > kSpin) {
+                    std::unique_lock<std::mutex> lk(h.mu);
+                    h.cv.wait(lk, [&] { return h.posted.load(std::memory_order_acquire) != seen; });
+                }
+            }
+            seen = h.posted.load(std::memory_order_acquire);
+            (*h.fn)(h.arg);
+            h.done.store(seen, std::memory_order_release);
+        }
+    }
+    const pid_t pid_;
+    std::atomic<bool> busy_{false};
+    Helper helper_[2];
+};
 
 // Model A at natural parameters (SetParametersNSsites, codeml.c:2470-2520):
 // proportions (p0, p1, (1-p0-p1)p0/(p0+p1), (1-p0-p1)p1/(p0+p1)); Qfactor of
@@ -330,10 +456,14 @@ bool setup_model_a(double kappa, double p0, double p1, double w0, double w2, con
     for (int s = 0; s < 3; s++)
         if (src[s] == s) own[nown++] = s;
     if (threads && nown > 1) {
-        std::thread helpers[2];
-        for (int k = 1; k < nown; k++) helpers[k - 1] = std::thread(one, own[k]);
-        one(own[0]);
-        for (int k = 1; k < nown; k++) helpers[k - 1].join();
+        const std::function<void(int)> task = [&](int k) { one(own[k]); };
+        if (!EigenHelpers::get().try_run(nown, task)) {
+            // the helpers serve another engine right now: threads of our own
+            std::thread helpers[2];
+            for (int k = 1; k < nown; k++) helpers[k - 1] = std::thread(one, own[k]);
+            one(own[0]);
+            for (int k = 1; k < nown; k++) helpers[k - 1].join();
+        }
     } else {
         for (int k = 0; k < nown; k++) one(own[k]);
     }

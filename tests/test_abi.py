@@ -146,6 +146,35 @@ def test_host_model_setup_matches_oracle():
     assert U[0][5, 5] == 1 and V[0][30, 30] == 1 and R[0][5] == 0
 
 
+@pytest.mark.parametrize("kind", ["uniform", "two", "three", "five"])
+def test_host_eigen_degenerate_and_small_systems(kind):
+    """Equal codon frequencies give repeated eigenvalues (the genetic code's
+    symmetries), and frequencies on only 2, 3 or 5 codons leave tiny
+    systems: the reconstruction Q = U diag(R) V, U V = I and the eigenvalues
+    against numpy's symmetric solver hold there too."""
+    from oracle import oracle
+    if kind == "uniform":
+        pi = np.full(61, 1 / 61)
+    else:
+        pi = np.zeros(61)
+        k = {"two": [0, 1], "three": [3, 7, 19], "five": [2, 11, 29, 40, 60]}[kind]
+        pi[k] = np.arange(1, len(k) + 1)
+        pi /= pi.sum()
+    keep = pi > 0
+    for w in (0.05, 1.0, 4.0):
+        _, _, R, U, V = engine.model_a_setup(2.5, 0.5, 0.3, w, w, pi)
+        Q, _ = oracle.q_matrix(2.5, w, pi)
+        QQ = U[0] @ np.diag(R[0]) @ V[0]
+        assert np.max(np.abs((QQ - Q)[np.ix_(keep, keep)])) < 1e-12
+        assert np.max(np.abs(U[0] @ V[0] - np.eye(61))) < 1e-12
+        sq = np.sqrt(pi[keep])
+        S = Q[np.ix_(keep, keep)] * sq[:, None] / sq[None, :]
+        ev = np.linalg.eigvalsh((S + S.T) / 2)
+        assert np.max(np.abs(np.sort(R[0][keep]) - ev)) < 1e-13
+        if kind == "uniform":   # eigenvalues in descending order
+            assert np.all(np.diff(R[0]) <= 0)
+
+
 @pytest.mark.parametrize("w0,w2,same", [(0.2, 1.0, {2: 1}), (1.0, 3.0, {0: 1}),
                                         (1.0, 1.0, {0: 1, 2: 1}), (0.4, 0.4, {2: 0})])
 def test_coinciding_omegas_share_one_eigen_system(w0, w2, same):
