@@ -100,129 +100,112 @@ double mean_rate(double kappa, double omega, const double *pi)
     return mr;
 }
 
-// Symmetric eigen decomposition of the n x n (n <= 64) row-major matrix in
-// z: eigenvalues in d (descending), eigenvectors in the columns of z.
-//
-// 1. Householder tridiagonalisation (Golub & Van Loan, Matrix Computations,
-//    Algorithm 8.3.1) on full symmetric storage: step k reflects row k's
-//    entries right of the sub-diagonal to alpha e_1 and applies the
-//    reflector to the trailing block as a rank-2 update.  Every inner loop is
-//    a row AXPY (the symmetric product B v is a sum of rows of B), so all of
-//    them run on the vector units; the reflectors are kept in the rows they
-//    zeroed.
-// 2. Q = H_0 H_1 ... H_{n-3}, accumulated backwards so that step k only
-//    touches the trailing block (left products with H_k).
-// 3. QL iteration with implicit shifts on the tridiagonal matrix (Wilkinson
-//    & Reinsch, Handbook for Automatic Computation vol. II, procedure tql2),
-//    accumulating the rotations into Q.  A sweep's rotations depend only on
-//    d and e, so they are applied to the vectors while the next sweep's are
-//    computed.
-typedef double v4d __attribute__((vector_size(32)));
-
+// Householder reduction of the symmetric matrix in z (n x n, row-major) to
+// tridiagonal form (diagonal d, sub-diagonal e), accumulating the orthogonal
+// transform in z; then the implicit-shift QL iteration on the tridiagonal
+// matrix, rotating z into the eigenvectors (columns).  This is the classic
+// Householder + QL pair of algorithms (Wilkinson & Reinsch, Handbook for
+// Automatic Computation vol. II, procedures tred2 and tql2).  A QL sweep's
+// rotations depend only on d and e, so they are applied to the vectors while
+// the next sweep's are computed.
 static bool sym_eigen(int n, double *z, double *d, double *e)
 {
-    constexpr int ld = 64;
-    if (n < 1 || n > ld) return false;
-    if (n == 1) {
-        d[0] = z[0];
-        z[0] = 1;
-        return std::isfinite(d[0]);
+    // The reduction reads and updates columns of the (symmetric) input and
+    // of the accumulated transform; it runs on the transposed view so that
+    // those are contiguous rows.  Afterwards row v of the buffer holds
+    // column v of the transform, which is the layout the QL rotations want.
+#define Z(i, j) z[(j) * n + (i)]
+    for (int j = 0; j < n; j++) d[j] = Z(n - 1, j);
+    for (int i = n - 1; i > 0; i--) {
+        double scale = 0, h = 0;
+        for (int k = 0; k < i; k++) scale += std::fabs(d[k]);
+        if (scale == 0) {
+            e[i] = d[i - 1];
+            for (int j = 0; j < i; j++) {
+                d[j] = Z(i - 1, j);
+                Z(i, j) = 0;
+                Z(j, i) = 0;
+            }
+        } else {
+            for (int k = 0; k < i; k++) {
+                d[k] /= scale;
+                h += d[k] * d[k];
+            }
+            double f = d[i - 1];
+            double g = std::sqrt(h);
+            if (f > 0) g = -g;
+            e[i] = scale * g;
+            h -= f * g;
+            d[i - 1] = f - g;
+            for (int j = 0; j < i; j++) e[j] = 0;
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                Z(j, i) = f;
+                g = e[j] + Z(j, j) * f;
+                for (int k = j + 1; k <= i - 1; k++) {
+                    g += Z(k, j) * d[k];
+                    e[k] += Z(k, j) * f;
+                }
+                e[j] = g;
+            }
+            f = 0;
+            for (int j = 0; j < i; j++) {
+                e[j] /= h;
+                f += e[j] * d[j];
+            }
+            double hh = f / (h + h);
+            for (int j = 0; j < i; j++) e[j] -= hh * d[j];
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                g = e[j];
+                for (int k = j; k <= i - 1; k++) Z(k, j) -= (f * e[k] + g * d[k]);
+                d[j] = Z(i - 1, j);
+                Z(i, j) = 0;
+            }
+        }
+        d[i] = h;
     }
-    alignas(64) double A[ld * ld];
-    alignas(64) double Z[ld * ld];
-    alignas(64) double p[ld];
-    double beta[ld];
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) A[i * ld + j] = z[i * n + j];
+    for (int i = 0; i < n - 1; i++) {
+        Z(n - 1, i) = Z(i, i);
+        Z(i, i) = 1;
+        double h = d[i + 1];
+        if (h != 0) {
+            for (int k = 0; k <= i; k++) d[k] = Z(k, i + 1) / h;
+            for (int j = 0; j <= i; j++) {
+                double g = 0;
+                for (int k = 0; k <= i; k++) g += Z(k, i + 1) * Z(k, j);
+                for (int k = 0; k <= i; k++) Z(k, j) -= g * d[k];
+            }
+        }
+        for (int k = 0; k <= i; k++) Z(k, i + 1) = 0;
+    }
+    for (int j = 0; j < n; j++) {
+        d[j] = Z(n - 1, j);
+        Z(n - 1, j) = 0;
+    }
+    Z(n - 1, n - 1) = 1;
+    e[0] = 0;
+#undef Z
+    // ZT(v, k) = component k of vector v
+#define ZT(v, k) z[(v) * n + (k)]
 
-    // 1. tridiagonalisation; e[k] couples k and k + 1
-    for (int k = 0; k + 2 < n; k++) {
-        double *x = A + k * ld + k + 1;   // row k right of the diagonal (= column k below it)
-        const int L = n - k - 1;
-        d[k] = A[k * ld + k];
-        double ss = 0;
-        for (int i = 1; i < L; i++) ss += x[i] * x[i];
-        if (ss == 0) {   // nothing to annihilate
-            e[k] = x[0];
-            beta[k] = 0;
-            continue;
-        }
-        const double x0 = x[0];
-        double alpha = std::sqrt(x0 * x0 + ss);
-        if (x0 > 0) alpha = -alpha;
-        const double v0 = x0 - alpha;      // x0 and -alpha have the same sign: no cancellation
-        const double b = -1.0 / (alpha * v0);   // 2 / v^T v
-        x[0] = v0;                         // x is now the reflector v
-        beta[k] = b;
-        e[k] = alpha;
-        double *B = A + (k + 1) * ld + (k + 1);
-        for (int i = 0; i < L; i++) p[i] = 0;
-        for (int j = 0; j < L; j++) {   // p = b B v = b sum_j v_j B[j][.]
-            const double f = b * x[j];
-            const double *Bj = B + j * ld;
-            for (int i = 0; i < L; i++) p[i] += f * Bj[i];
-        }
-        double pv = 0;
-        for (int i = 0; i < L; i++) pv += p[i] * x[i];
-        const double K = 0.5 * b * pv;
-        for (int i = 0; i < L; i++) p[i] -= K * x[i];   // w = p - K v
-        for (int i = 0; i < L; i++) {   // B -= v w^T + w v^T
-            const double vi = x[i], wi = p[i];
-            double *Bi = B + i * ld;
-            for (int j = 0; j < L; j++) Bi[j] -= vi * p[j] + wi * x[j];
-        }
-    }
-    d[n - 2] = A[(n - 2) * ld + n - 2];
-    e[n - 2] = A[(n - 2) * ld + n - 1];
-    d[n - 1] = A[(n - 1) * ld + n - 1];
+    // QL with implicit shifts.  The rotations of one sweep are applied to the
+    // vectors while the next sweep's rotations are computed: that chain is
+    // bound by the latency of its square roots and divisions, the vector
+    // updates by throughput, so the two overlap.  Every vector component
+    // still sees the same rotations in the same order.
+    for (int i = 1; i < n; i++) e[i - 1] = e[i];
     e[n - 1] = 0;
-
-    // 2. Q, then its transpose in A (row v = column v of Q, zero-padded to a
-    // multiple of 4 components for the blocked rotations)
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) Z[i * ld + j] = i == j ? 1.0 : 0.0;
-    for (int k = n - 3; k >= 0; k--) {
-        if (beta[k] == 0) continue;
-        const double *v = A + k * ld + k + 1;
-        const int L = n - k - 1;
-        double *S = Z + (k + 1) * ld + (k + 1);
-        for (int j = 0; j < L; j++) p[j] = 0;
-        for (int i = 0; i < L; i++) {   // p = v^T S
-            const double vi = v[i];
-            const double *Si = S + i * ld;
-            for (int j = 0; j < L; j++) p[j] += vi * Si[j];
-        }
-        for (int i = 0; i < L; i++) {   // S -= b v p^T
-            const double f = beta[k] * v[i];
-            double *Si = S + i * ld;
-            for (int j = 0; j < L; j++) Si[j] -= f * p[j];
-        }
-    }
-    const int nv = (n + 3) & ~3;
-    for (int v = 0; v < n; v++) {
-        for (int k = 0; k < n; k++) A[v * ld + k] = Z[k * ld + v];
-        for (int k = n; k < nv; k++) A[v * ld + k] = 0;
-    }
-#define ZT(v, k) A[(v) * ld + (k)]
-
-    // 3. QL with implicit shifts.  The rotations of one sweep are applied to
-    // the vectors while the next sweep's rotations are computed: that chain
-    // is bound by the latency of its square roots and divisions, the vector
-    // updates by throughput, so the two overlap.  Rotations still reach the
-    // vectors in the order the sweeps make them.
-    double cs[2][ld][2];
+    double cs[2][64][2];
     int cur = 0;           // the sweep being computed writes cs[cur]
     int pj = -1, pl = 0;   // rotations pj, pj - 1, .., pl of cs[cur ^ 1] still to apply
     auto apply_one = [&]() {
         const double c = cs[cur ^ 1][pj][0], s = cs[cur ^ 1][pj][1];
         double *zi = &ZT(pj, 0), *zi1 = &ZT(pj + 1, 0);
-        for (int k = 0; k < nv; k += 4) {
-            v4d x, y;
-            std::memcpy(&x, zi + k, sizeof(v4d));
-            std::memcpy(&y, zi1 + k, sizeof(v4d));
-            const v4d up = s * x + c * y, lo = c * x - s * y;
-            std::memcpy(zi1 + k, &up, sizeof(v4d));
-            std::memcpy(zi + k, &lo, sizeof(v4d));
+        for (int k = 0; k < n; k++) {
+            const double a = zi[k], b = zi1[k];
+            zi1[k] = s * a + c * b;
+            zi[k] = c * a - s * b;
         }
         pj--;
     };
@@ -240,30 +223,30 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
             do {
                 if (++iter > 60) return false;
                 double g = d[l];
-                double q = (d[l + 1] - g) / (2 * e[l]);
-                double r = std::hypot(q, 1.0);
-                if (q < 0) r = -r;
-                d[l] = e[l] / (q + r);
-                d[l + 1] = e[l] * (q + r);
+                double p = (d[l + 1] - g) / (2 * e[l]);
+                double r = std::hypot(p, 1.0);
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
                 double dl1 = d[l + 1];
                 double h = g - d[l];
                 for (int i = l + 2; i < n; i++) d[i] -= h;
                 f += h;
-                q = d[m];
+                p = d[m];
                 double c = 1, c2 = 1, c3 = 1, el1 = e[l + 1], s = 0, s2 = 0;
                 for (int i = m - 1; i >= l; i--) {
                     c3 = c2;
                     c2 = c;
                     s2 = s;
                     g = c * e[i];
-                    h = c * q;
-                    // q and e[i] are of the order of the matrix norm here, so
+                    h = c * p;
+                    // p and e[i] are of the order of the matrix norm here, so
                     // the plain square root cannot overflow (hypot is ~3x slower)
-                    r = std::sqrt(q * q + e[i] * e[i]);
+                    r = std::sqrt(p * p + e[i] * e[i]);
                     e[i + 1] = s * r;
                     s = e[i] / r;
-                    c = q / r;
-                    q = c * d[i] - s * g;
+                    c = p / r;
+                    p = c * d[i] - s * g;
                     d[i + 1] = h + s * (c * g + s * d[i]);
                     cs[cur][i][0] = c;
                     cs[cur][i][1] = s;
@@ -273,34 +256,32 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
                 pj = m - 1;                     // this sweep's rotations are next
                 pl = l;
                 cur ^= 1;
-                q = -s * s2 * c3 * el1 * e[l] / dl1;
-                e[l] = s * q;
-                d[l] = c * q;
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = c * p;
             } while (std::fabs(e[l]) > eps * tst1);
         }
         d[l] = d[l] + f;
         e[l] = 0;
     }
     while (pj >= pl) apply_one();
-    // sort descending (eigenvalues and vectors)
+    // sort descending (eigenvalues and vector columns)
     for (int i = 0; i < n - 1; i++) {
         int k = i;
-        double q = d[i];
+        double p = d[i];
         for (int j = i + 1; j < n; j++)
-            if (d[j] > q) { k = j; q = d[j]; }
+            if (d[j] > p) { k = j; p = d[j]; }
         if (k != i) {
             d[k] = d[i];
-            d[i] = q;
+            d[i] = p;
             for (int j = 0; j < n; j++) std::swap(ZT(i, j), ZT(k, j));
         }
     }
-    // eigenvectors back in the columns of z
-    for (int v = 0; v < n; v++)
-        for (int k = 0; k < n; k++) z[k * n + v] = ZT(v, k);
-#undef ZT
+    // back to eigenvectors in columns
     for (int i = 0; i < n; i++)
-        if (!std::isfinite(d[i])) return false;
+        for (int j = i + 1; j < n; j++) std::swap(ZT(i, j), ZT(j, i));
     return true;
+#undef ZT
 }
 
 // Q = U diag(root) V with UV = I, via the symmetric S = Pi^1/2 Q Pi^-1/2;
