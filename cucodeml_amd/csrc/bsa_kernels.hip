@@ -327,12 +327,14 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
 }
 
 // The same job with every interior child's P resident in LDS, for jobs with
-// at most two interior children: a block of 16 waves takes one (node, class)
-// job and tpb tiles of 256 patterns.  The P tiles are staged once; then each
+// at most two interior children: a block of 8 waves takes one (node, class)
+// job and tpb tiles of 128 patterns.  The P tiles are staged once; then each
 // wave runs its 16-pattern rows of every tile with no further barrier, so
-// the waves overlap one another's loads and products freely.  Per pattern
-// the arithmetic is that of k_prune_mfma, so results are the same bitwise.
-constexpr int kResWaves = 16;
+// the waves overlap one another's loads and products freely.  At 128 VGPRs
+// and 64 KB of LDS two such blocks share a CU, so one block's staging
+// overlaps the other's products.  Per pattern the arithmetic is that of
+// k_prune_mfma, so results are the same bitwise.
+constexpr int kResWaves = 8;
 
 template <typename T>
 __global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_res(const PruneJob *__restrict__ jobs,
@@ -1094,7 +1096,7 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
 {
     if (njobs <= 0) return;
     // P resident in LDS where every job has at most two interior children
-    // and the launch has at least half as many blocks of 16 waves as CUs
+    // and the launch has at least half as many blocks of 8 waves as CUs
     // (below that the 4-wave blocks spread the work better)
     const int nres = (npad + 16 * kResWaves - 1) / (16 * kResWaves);
     if (kernel == 0 && tile == 64 && maxint <= 2 && (long)nres * njobs >= 128) {
