@@ -157,12 +157,13 @@ template <typename T> struct Store;
 template <> struct Store<double> {
     static __device__ inline double get(const double *p) { return *p; }
     static __device__ inline void put(double *p, double v, uint32_t) { *p = v; }
-    // 16 consecutive values from a 128-byte aligned address
+    // N (8 or 16) consecutive values from an N*8-byte aligned address
+    template <int N = 16>
     static __device__ inline void load16(const double *p, double *a)
     {
         const d2 *q = reinterpret_cast<const d2 *>(p);
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < N / 2; i++) {
             d2 v = q[i];
             a[2 * i] = v.x;
             a[2 * i + 1] = v.y;
@@ -180,11 +181,12 @@ template <> struct Store<float> {
         u &= ~0x1FFFFFFFull;
         *p = (float)__longlong_as_double((long long)u);
     }
+    template <int N = 16>
     static __device__ inline void load16(const float *p, double *a)
     {
         const f4 *q = reinterpret_cast<const f4 *>(p);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < N / 4; i++) {
             f4 v = q[i];
             a[4 * i] = v.x;
             a[4 * i + 1] = v.y;
@@ -685,15 +687,16 @@ __device__ inline double pow2i(int d)
     return ldexp(1.0, d);
 }
 
-// Value of X_t(h, j) = Abar_parent,t(h, j) * prod_{siblings} C_s,t(h, j) for 16
-// consecutive states j0..j0+15 of pattern h (one lane), and its exponent.
-template <typename T>
+// Value of X_t(h, j) = Abar_parent,t(h, j) * prod_{siblings} C_s,t(h, j) for N
+// (16 or 8) consecutive states j0..j0+N-1 of pattern h (one lane), and its
+// exponent.
+template <typename T, int N = 16>
 __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npad,
                               const unsigned char *z, const double *TT, int ncodes, const int *E,
                               const T *Cb, const T *A, const int *EA, double *x)
 {
     const int asl = jb.a_aslot[term];
-    Store<T>::load16(A + ((size_t)asl * npad + h) * 64 + j0, x);
+    Store<T>::template load16<N>(A + ((size_t)asl * npad + h) * 64 + j0, x);
     int ex = EA[(size_t)asl * npad + h];
     for (int q = 0; q < jb.nsib; q++) {
         const int tip = jb.sib_tip[q];
@@ -701,33 +704,34 @@ __device__ inline int x_row16(const AdjJob &jb, int term, int h, int j0, int npa
             const int pm = jb.sib_pmat[term][q];
             const int code = z[(size_t)tip * npad + h];
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) x[kk] *= tip_value(TT, pm, ncodes, code, j0 + kk);
+            for (int kk = 0; kk < N; kk++) x[kk] *= tip_value(TT, pm, ncodes, code, j0 + kk);
         } else {
-            double c[16];
-            Store<T>::load16(Cb + ((size_t)jb.sib_cslot[term][q] * npad + h) * 64 + j0, c);
+            double c[N];
+            Store<T>::template load16<N>(Cb + ((size_t)jb.sib_cslot[term][q] * npad + h) * 64 + j0,
+                                         c);
 #pragma unroll
-            for (int kk = 0; kk < 16; kk++) x[kk] *= c[kk];
+            for (int kk = 0; kk < N; kk++) x[kk] *= c[kk];
             ex += E[(size_t)jb.sib_slot[term][q] * npad + h];
         }
     }
     return ex;
 }
 
-// X(h, j0..j0+15) = sum over the job's terms of X_t, exponents aligned to the
-// largest; returns the exponent.
-template <typename T>
+// X(h, j0..j0+N-1) = sum over the job's terms of X_t, exponents aligned to
+// the largest; returns the exponent (the same for every j0 of a pattern).
+template <typename T, int N = 16>
 __device__ inline int x_sum16(const AdjJob &jb, int h, int j0, int npad, const unsigned char *z,
                               const double *TT, int ncodes, const int *E, const T *Cb,
                               const T *A, const int *EA, double *x)
 {
-    int ex = x_row16<T>(jb, 0, h, j0, npad, z, TT, ncodes, E, Cb, A, EA, x);
+    int ex = x_row16<T, N>(jb, 0, h, j0, npad, z, TT, ncodes, E, Cb, A, EA, x);
     for (int term = 1; term < jb.nterm; term++) {
-        double t[16];
-        const int et = x_row16<T>(jb, term, h, j0, npad, z, TT, ncodes, E, Cb, A, EA, t);
+        double t[N];
+        const int et = x_row16<T, N>(jb, term, h, j0, npad, z, TT, ncodes, E, Cb, A, EA, t);
         const int em = ex > et ? ex : et;
         const double f0 = pow2i(ex - em), f1 = pow2i(et - em);
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) x[kk] = x[kk] * f0 + t[kk] * f1;
+        for (int kk = 0; kk < N; kk++) x[kk] = x[kk] * f0 + t[kk] * f1;
         ex = em;
     }
     return ex;
@@ -816,7 +820,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
 // strips that the tile's codes reach (usually one of four).  tperm entries
 // hold the pattern in the low 24 bits and its code in the top 8.
 template <typename T>
-__global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ jobs,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_adj_m(const AdjJob *__restrict__ jobs,
                                                   const int *__restrict__ sel, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ TT, int ncodes,
@@ -854,13 +858,26 @@ __global__ __launch_bounds__(256, 3) void k_adj_m(const AdjJob *__restrict__ job
             hx = pnext & 0xFFFFFF;
             if (t + 1 < tpc && tile + 1 < ntile) pnext = perm[h0 + 64 + 16 * w + r];
         }
-        double x[16];
-        int ex = x_sum16<T>(jb, hx, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
-        if (!vtip) ex += E[(size_t)jb.v_slot * npad + hx];
-        const double sc = pow2i(ex);
-        __syncthreads();   // the previous tile's reads of sX are done
+        // X in two halves of 8 states, so that at most 8 values of each
+        // operand are live at a time (VGPRs for four blocks per CU)
+        const int exv = vtip ? 0 : E[(size_t)jb.v_slot * npad + hx];
+        {
+            double x[8];
+            const int ex = x_sum16<T, 8>(jb, hx, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
+            const double sc = pow2i(ex + exv);
+            __syncthreads();   // the previous tile's reads of sX are done
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) sX[(16 * w + r) * kLdsStride + 16 * g + kk] = x[kk] * sc;
+            for (int kk = 0; kk < 8; kk++) sX[(16 * w + r) * kLdsStride + 16 * g + kk] = x[kk] * sc;
+        }
+        {
+            double x[8];
+            const int ex =
+                x_sum16<T, 8>(jb, hx, 16 * g + 8, npad, z, TT, ncodes, E, Cb, A, EA, x);
+            const double sc = pow2i(ex + exv);
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++)
+                sX[(16 * w + r) * kLdsStride + 16 * g + 8 + kk] = x[kk] * sc;
+        }
         if (vtip) {
             // the tip's codes in rows 4kk + g of the tile, and the column
             // strips they reach (every strip for an ambiguous code)
