@@ -273,7 +273,7 @@ def main():
                     help="genes of the genome-scan sample timed beside (0: skip)")
     ap.add_argument("--scan-batch", type=int, default=256,
                     help="genes per batch engine in the scan (0: one engine per gene)")
-    ap.add_argument("--scan-concurrent", type=int, default=3,
+    ap.add_argument("--scan-concurrent", type=int, default=6,
                     help="batches fitted at once in the scan")
     ap.add_argument("--scan-workers", type=int, default=8,
                     help="engines at once when the scan runs one engine per gene")

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -292,6 +293,78 @@ extern "C" int bsa_batch_eval_genes(bsa_batch *b, const int *genes, int n, const
     return BSA_OK;
 }
 
+// One bounded BFGS fit of a gene under its current model options, from x
+// (overwritten with the estimates); its evaluations go through the round
+// server.
+using GeneFit = std::function<int(double *x, const double *lo, const double *hi, FitResult *res)>;
+
+// Runs chain(i, fit) for every gene i on at most kMaxFitThreads optimiser
+// threads (a worker takes the next gene when a chain ends) and serves the
+// fits' evaluation requests in rounds on this thread.  A chain may change
+// its own gene's model options between fits: only the gene's own thread
+// stages it, and no request of the gene is pending then.
+static int run_fit_chains(bsa_batch *b, const bsa_fit_opts *opts,
+                          const std::function<int(int, const GeneFit &)> &chain)
+{
+    const int G = (int)b->genes.size();
+    Forest &f = b->f;
+    int rc = forest_setup_adjoint(f);
+    if (rc) return rc;
+    RoundServer srv(&f);
+    std::vector<int> rcs(G, BSA_OK);
+    std::vector<std::string> errs(G);
+    auto chain_one = [&](int i) {
+        Gene &g = *b->genes[i];
+        const GeneFit fit = [&](double *x0, const double *lo, const double *hi, FitResult *res) {
+            ObjFn fg = [&](const double *xx, double *fv, double *gv) {
+                int r = gene_stage_x(g, f, xx, true);
+                if (r) return r;
+                if ((r = srv.request(i))) return r;
+                *fv = -gene_collect(g, f);
+                gene_chain_rule(g, f, xx, gv);
+                return BSA_OK;
+            };
+            return minimize_box(fg, gene_num_params(g), x0, lo, hi, opts, res);
+        };
+        const int r = chain(i, fit);
+        if (r) {
+            rcs[i] = r;
+            errs[i] = last_error_text();
+        }
+        srv.finished();
+    };
+    {
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            for (int i = next++; i < G; i = next++) chain_one(i);
+        };
+        const int nw = std::min(G, kMaxFitThreads);
+        std::vector<std::thread> th;
+        th.reserve(nw);
+        for (int t = 0; t < nw; t++) th.emplace_back(worker);
+        srv.serve();   // HIP work on this thread only
+        for (auto &t : th) t.join();
+    }
+    f.n_eval += (double)srv.rounds;
+    for (int i = 0; i < G; i++)
+        if (rcs[i]) return fail(rcs[i], "gene %d: %s", i, errs[i].c_str());
+    return BSA_OK;
+}
+
+// every gene evaluated at x[gene] (its last evaluation left there), -lnL out
+static int eval_all_at(bsa_batch *b, double *const *x, double *neg_lnl)
+{
+    Forest &f = b->f;
+    const int G = (int)b->genes.size();
+    int rc = parallel_for(G, [&](int i) { return gene_stage_x(*b->genes[i], f, x[i], false); });
+    if (rc) return rc;
+    std::vector<int> all(G);
+    for (int i = 0; i < G; i++) all[i] = i;
+    if ((rc = forest_run_round(f, all, false))) return rc;
+    for (int i = 0; i < G; i++) neg_lnl[i] = -gene_collect(*b->genes[i], f);
+    return BSA_OK;
+}
+
 extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const *lo,
                              const double *const *hi, const bsa_fit_opts *opts, double *neg_lnl,
                              int *info)
@@ -317,59 +390,93 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
     HIP_SHARED;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(f.dev));
-    int rc = forest_setup_adjoint(f);
-    if (rc) return rc;
-    RoundServer srv(&f);
     std::vector<FitResult> res(G);
-    std::vector<int> rcs(G, BSA_OK);
-    std::vector<std::string> errs(G);
-    auto fit_one = [&](int i) {
-        Gene &g = *b->genes[i];
-        const int np = gene_num_params(g);
-        ObjFn fg = [&](const double *xx, double *fv, double *gv) {
-            int r = gene_stage_x(g, f, xx, true);
-            if (r) return r;
-            if ((r = srv.request(i))) return r;
-            *fv = -gene_collect(g, f);
-            gene_chain_rule(g, f, xx, gv);
-            return BSA_OK;
-        };
-        const int r = minimize_box(fg, np, x[i], blo[i].data(), bhi[i].data(), opts, &res[i]);
-        if (r) {
-            rcs[i] = r;
-            errs[i] = last_error_text();
-        }
-        srv.finished();
-    };
-    {
-        // at most kMaxFitThreads fits in flight; a worker takes the next gene
-        // when its fit ends
-        std::atomic<int> next{0};
-        auto worker = [&]() {
-            for (int i = next++; i < G; i = next++) fit_one(i);
-        };
-        const int nw = std::min(G, kMaxFitThreads);
-        std::vector<std::thread> th;
-        th.reserve(nw);
-        for (int t = 0; t < nw; t++) th.emplace_back(worker);
-        srv.serve();   // HIP work on this thread only
-        for (auto &t : th) t.join();
-    }
-    f.n_eval += (double)srv.rounds;
-    for (int i = 0; i < G; i++)
-        if (rcs[i]) return fail(rcs[i], "gene %d: %s", i, errs[i].c_str());
-    // leave every gene's last evaluation at its estimates
-    rc = parallel_for(G, [&](int i) { return gene_stage_x(*b->genes[i], f, x[i], false); });
+    int rc = run_fit_chains(b, opts, [&](int i, const GeneFit &fit) {
+        return fit(x[i], blo[i].data(), bhi[i].data(), &res[i]);
+    });
     if (rc) return rc;
-    std::vector<int> all(G);
-    for (int i = 0; i < G; i++) all[i] = i;
-    if ((rc = forest_run_round(f, all, false))) return rc;
-    for (int i = 0; i < G; i++) {
-        neg_lnl[i] = -gene_collect(*b->genes[i], f);
-        if (info) {
+    // leave every gene's last evaluation at its estimates
+    if ((rc = eval_all_at(b, x, neg_lnl))) return rc;
+    if (info)
+        for (int i = 0; i < G; i++) {
             info[3 * i] = res[i].iterations;
             info[3 * i + 1] = res[i].evaluations + 1;
             info[3 * i + 2] = res[i].converged;
+        }
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    f.timing[6] = ms;
+    f.timing_sum[6] += ms;
+    return BSA_OK;
+}
+
+extern "C" int bsa_batch_fit_lrt(bsa_batch *b, double *const *x1, double *const *x0,
+                                 const bsa_fit_opts *opts, double *neg_lnl, int *info)
+{
+    if (!b || !x1 || !x0 || !neg_lnl) return fail(BSA_EINVAL, "null argument");
+    if (opts && opts->gradient != BSA_GRAD_ADJOINT)
+        return fail(BSA_EINVAL, "batch fits use the adjoint gradient");
+    const int G = (int)b->genes.size();
+    const bsa_model_opts h1{0, 1.0, 1}, h0{1, 1.0, 1};
+    for (int i = 0; i < G; i++) {
+        if (!x1[i] || !x0[i]) return fail(BSA_EINVAL, "null x for gene %d", i);
+        b->genes[i]->opts = h1;
+    }
+    Forest &f = b->f;
+    HIP_SHARED;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(f.dev));
+    std::vector<FitResult> r1(G), r0(G), r1b(G);
+    std::vector<double> f1(G), f0(G);
+    std::vector<char> refit(G, 0);
+    int rc = run_fit_chains(b, opts, [&](int i, const GeneFit &fit) {
+        Gene &g = *b->genes[i];
+        const int np1 = gene_num_params(g);
+        std::vector<double> lo(np1), hi(np1);
+        default_bounds(g, lo.data(), hi.data());
+        int r = fit(x1[i], lo.data(), hi.data(), &r1[i]);
+        if (r) return r;
+        f1[i] = r1[i].f;
+        // H0 from the H1 estimates without w2 (the last variable)
+        g.opts = h0;
+        std::memcpy(x0[i], x1[i], sizeof(double) * (np1 - 1));
+        default_bounds(g, lo.data(), hi.data());
+        r = fit(x0[i], lo.data(), hi.data(), &r0[i]);
+        g.opts = h1;
+        if (r) return r;
+        f0[i] = r0[i].f;
+        if (-f0[i] > -f1[i] + 1e-6) {
+            // H1 contains H0 (w2 = 1): a lower H1 maximum is a local one;
+            // fit H1 again from the H0 estimates and keep the higher maximum
+            std::vector<double> xb(np1);
+            std::memcpy(xb.data(), x0[i], sizeof(double) * (np1 - 1));
+            xb[np1 - 1] = 1.0;
+            default_bounds(g, lo.data(), hi.data());
+            if ((r = fit(xb.data(), lo.data(), hi.data(), &r1b[i]))) return r;
+            refit[i] = 1;
+            if (r1b[i].f < f1[i]) {
+                std::memcpy(x1[i], xb.data(), sizeof(double) * np1);
+                f1[i] = r1b[i].f;
+            }
+        }
+        return BSA_OK;
+    });
+    if (rc) return rc;
+    // leave every gene's last evaluation at its H1 estimates
+    std::vector<double> nl(G);
+    if ((rc = eval_all_at(b, x1, nl.data()))) return rc;
+    for (int i = 0; i < G; i++) {
+        neg_lnl[2 * i] = nl[i];
+        neg_lnl[2 * i + 1] = f0[i];
+        if (info) {
+            int *o = info + 7 * i;
+            o[0] = r1[i].iterations + (refit[i] ? r1b[i].iterations : 0);
+            o[1] = r1[i].evaluations + (refit[i] ? r1b[i].evaluations : 0) + 1;
+            o[2] = refit[i] ? (r1[i].converged && r1b[i].converged) : r1[i].converged;
+            o[3] = r0[i].iterations;
+            o[4] = r0[i].evaluations;
+            o[5] = r0[i].converged;
+            o[6] = refit[i];
         }
     }
     const double ms =

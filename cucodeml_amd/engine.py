@@ -27,7 +27,7 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_beb", "bsa_set_graph", "bsa_batch_create", "bsa_batch_destroy",
             "bsa_batch_size", "bsa_batch_set_model", "bsa_batch_num_params",
             "bsa_batch_default_bounds", "bsa_batch_lnl", "bsa_batch_lnl_grad",
-            "bsa_batch_eval_genes", "bsa_batch_fit",
+            "bsa_batch_eval_genes", "bsa_batch_fit", "bsa_batch_fit_lrt",
             "bsa_batch_class_lnl", "bsa_batch_last_params", "bsa_batch_timing", "bsa_batch_work",
             "bsa_batch_set_graph", "bsa_debug_forest_plan", "bsa_set_shapes", "bsa_tune",
             "bsa_batch_tune"]
@@ -106,6 +106,7 @@ def lib():
         L.bsa_batch_lnl.argtypes = [C.c_void_p, dpp, dp]
         L.bsa_batch_lnl_grad.argtypes = [C.c_void_p, dpp, dp, dpp]
         L.bsa_batch_fit.argtypes = [C.c_void_p, dpp, dpp, dpp, C.POINTER(_FitOpts), dp, ip]
+        L.bsa_batch_fit_lrt.argtypes = [C.c_void_p, dpp, dpp, C.POINTER(_FitOpts), dp, ip]
         L.bsa_batch_eval_genes.argtypes = [C.c_void_p, ip, C.c_int, dpp, dp, C.c_void_p]
         L.bsa_batch_class_lnl.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.bsa_batch_last_params.argtypes = [C.c_void_p, C.c_int, dp]
@@ -430,6 +431,25 @@ class BranchSiteBatch:
         infos = [{"iterations": int(info[3 * i]), "evaluations": int(info[3 * i + 1]),
                   "converged": bool(info[3 * i + 2])} for i in range(len(xs))]
         return xs, out, infos
+
+    def fit_lrt(self, x1s, e=1e-8, max_iter=2000, max_step=5.0):
+        """H1, then H0 from the H1 estimates, then (where H0 lies above H1)
+        H1 again from the H0 estimates, for every gene, each gene's chain in
+        its own optimiser thread (bsa_batch_fit_lrt).  Every gene's model is
+        left at H1.  Returns (list of H1 x, list of H0 x, -lnL of H1 and of
+        H0 per gene (n, 2), list of info dicts)."""
+        self.set_model(-1, False)
+        x1 = [np.array(x, dtype=np.float64) for x in self._xs(x1s)]
+        x0 = [np.zeros(len(x) - 1) for x in x1]
+        o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
+        out = np.zeros(2 * len(x1))
+        info = np.zeros(7 * len(x1), dtype=np.int32)
+        _check(lib().bsa_batch_fit_lrt(self._h, _ptrs(x1), _ptrs(x0), C.byref(o), _dptr(out),
+                                       info.ctypes.data_as(C.POINTER(C.c_int))))
+        keys = ["h1_iterations", "h1_evaluations", "h1_converged", "h0_iterations",
+                "h0_evaluations", "h0_converged", "refit"]
+        infos = [{k: int(info[7 * i + j]) for j, k in enumerate(keys)} for i in range(len(x1))]
+        return x1, x0, out.reshape(-1, 2), infos
 
     def fhK(self, gene):
         n = self.genes[gene][1].npatt

@@ -138,10 +138,11 @@ def batch_groups(genes, batch_genes=64, max_waste=0.25):
 
 def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
     """H1 and H0 fits and the LRT of a list of genes on one batch engine
-    (bsa_batch_fit).  Same starting points and optimiser as ``fit_gene``;
-    a gene whose H0 maximum lies above its H1 one is fitted again under H1
-    from the H0 estimates on its own engine.  Returns the per-gene dicts
-    of ``fit_gene`` (seconds = the batch's wall time)."""
+    (bsa_batch_fit_lrt): the sequence of ``fit_gene`` (H1, H0 from the H1
+    estimates, H1 again from the H0 estimates where H0 lies above H1), each
+    gene's chain in its own optimiser thread, so no gene waits for another
+    gene's fit.  Returns the per-gene dicts of ``fit_gene`` (seconds = the
+    batch's wall time)."""
     t0 = time.perf_counter()
     n = len(genes)
     b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision=precision,
@@ -153,26 +154,20 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
             s = codeml.default_start(g.tree, 2.0, 1.5, False)
             s[:g.tree.nbranch] = 0.1
             starts.append(s)
-        x1, f1, i1 = b.fit(starts, e=e)
+        x1, x0, negl, info = b.fit_lrt(starts, e=e)
         par1 = [b.last_params(i) for i in range(n)]
-        b.set_model(-1, True, 1.0)
-        x0, f0, i0 = b.fit([x[:-1] for x in x1], e=e)
     finally:
         b.close()
     wall = time.perf_counter() - t0
     out = []
     for i, g in enumerate(genes):
-        h1 = codeml.Fit("H1", -f1[i], x1[i], *par1[i][:5], par1[i][7:11].copy(),
-                        x1[i][:g.tree.nbranch].copy(), len(x1[i]), i1[i]["iterations"],
-                        i1[i]["evaluations"], i1[i]["converged"], wall, {})
-        h0 = codeml.Fit("H0", -f0[i], x0[i], 0, 0, 0, 0, 1.0, None, None, len(x0[i]),
-                        i0[i]["iterations"], i0[i]["evaluations"], i0[i]["converged"], wall, {})
-        if h0.lnL > h1.lnL + 1e-6:
-            h1b = codeml.fit_model_a(g.tree, g.data, False, 1.5, start=np.append(h0.x, 1.0),
-                                     precision=precision, e=e, device=device)
-            h1b.iterations += h1.iterations
-            h1b.evaluations += h1.evaluations
-            h1 = h1b if h1b.lnL > h1.lnL else h1
+        inf = info[i]
+        h1 = codeml.Fit("H1", -negl[i, 0], x1[i], *par1[i][:5], par1[i][7:11].copy(),
+                        x1[i][:g.tree.nbranch].copy(), len(x1[i]), inf["h1_iterations"],
+                        inf["h1_evaluations"], bool(inf["h1_converged"]), wall, {})
+        h0 = codeml.Fit("H0", -negl[i, 1], x0[i], 0, 0, 0, 0, 1.0, None, None, len(x0[i]),
+                        inf["h0_iterations"], inf["h0_evaluations"], bool(inf["h0_converged"]),
+                        wall, {})
         r = codeml.lrt(h0, h1)
         out.append({"gene": g.name, "ns": g.tree.ns, "npatt": g.data.npatt,
                     "lnL1": h1.lnL, "lnL0": h0.lnL, "stat": r["stat"], "p": r["p_mixture"],
@@ -180,11 +175,12 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
                     "params1": [h1.kappa, h1.p0, h1.p1, h1.w0, h1.w2],
                     "iterations": h1.iterations + h0.iterations,
                     "evaluations": h1.evaluations + h0.evaluations,
-                    "converged": h1.converged and h0.converged, "seconds": wall})
+                    "converged": h1.converged and h0.converged, "refit": bool(inf["refit"]),
+                    "seconds": wall})
     return out
 
 
-def scan_batched(genes, batch_genes=64, concurrent=2, precision="fp64", device=0, e=1e-8,
+def scan_batched(genes, batch_genes=64, concurrent=6, precision="fp64", device=0, e=1e-8,
                  graph=False, max_waste=0.25):
     """Fit every gene (H1, H0, LRT) in batches of up to `batch_genes` genes
     of similar size, `concurrent` batches at a time.  Returns (results in

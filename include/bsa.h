@@ -268,6 +268,24 @@ int  bsa_batch_eval_genes(bsa_batch *batch, const int *genes, int n, const doubl
 int  bsa_batch_fit(bsa_batch *batch, double *const *x, const double *const *lo,
                    const double *const *hi, const bsa_fit_opts *opts, double *neg_lnl,
                    int *info);
+/* The per-gene likelihood-ratio fits of a genome scan on the batch engine:
+ * for every gene, H1 (model A, w2 free) from x1[gene], then H0 (w2 fixed
+ * at 1, codeml.c:1648-1649) from the H1 estimates, then, where the H0
+ * maximum lies above the H1 one by more than 1e-6 in lnL, H1 again from
+ * the H0 estimates with w2 = 1, keeping the higher H1 maximum.  This is the
+ * sequence codeml's users run as two control files per gene (the reference
+ * has no single entry for it).  Each gene's chain runs in its own optimiser
+ * thread and does not wait for other genes' fits; requests are served in
+ * rounds as in bsa_batch_fit, with codeml's bounds.  Every gene's model
+ * options are set to H1 (fix_omega = 0) first and are H1 on return.
+ * x1[gene] (np of H1): start in, H1 estimates out; x0[gene] (np - 1): H0
+ * estimates out.  neg_lnl[2*gene] = -lnL at the H1 estimates,
+ * neg_lnl[2*gene+1] at the H0 ones.  info[7*gene ..] (may be NULL): H1
+ * iterations, evaluations, converged (both H1 fits summed), H0 iterations,
+ * evaluations, converged, and 1 where H1 was fitted again.  Every gene's
+ * last evaluation is left at its H1 estimates. */
+int  bsa_batch_fit_lrt(bsa_batch *batch, double *const *x1, double *const *x0,
+                       const bsa_fit_opts *opts, double *neg_lnl, int *info);
 /* bsa_tune for a batch, at x[gene] for every gene */
 int  bsa_batch_tune(bsa_batch *batch, const double *const *x, int reps, double *out6);
 /* per-gene outputs of the last evaluation, as bsa_class_lnl / bsa_last_params */

@@ -220,3 +220,47 @@ def test_batch_fit_of_more_genes_than_fit_threads():
         finally:
             e.close()
         assert abs(fb[k] - fs) <= 1e-6 * abs(fs)
+
+
+def test_chained_lrt_fits_equal_separate_batch_fits():
+    """bsa_batch_fit_lrt runs each gene's H1 -> H0 (-> H1 again) chain
+    without waiting for the other genes; a gene's result does not depend on
+    which genes share a round, so its H1 and H0 fits equal those of two
+    whole-batch fits from the same starts bitwise (genes that need the H1
+    refit are compared against a separate fit from the H0 estimates)."""
+    say("case: chained H1/H0 batch fits vs separate batch fits, 12 genes")
+    genes = scan.simulate_genes(12, seed=5, median=150, clip=(40, 400), taxa=(5, 16))
+    starts = []
+    for g in genes:
+        s = codeml.default_start(g.tree, 2.0, 1.5, False)
+        s[:g.tree.nbranch] = 0.1
+        starts.append(s)
+    b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes])
+    try:
+        x1c, x0c, negl, info = b.fit_lrt(starts)
+        par_c = [b.last_params(i) for i in range(len(genes))]
+        x1, f1, i1 = b.fit(starts)
+        b.set_model(-1, True, 1.0)
+        x0, f0, i0 = b.fit([x[:-1] for x in x1])
+        b.set_model(-1, False)
+        refits = [i for i in range(len(genes)) if info[i]["refit"]]
+        if refits:
+            xb, fb, _ = b.fit([np.append(x0[i], 1.0) if i in refits else x1[i]
+                               for i in range(len(genes))])
+    finally:
+        b.close()
+    for i in range(len(genes)):
+        say(f"  gene {i}: H1 {-negl[i, 0]:.6f} / {-f1[i]:.6f}  H0 {-negl[i, 1]:.6f} / "
+            f"{-f0[i]:.6f}  refit {info[i]['refit']}")
+        assert info[i]["h1_converged"] and info[i]["h0_converged"]
+        assert np.array_equal(x0c[i], x0[i]) and negl[i, 1] == f0[i]
+        assert info[i]["h0_evaluations"] == i0[i]["evaluations"] - 1
+        assert info[i]["refit"] == int(-f0[i] > -f1[i] + 1e-6)
+        if info[i]["refit"]:
+            best = (xb[i], fb[i]) if fb[i] < f1[i] else (x1[i], f1[i])
+            assert np.array_equal(x1c[i], best[0]) and negl[i, 0] == best[1]
+        else:
+            assert np.array_equal(x1c[i], x1[i]) and negl[i, 0] == f1[i]
+            assert info[i]["h1_evaluations"] == i1[i]["evaluations"]
+        # the engine is left at the H1 estimates
+        assert abs(par_c[i][4] - (x1c[i][-1])) <= 1e-12 * max(1.0, abs(x1c[i][-1]))
