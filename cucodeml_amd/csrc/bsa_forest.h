@@ -198,7 +198,8 @@ void gene_chain_rule(const Gene &g, const Forest &f, const double *x, double *gr
 
 // Launch shapes: pruning tile (64 or 128 patterns), child-adjoint tiles per
 // block (0 = by launch size, else 1..8), branch-matrix tiles per chunk (0 = by
-// gene size, else 1..kAdjTilesPerChunk).  Captured graphs are dropped.
+// gene size, up to kAdjTilesPerChunkAuto, else 1..kAdjTilesPerChunk).
+// Captured graphs are dropped.
 int forest_set_shapes(Forest &f, int tile, int tpb, int tpc);
 // Autotuner: for each shape parameter in turn (pruning tile, child-adjoint
 // tiles per block, branch-matrix tiles per chunk) every candidate is timed

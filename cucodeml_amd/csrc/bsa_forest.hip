@@ -662,10 +662,11 @@ void forest_plan_adjoint(Forest &f)
     }
     f.nmrow = (int)f.adj.size();   // one branch matrix per job
     const int ntile = f.npad / 64;
-    // tiles per branch-matrix block: up to kAdjTilesPerChunk, fewer when the
-    // genes are small, so that the launch still has about 1024 blocks
+    // tiles per branch-matrix block: up to kAdjTilesPerChunkAuto, fewer when
+    // the genes are small, so that the launch still has about 1024 blocks
     f.tpc = f.tpc_set > 0 ? f.tpc_set
-                          : std::max(1, std::min(kAdjTilesPerChunk, ntile * (int)f.adj.size() / 1024));
+                          : std::max(1, std::min(kAdjTilesPerChunkAuto,
+                                                 ntile * (int)f.adj.size() / 1024));
     f.nchunk = (ntile + f.tpc - 1) / f.tpc;
     for (int k = 6; k < 10; k++) {
         f.work[k] = 0;
@@ -1295,7 +1296,8 @@ int forest_set_shapes(Forest &f, int tile, int tpb, int tpc)
     if (!f.adj_ready) return BSA_OK;   // the chunking is set when the adjoint is
     const int ntile = f.npad / 64;
     f.tpc = tpc > 0 ? tpc
-                    : std::max(1, std::min(kAdjTilesPerChunk, ntile * (int)f.adj.size() / 1024));
+                    : std::max(1, std::min(kAdjTilesPerChunkAuto,
+                                           ntile * (int)f.adj.size() / 1024));
     f.nchunk = (ntile + f.tpc - 1) / f.tpc;
     if (f.nchunk > f.mpart_chunks) {
         HIPCHK(hipSetDevice(f.dev));
@@ -1336,7 +1338,7 @@ int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *o
     const std::vector<int> cand[3] = {
         f.kernel == BSA_KERNEL_MFMA ? std::vector<int>{64, 128} : std::vector<int>{64},
         {0, 1, 2, 4, 8},
-        {0, 4, 8, 16, 32}};
+        {0, 4, 8, 16, 32, 40, 64}};
     for (int k = 0; k < 3; k++) {
         int keep = cur[k];
         for (int v : cand[k]) {

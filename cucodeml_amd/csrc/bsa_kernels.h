@@ -87,7 +87,8 @@ struct EigJob {
     double tau;                 // t * Qfactor
 };
 
-constexpr int kAdjTilesPerChunk = 32;   // most 64-pattern tiles per branch-matrix block
+constexpr int kAdjTilesPerChunk = 64;   // most 64-pattern tiles per branch-matrix block
+constexpr int kAdjTilesPerChunkAuto = 32;   // most of them when chosen by the gene's size
 constexpr int kWParts = 8;              // partial sums per eigen set in the W reduction
 
 // adjoints of one depth's interior children (jobs with v_tip < 0 only);

@@ -177,7 +177,7 @@ int  bsa_work(bsa_engine *eng, double *out10);
 
 /* launch shapes: pruning tile (64 or 128 patterns per block), child-adjoint
  * tiles per block (0: chosen by each launch's size, else 1..8), branch-matrix
- * tiles per chunk (0: chosen by the gene's size, else 1..32).  Results do not
+ * tiles per chunk (0: chosen by the gene's size, at most 32, else 1..64).  Results do not
  * depend on the first two; the chunking sets the order in which a branch
  * matrix is summed (gradients move in the 16th digit).  No counterpart in
  * the reference. */

@@ -32,7 +32,8 @@ def test_shapes_leave_results_unchanged(gene, precision):
     e = engine.BranchSiteEngine(tree, data, precision=precision)
     try:
         f0, g0 = e.lnl_grad(x)
-        for tile, tpb, tpc in [(128, 0, 0), (64, 1, 4), (64, 8, 32), (128, 2, 1), (64, 0, 0)]:
+        for tile, tpb, tpc in [(128, 0, 0), (64, 1, 4), (64, 8, 32), (64, 2, 64), (128, 2, 1),
+                               (64, 0, 0)]:
             e.set_shapes(tile, tpb, tpc)
             f, g = e.lnl_grad(x)
             rel = np.abs(g - g0).max() / np.abs(g0).max()
@@ -40,7 +41,7 @@ def test_shapes_leave_results_unchanged(gene, precision):
                   flush=True)
             assert f == f0
             assert rel <= 1e-12
-        for bad in [(96, 0, 0), (64, 9, 0), (64, 0, 33), (64, -1, 0)]:
+        for bad in [(96, 0, 0), (64, 9, 0), (64, 0, 65), (64, -1, 0)]:
             with pytest.raises(engine.BsaError):
                 e.set_shapes(*bad)
     finally:
