@@ -193,3 +193,21 @@ def test_coinciding_omegas_share_one_eigen_system(w0, w2, same):
         _, _, Ri, Ui, Vi = engine.model_a_setup(2.2, 0.5, 0.3, a, b, pi)
         k = 0 if s == 0 else (1 if w[s] == 1.0 else 2)
         assert np.array_equal(R[s], Ri[k]) and np.array_equal(U[s], Ui[k])
+
+
+def test_host_model_setup_is_thread_safe():
+    """Evaluations on several host threads at once share two helper threads
+    for their eigen systems (one caller at a time; the others start threads
+    of their own): every thread's systems equal the ones computed alone,
+    bitwise."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(11)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    pars = [(1.5 + 0.1 * k, 0.5, 0.3, 0.05 + 0.02 * k, 2.0 + 0.3 * k) for k in range(16)]
+    alone = [engine.model_a_setup(*p, pi) for p in pars]
+    with ThreadPoolExecutor(8) as ex:
+        for _ in range(3):
+            got = list(ex.map(lambda p: engine.model_a_setup(*p, pi), pars))
+            for a, b in zip(alone, got):
+                for u, v in zip(a, b):
+                    assert np.array_equal(u, v)
