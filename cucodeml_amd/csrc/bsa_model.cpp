@@ -196,6 +196,7 @@ static bool sym_eigen(int n, double *z, double *d, double *e)
     // still sees the same rotations in the same order.
     for (int i = 1; i < n; i++) e[i - 1] = e[i];
     e[n - 1] = 0;
+    if (n > 64) return false;   // rotation buffers below hold 64 (n <= 61 here)
     double cs[2][64][2];
     int cur = 0;           // the sweep being computed writes cs[cur]
     int pj = -1, pl = 0;   // rotations pj, pj - 1, .., pl of cs[cur ^ 1] still to apply
