@@ -1150,9 +1150,11 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int
                           const int *E, const T *Cb, T *A, int *EA, int tpb)
 {
     if (njobs <= 0) return;
-    // several tiles per block (P_v staged once) when the level has blocks to spare
+    // several tiles per block (P_v staged once) when the level has blocks to
+    // spare: up to 4, while the launch keeps at least 512 tiles' worth of
+    // blocks per tile of a block
     const int ntile = npad / 64;
-    if (tpb <= 0) tpb = std::max(1, std::min(8, njobs * ntile / 2048));
+    if (tpb <= 0) tpb = std::max(1, std::min(4, njobs * ntile / 512));
     hipLaunchKernelGGL(k_adj_a<T>, dim3((ntile + tpb - 1) / tpb, njobs), dim3(256), 0, s, jobs,
                        sel, npad, z, PT, TT, ncodes, E, Cb, A, EA, tpb);
 }
