@@ -2,12 +2,20 @@
 
 Reads a codeml control file (the subset the model-A path uses: seqfile,
 treefile, outfile, seqtype = 1, model = 2, NSsites = 2, CodonFreq = 2,
-kappa, fix_omega, omega, cleandata = 0, ndata; codeml.c:1430-1768), the PHYLIP
-codon alignment(s) and the Newick tree with ``#1`` on the foreground branch,
-fits the model by maximum likelihood on the GPU (bsa_fit, in place of ming2
-at codeml.c:688) and writes the estimates in the layout of codeml's ``out``
-file (lnL line codeml.c:738-741, parameter vector 751, site-class table
-989-1001).
+kappa, fix_omega, omega, cleandata = 0, ndata, getSE, Small_Diff,
+fix_blength 0, 1 or 2; codeml.c:1430-1768), the PHYLIP codon alignment(s)
+and the Newick tree with ``#1`` on the foreground branch, fits the model by
+maximum likelihood on the GPU (bsa_fit, in place of ming2 at codeml.c:688)
+and writes the estimates in the layout of codeml's ``out`` file (lnL line
+codeml.c:738-741, parameter vector 751, site-class table 989-1001), the
+per-pattern log f of the final evaluation in the layout of its ``lnf``
+file (codeml.c:596-597, 613, 832-834; treesub.c:7332-7337) and, under H1,
+the site posteriors in ``rst``.
+
+With fix_blength = 2 the branch lengths are held at the tree file's values
+and only kappa, the p's and the omegas are estimated (treesub.c:3729-3730,
+codeml.c:617-627); 0 and 1 estimate them (the tree's lengths, 0.1 where
+absent, are the start); -1 (random starts) and 3 are refused.
 
 Starting values: an ``in.codeml`` file beside the control file is read as
 codeml reads it (treesub.c:3925-3927: the parameter vector in codeml's
@@ -67,6 +75,9 @@ def check_ctl(ctl):
         raise ValueError("only the universal code (icode = 0) is implemented")
     if int(_num(ctl, "fix_kappa", 0)) != 0:
         raise ValueError("fix_kappa = 1 is not implemented")
+    fb = int(_num(ctl, "fix_blength", 0))
+    if fb not in (0, 1, 2):
+        raise ValueError(f"fix_blength = {fb} is not implemented (0, 1 and 2 are)")
 
 
 @dataclass
@@ -87,13 +98,16 @@ class Fit:
     converged: bool
     seconds: float
     extra: dict = field(default_factory=dict)
+    stop: str = ""                # why the optimiser stopped (engine.STOP_NAMES)
+    fixed_branches: bool = False  # fix_blength = 2: branch lengths not estimated
 
     def x_untransformed(self):
-        """x as codeml prints it: p0, p1 themselves in place of the logistic variables."""
+        """x as codeml prints it: p0, p1 themselves in place of the logistic
+        variables, without the branch lengths when they are fixed."""
         nb = len(self.branch)
         x = self.x.copy()
         x[nb + 1], x[nb + 2] = self.p0, self.p1
-        return x
+        return x[nb:] if self.fixed_branches else x
 
 
 def default_start(tree, kappa=2.0, omega=1.5, fix_omega=False):
@@ -105,10 +119,32 @@ def default_start(tree, kappa=2.0, omega=1.5, fix_omega=False):
     return np.array(x)
 
 
+def _bounds(eng_bounds, x0, nb, fixed_branches):
+    """codeml's bounds, with lo = hi = x0 on the branch lengths when they are
+    held fixed (bsa_fit keeps a variable with lo == hi where it is)."""
+    lo, hi = eng_bounds
+    if fixed_branches:
+        lo, hi = lo.copy(), hi.copy()
+        lo[:nb] = hi[:nb] = x0[:nb]
+    return lo, hi
+
+
+def _merge_info(a, b):
+    """Info of an FP32 fit finished by an FP64 one: counts summed, the
+    finishing fit's stop reason."""
+    return {"iterations": a["iterations"] + b["iterations"],
+            "evaluations": a["evaluations"] + b["evaluations"],
+            "converged": b["converged"], "stop": b["stop"]}
+
+
 def fit_model_a(tree, data, fix_omega=False, omega=1.0, start=None, precision="fp64",
-                gradient="adjoint", e=1e-8, device=0, kappa=2.0, eng=None, polish=True):
+                gradient="adjoint", e=1e-8, device=0, kappa=2.0, eng=None, polish=True,
+                fixed_branches=False):
     """Maximum-likelihood fit of H1 (fix_omega False) or H0 (w2 fixed at
-    omega).  ``start`` is in codeml's optimisation variables.
+    omega).  ``start`` is in codeml's optimisation variables.  With
+    ``fixed_branches`` (fix_blength = 2) the branch lengths stay at the
+    start's values (the tree's, when start is None) and only kappa, the p's
+    and the omegas are fitted.
 
     With FP32 partials the likelihood carries rounding noise of about 1e-9
     of |lnL| (stochastic rounding of every stored value), which stalls BFGS
@@ -121,54 +157,69 @@ def fit_model_a(tree, data, fix_omega=False, omega=1.0, start=None, precision="f
                                       fix_omega=fix_omega, omega_fix=omega)
     else:
         eng.set_model(fix_omega, omega, True)
+    nb = tree.nbranch
     try:
-        x0 = default_start(tree, kappa, omega if not fix_omega else 1.5, fix_omega) \
-            if start is None else np.asarray(start, dtype=np.float64)
+        if start is None:
+            x0 = default_start(tree, kappa, omega if not fix_omega else 1.5, fix_omega)
+            if fixed_branches:
+                x0[:nb] = tree.branch_vector()
+        else:
+            x0 = np.asarray(start, dtype=np.float64)
         t0 = time.perf_counter()
-        x, f, info = eng.fit(x0, e=e, gradient=gradient)
+        lo, hi = _bounds(eng.default_bounds(), x0, nb, fixed_branches)
+        x, f, info = eng.fit(x0, lo, hi, e=e, gradient=gradient)
         extra = {}
         par = eng.last_params()
         if eng.precision == "fp32" and polish:
             extra = {"fp32_lnL": -f, "fp32_iterations": info["iterations"],
-                     "fp32_evaluations": info["evaluations"]}
+                     "fp32_evaluations": info["evaluations"], "fp32_stop": info["stop"]}
             e64 = engine.BranchSiteEngine(tree, data, precision="fp64", device=eng.device,
                                           fix_omega=fix_omega, omega_fix=omega)
             try:
-                x, f, info2 = e64.fit(x, e=e, gradient=gradient)
+                x, f, info2 = e64.fit(x, lo, hi, e=e, gradient=gradient)
                 par = e64.last_params()
             finally:
                 e64.close()
-            info = {"iterations": info["iterations"] + info2["iterations"],
-                    "evaluations": info["evaluations"] + info2["evaluations"],
-                    "converged": info2["converged"]}
+            info = _merge_info(info, info2)
         sec = time.perf_counter() - t0
-        nb = tree.nbranch
+        npar = eng.np - (nb if fixed_branches else 0)
         return Fit("H0" if fix_omega else "H1", -f, x, par[0], par[1], par[2], par[3], par[4],
-                   par[7:11].copy(), x[:nb].copy(), eng.np, info["iterations"],
-                   info["evaluations"], info["converged"], sec, extra)
+                   par[7:11].copy(), x[:nb].copy(), npar, info["iterations"],
+                   info["evaluations"], info["converged"], sec, extra, info["stop"],
+                   fixed_branches)
     finally:
         if own:
             eng.close()
 
 
 def fit_model_a_batch(problems, fix_omega=False, omega=1.0, starts=None, precision="fp64",
-                      e=1e-8, device=0, kappa=2.0, polish=True):
+                      e=1e-8, device=0, kappa=2.0, polish=True, fixed_branches=False):
     """``fit_model_a`` for several data sets at once, on one batch engine
     (adjoint gradient; every data set has its own tree and alignment).
     ``problems`` is a list of (tree, data); ``starts`` a list of starting
-    vectors or None entries.  Same optimiser, starting values and FP32
-    polish as the one-data-set fit, so each result equals that fit's."""
+    vectors or None entries.  Same optimiser, starting values, fixed branch
+    lengths and FP32 polish as the one-data-set fit, so each result equals
+    that fit's."""
     n = len(problems)
     starts = [None] * n if starts is None else list(starts)
-    x0 = [default_start(t, kappa, omega if not fix_omega else 1.5, fix_omega) if s is None
-          else np.asarray(s, dtype=np.float64) for (t, _), s in zip(problems, starts)]
+    x0 = []
+    for (t, _), st in zip(problems, starts):
+        if st is None:
+            v = default_start(t, kappa, omega if not fix_omega else 1.5, fix_omega)
+            if fixed_branches:
+                v[:t.nbranch] = t.branch_vector()
+        else:
+            v = np.asarray(st, dtype=np.float64)
+        x0.append(v)
     t0 = time.perf_counter()
 
     def one_pass(prec, xs):
         b = engine.BranchSiteBatch(problems, precision=prec, device=device,
                                    fix_omega=fix_omega, omega_fix=omega)
         try:
-            x, f, info = b.fit(xs, e=e)
+            bounds = [_bounds(b.default_bounds(i), xs[i], problems[i][0].nbranch,
+                              fixed_branches) for i in range(n)]
+            x, f, info = b.fit(xs, e=e, lo=[a for a, _ in bounds], hi=[c for _, c in bounds])
             return x, f, info, [b.last_params(i) for i in range(n)], b.np
         finally:
             b.close()
@@ -177,11 +228,10 @@ def fit_model_a_batch(problems, fix_omega=False, omega=1.0, starts=None, precisi
     extra = [{} for _ in range(n)]
     if precision == "fp32" and polish:
         extra = [{"fp32_lnL": -f[i], "fp32_iterations": info[i]["iterations"],
-                  "fp32_evaluations": info[i]["evaluations"]} for i in range(n)]
+                  "fp32_evaluations": info[i]["evaluations"], "fp32_stop": info[i]["stop"]}
+                 for i in range(n)]
         x2, f, info2, par, nps = one_pass("fp64", x)
-        info = [{"iterations": a["iterations"] + c["iterations"],
-                 "evaluations": a["evaluations"] + c["evaluations"],
-                 "converged": c["converged"]} for a, c in zip(info, info2)]
+        info = [_merge_info(a, c) for a, c in zip(info, info2)]
         x = x2
     sec = time.perf_counter() - t0
     out = []
@@ -189,23 +239,27 @@ def fit_model_a_batch(problems, fix_omega=False, omega=1.0, starts=None, precisi
         nb = tree.nbranch
         p = par[i]
         out.append(Fit("H0" if fix_omega else "H1", -f[i], x[i], p[0], p[1], p[2], p[3], p[4],
-                       p[7:11].copy(), x[i][:nb].copy(), nps[i], info[i]["iterations"],
-                       info[i]["evaluations"], info[i]["converged"], sec, extra[i]))
+                       p[7:11].copy(), x[i][:nb].copy(), nps[i] - (nb if fixed_branches else 0),
+                       info[i]["iterations"], info[i]["evaluations"], info[i]["converged"], sec,
+                       extra[i], info[i]["stop"], fixed_branches))
     return out
 
 
-def standard_errors(eng, x, small_diff=1e-6):
+def standard_errors(eng, x, small_diff=1e-6, free=None):
     """SEs of the parameters at x in codeml's printed form (p0, p1
     untransformed) from the Hessian approximation of Seo et al. 2004, as the
     reference computes it with getSE = 1 (HessianSKT2004,
     treesub.c:6979-7048): per-site log f at x +- 2 Small_Diff (|x_i| + 1) for
     every parameter (2 np GPU evaluations), H = -sum_h fpatt_h d_i d_j,
     SE = sqrt(diag((-H)^-1)), -1 where not positive (codeml.c:779-784).
-    The engine is left set to untransformed p's."""
+    ``free`` (indices into x) limits the parameters to those estimated (the
+    non-branch ones under fix_blength = 2, where codeml's np excludes the
+    branch lengths).  The engine is left set to untransformed p's."""
     x = np.asarray(x, dtype=np.float64)
     eng.set_model(eng.fix_omega, eng.omega_fix, p_transformed=False)
-    D = np.zeros((len(x), eng.data.npatt))
-    for i in range(len(x)):
+    free = list(range(len(x))) if free is None else list(free)
+    D = np.zeros((len(free), eng.data.npatt))
+    for r, i in enumerate(free):
         eh = 2 * small_diff * (abs(x[i]) + 1)
         xs = x.copy()
         xs[i] = x[i] + eh
@@ -213,7 +267,7 @@ def standard_errors(eng, x, small_diff=1e-6):
         up = eng.fhK()[1]
         xs[i] = x[i] - eh
         eng.lnl(xs)
-        D[i] = (up - eng.fhK()[1]) / (2 * eh)
+        D[r] = (up - eng.fhK()[1]) / (2 * eh)
     H = -(D * eng.data.fpatt[None, :]) @ D.T
     d = np.diag(np.linalg.inv(-H))
     return np.where(d > 0, np.sqrt(np.abs(d)), -1.0)
@@ -238,7 +292,8 @@ def format_out(tree, data, fit, names):
                       tree.ibranch, names)
     lines = [f"Model A: branch-site (model = 2, NSsites = 2), {fit.hypothesis}",
              f"ns = {data.ns}  ls = {data.ls}  patterns = {data.npatt}", ""]
-    lines.append(f"lnL(ntime: {nb}  np: {fit.np}):  {fit.lnL:12.6f}      +0.000000")
+    ntime = 0 if fit.fixed_branches else nb
+    lines.append(f"lnL(ntime: {ntime}  np: {fit.np}):  {fit.lnL:12.6f}      +0.000000")
     lines.append("  ".join(f"{v:8.6f}" for v in fit.x_untransformed()))
     lines.append("")
     lines.append(tr.newick(lengths=True, labels=True, names=names, digits=6))
@@ -254,7 +309,8 @@ def format_out(tree, data, fit, names):
     lines.append("foreground w     " + " ".join(f"{v:8.5f}" for v in (fit.w0, 1.0, fit.w2, fit.w2)))
     lines.append("")
     lines.append(f"iterations {fit.iterations}  evaluations {fit.evaluations}  "
-                 f"converged {int(fit.converged)}  time {fit.seconds:.3f} s")
+                 f"converged {int(fit.converged)} ({fit.stop or 'fixed values'})  "
+                 f"time {fit.seconds:.3f} s")
     return "\n".join(lines) + "\n"
 
 
@@ -339,6 +395,8 @@ def load_problems(ctl, base):
     sets = formats.read_phylip_sets(os.path.join(base, ctl["seqfile"]), ndata)
     txt = read_tree_text(os.path.join(base, ctl["treefile"]))
     out = []
+    if int(_num(ctl, "fix_blength", 0)) == 2 and ":" not in txt:
+        raise ValueError("fix_blength = 2: no branch lengths in tree")   # codeml.c:617
     for names, seqs in sets:
         tree = formats.parse_newick(txt, names)
         if len(tree.sons[tree.root]) < 3:
@@ -350,18 +408,51 @@ def load_problems(ctl, base):
     return out
 
 
-def _fixed_fit(eng, tree, vals, fix_omega, omega):
+def _fixed_fit(eng, tree, vals, fix_omega, omega, fixed_branches=False):
     """A Fit holding the likelihood at the fixed values of an in.codeml file
-    that starts with -1 (p0, p1 untransformed); no optimisation."""
+    that starts with -1 (p0, p1 untransformed; without branch lengths under
+    fix_blength = 2, which come from the tree); no optimisation."""
     eng.set_model(fix_omega, omega, p_transformed=False)
+    nb = tree.nbranch
+    if fixed_branches:
+        vals = np.concatenate([tree.branch_vector(), vals])
     negl = eng.lnl(vals[:eng.np])
     par = eng.last_params()
-    nb = tree.nbranch
     fit = Fit("H0" if fix_omega else "H1", -negl, vals[:eng.np].copy(), par[0], par[1], par[2],
-              par[3], par[4], par[7:11].copy(), vals[:nb].copy(), eng.np, 0, 1, True, 0.0)
+              par[3], par[4], par[7:11].copy(), vals[:nb].copy(),
+              eng.np - (nb if fixed_branches else 0), 0, 1, True, 0.0, {}, "",
+              fixed_branches)
     fit.x = engine.x_from_params(vals[:nb], par[0], par[1], par[2], par[3],
                                  None if fix_omega else par[4])
     return fit, vals[:eng.np].copy()
+
+
+def reference_pattern_order(data):
+    """Our pattern indices in the order codeml numbers its patterns:
+    PatternWeight (treesub.c:1302-1435) keeps them sorted by the text of the
+    column, the species' codon triplets one after another (strcmp order)."""
+    text = [formats.code_triplet(data, c) for c in range(data.ncodes)]
+    keys = ["".join(text[int(c)] for c in data.z[:, h]) for h in range(data.npatt)]
+    return sorted(range(data.npatt), key=lambda h: keys[h])
+
+
+def format_lnf(data, site_lnf, ntree=1, itree=1):
+    """Text of codeml's lnf file for one tree: the header (codeml.c:597),
+    the tree number (codeml.c:613) and, from the evaluation at the
+    estimates (codeml.c:832-834), one line per site pattern in codeml's
+    pattern order (print_lnf_site, treesub.c:7332-7337: pattern, count,
+    log f, f, ls f, then each species' codon and amino acid as print1site,
+    treesub.c:7647-7669)."""
+    out = [f"{ntree:6d} {data.ls:6d} {data.npatt:6d}\n", f"\n\n{itree:2d}\n"]
+    text = [formats.code_triplet(data, c) for c in range(data.ncodes)]
+    for k, h in enumerate(reference_pattern_order(data)):
+        lf = float(site_lnf[h])
+        f = math.exp(lf)
+        row = "".join(f"{text[int(c)]} ({formats.code_amino_acid(data, int(c))}) "
+                      for c in data.z[:, h])
+        out.append(f"\n{k + 1:6d} {data.fpatt[h]:6.0f} {lf:16.10f} {f:16.12f} "
+                   f"{data.ls * f:12.4f}  {row}")
+    return "".join(out)
 
 
 def _report(tree, data, names, fit, fix_omega):
@@ -389,6 +480,7 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
     check_ctl(ctl)
     base = os.path.dirname(os.path.abspath(ctl_path))
     problems = load_problems(ctl, base)
+    fixed_branches = int(_num(ctl, "fix_blength", 0)) == 2
     fix_omega = int(_num(ctl, "fix_omega", 0)) == 1
     omega = _num(ctl, "omega", 1.0)
     kappa = _num(ctl, "kappa", 2.0)
@@ -400,12 +492,21 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
             fixed_vals = vals
         else:
             start = vals
+    def full_start(tree, m):
+        # in.codeml under fix_blength = 2 holds no branch lengths
+        if start is None:
+            return None
+        if fixed_branches:
+            return np.concatenate([tree.branch_vector(), start[:m - tree.nbranch]])
+        return start[:m]
+
     fits = [None] * len(problems)
     if len(problems) > 1 and fixed_vals is None and gradient == "adjoint":
         nps = [t.nbranch + (4 if fix_omega else 5) for t, _, _ in problems]
         fits = fit_model_a_batch([(t, d) for t, d, _ in problems], fix_omega, omega,
-                                 [None if start is None else start[:m] for m in nps],
-                                 precision=precision, device=device, kappa=kappa)
+                                 [full_start(t, m) for (t, _, _), m in zip(problems, nps)],
+                                 precision=precision, device=device, kappa=kappa,
+                                 fixed_branches=fixed_branches)
     out_parts, rst_parts = [], []
     for k, (tree, data, names) in enumerate(problems):
         eng = engine.BranchSiteEngine(tree, data, precision=precision, device=device,
@@ -414,17 +515,36 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
             fit, xe = fits[k], None
             fixed_run = fixed_vals is not None
             if fixed_run:
-                fit, xe = _fixed_fit(eng, tree, fixed_vals, fix_omega, omega)
+                fit, xe = _fixed_fit(eng, tree, fixed_vals, fix_omega, omega, fixed_branches)
             elif fit is None:
                 fit = fit_model_a(tree, data, fix_omega, omega,
-                                  start=None if start is None else start[:eng.np],
-                                  gradient=gradient, kappa=kappa, eng=eng)
+                                  start=full_start(tree, eng.np), gradient=gradient,
+                                  kappa=kappa, eng=eng, fixed_branches=fixed_branches)
             if xe is None:
                 xe = fit.x
-            if int(_num(ctl, "getSE", 0)) >= 1:
-                fit.extra["se"] = standard_errors(eng, fit.x_untransformed(),
-                                                  _num(ctl, "Small_Diff", 1e-6))
-                eng.set_model(fix_omega, omega, p_transformed=not fixed_run)
+            # standard errors and the lnf values come from FP64 partials (the
+            # reported lnL of an FP32 fit is the FP64 one, and the per-site
+            # differences of 2e-6 steps would drown in FP32 rounding noise)
+            e64 = eng if eng.precision == "fp64" else engine.BranchSiteEngine(
+                tree, data, precision="fp64", device=device, fix_omega=fix_omega,
+                omega_fix=omega)
+            try:
+                if int(_num(ctl, "getSE", 0)) >= 1:
+                    xu = fit.x.copy()
+                    nb = tree.nbranch
+                    xu[nb + 1], xu[nb + 2] = fit.p0, fit.p1
+                    free = range(nb if fixed_branches else 0, len(xu))
+                    fit.extra["se"] = standard_errors(e64, xu, _num(ctl, "Small_Diff", 1e-6),
+                                                      free)
+                # the final evaluation at the estimates writes lnf
+                # (codeml.c:832-834)
+                e64.set_model(fix_omega, omega, p_transformed=not fixed_run)
+                e64.lnl(xe)
+                fit.extra["site_lnf"] = e64.fhK()[1].copy()
+            finally:
+                if e64 is not eng:
+                    e64.close()
+            eng.set_model(fix_omega, omega, p_transformed=not fixed_run)
             # site posteriors under H1, as the reference prints after the fit
             # (NEB codeml.c:5079-5129, BEB 6617-6826)
             if not fix_omega:
@@ -442,6 +562,11 @@ def run(ctl_path, precision="fp64", gradient="adjoint", device=0):
     if not fix_omega:
         with open(os.path.join(base, "rst"), "w") as f:
             f.write("".join(rst_parts))
+    # codeml opens lnf afresh for each data set (codeml.c:596), so the file
+    # holds the last one's
+    tree, data, _ = problems[-1]
+    with open(os.path.join(base, "lnf"), "w") as f:
+        f.write(format_lnf(data, fits[-1].extra["site_lnf"]))
     with open(os.path.join(base, ctl.get("outfile", "out")), "w") as f:
         f.write("".join(out_parts))
     return fits[0] if len(fits) == 1 else fits

@@ -401,7 +401,7 @@ extern "C" int bsa_batch_fit(bsa_batch *b, double *const *x, const double *const
         for (int i = 0; i < G; i++) {
             info[3 * i] = res[i].iterations;
             info[3 * i + 1] = res[i].evaluations + 1;
-            info[3 * i + 2] = res[i].converged;
+            info[3 * i + 2] = res[i].stop;
         }
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -472,10 +472,11 @@ extern "C" int bsa_batch_fit_lrt(bsa_batch *b, double *const *x1, double *const 
             int *o = info + 7 * i;
             o[0] = r1[i].iterations + (refit[i] ? r1b[i].iterations : 0);
             o[1] = r1[i].evaluations + (refit[i] ? r1b[i].evaluations : 0) + 1;
-            o[2] = refit[i] ? (r1[i].converged && r1b[i].converged) : r1[i].converged;
+            // stop reason of the H1 fit whose maximum was kept
+            o[2] = refit[i] && r1b[i].f < r1[i].f ? r1b[i].stop : r1[i].stop;
             o[3] = r0[i].iterations;
             o[4] = r0[i].evaluations;
-            o[5] = r0[i].converged;
+            o[5] = r0[i].stop;
             o[6] = refit[i];
         }
     }

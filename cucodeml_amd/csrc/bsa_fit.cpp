@@ -7,7 +7,8 @@
 // Method (an active-set projected BFGS; Nocedal & Wright, Numerical
 // Optimization, 2nd ed., sections 6.1 and 16.7):
 //   * x is kept inside the box [lo, hi]; a variable sitting on a bound whose
-//     gradient points out of the box is held there for the iteration;
+//     gradient points out of the box is held there for the iteration, and
+//     one with lo == hi is held there throughout;
 //   * direction d = -H g on the free variables, H the inverse-Hessian
 //     approximation (identity, scaled by s'y / y'y after the first step, as
 //     in N&W eq. 6.20);
@@ -16,15 +17,22 @@
 //     quadratic interpolation;
 //   * BFGS inverse update restricted to the free variables, skipped when the
 //     curvature s'y is not positive;
-//   * stop when two successive steps change f by less than e |f| and x by
-//     less than e |x| (the reference's H_end rule, tools.c:5724-5740),
-//     confirmed once more after restarting H from the scaled identity, or
-//     when the projected gradient vanishes or no descent step can be found
-//     along the scaled steepest descent, or when f has fallen by less than
-//     1e-3 e |f| over the last 50 iterations (a stall: on degenerate
-//     likelihood surfaces, e.g. p0 -> 1 where w2 and p1 no longer matter, x
-//     can keep drifting along a flat direction without an H_end stop; the
-//     1e-3 keeps slow but real progress along a ridge going).
+//   * stop (the reason is returned, BSA_STOP_*):
+//       BSA_STOP_HEND   two successive steps change f by less than e |f| and
+//                       x by less than e |x| (the reference's H_end rule,
+//                       tools.c:5724-5740), confirmed once more after
+//                       restarting H from the scaled identity;
+//       BSA_STOP_PGRAD  the projected gradient vanishes;
+//       BSA_STOP_STALL  f has fallen by less than 1e-3 e |f| over the last
+//                       50 iterations (on degenerate likelihood surfaces,
+//                       e.g. p0 -> 1 where w2 and p1 no longer matter, x can
+//                       keep drifting along a flat direction without an
+//                       H_end stop; the 1e-3 keeps slow but real progress
+//                       along a ridge going);
+//       BSA_STOP_NODESCENT  no descent step along the scaled steepest
+//                       descent (or a zero direction on the free variables);
+//       BSA_STOP_LIMIT  the iteration limit.
+//     Only the first two are convergence in the reference's sense.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -66,7 +74,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     reset_h(1.0);
     bool scaled = false, fresh = true;
     double gamma = 1.0;
-    int it = 0, good = 0, converged = 0;
+    int it = 0, good = 0, stop = BSA_STOP_LIMIT;
     bool confirmed = false;
     // f after the last kStall + 1 accepted steps (ring buffer)
     constexpr int kStall = 50;
@@ -78,11 +86,12 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         double pg = 0;
         for (int i = 0; i < n; i++) {
             const double tol = 1e-12 * (1 + std::fabs(lo[i]) + std::fabs(hi[i]));
-            act[i] = (x[i] <= lo[i] + tol && g[i] > 0) || (x[i] >= hi[i] - tol && g[i] < 0);
+            act[i] = lo[i] == hi[i] || (x[i] <= lo[i] + tol && g[i] > 0) ||
+                     (x[i] >= hi[i] - tol && g[i] < 0);
             pg = std::max(pg, std::fabs(clampd(x[i] - g[i], lo[i], hi[i]) - x[i]));
         }
         if (pg < 1e-10) {
-            converged = 1;
+            stop = BSA_STOP_PGRAD;
             break;
         }
         double gd = 0, dmax = 0;
@@ -104,7 +113,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         }
         for (int i = 0; i < n; i++) dmax = std::max(dmax, std::fabs(d[i]));
         if (!(dmax > 0)) {
-            converged = 1;
+            stop = BSA_STOP_NODESCENT;
             break;
         }
         double a = 1.0;
@@ -137,7 +146,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         }
         if (!ok) {
             if (fresh) {   // no progress even along the scaled steepest descent
-                converged = 1;
+                stop = BSA_STOP_NODESCENT;
                 break;
             }
             reset_h(gamma);
@@ -190,14 +199,14 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         fring[nacc % (kStall + 1)] = f;
         if (nacc >= kStall &&
             fring[(nacc - kStall) % (kStall + 1)] - f < 1e-3 * e * std::max(1.0, std::fabs(f))) {
-            converged = 1;
+            stop = BSA_STOP_STALL;
             it++;
             break;
         }
         good = small_step ? good + 1 : 0;
         if (good >= 2) {
             if (confirmed) {
-                converged = 1;
+                stop = BSA_STOP_HEND;
                 it++;
                 break;
             }
@@ -210,7 +219,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     res->f = f;
     res->iterations = it;
     res->evaluations = nfev;
-    res->converged = converged;
+    res->stop = stop;
     return BSA_OK;
 }
 
@@ -241,7 +250,7 @@ extern "C" int bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, con
     if (info) {
         info[0] = r.iterations;
         info[1] = r.evaluations + 1;
-        info[2] = r.converged;
+        info[2] = r.stop;
     }
     return BSA_OK;
 }
@@ -263,7 +272,7 @@ extern "C" int bsa_debug_minimize(int n, bsa_objective fn, void *ctx, double *x,
     if (info) {
         info[0] = r.iterations;
         info[1] = r.evaluations;
-        info[2] = r.converged;
+        info[2] = r.stop;
     }
     return BSA_OK;
 }

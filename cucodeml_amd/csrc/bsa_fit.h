@@ -11,7 +11,7 @@ using ObjFn = std::function<int(const double *x, double *f, double *g)>;
 
 struct FitResult {
     double f = 0;
-    int iterations = 0, evaluations = 0, converged = 0;
+    int iterations = 0, evaluations = 0, stop = BSA_STOP_LIMIT;
 };
 
 // minimise fg over the box [lo, hi] from x (moved into the box); x holds

@@ -57,6 +57,19 @@ class _FitOpts(C.Structure):
 
 
 BSA_GRAD_ADJOINT, BSA_GRAD_FD = 0, 1
+# why a fit stopped (BSA_STOP_*); only "hend" and "pgrad" are convergence
+STOP_NAMES = {0: "limit", 1: "hend", 2: "pgrad", 3: "stall", 4: "nodescent"}
+CONVERGED_STOPS = ("hend", "pgrad")
+
+
+def fit_info(iterations, evaluations, stop):
+    """Info dict of a fit: iterations, evaluations, the stop reason and
+    whether it is convergence (the H_end rule or a vanishing projected
+    gradient; a stall on a flat ridge, no descent step or the iteration
+    limit are not)."""
+    name = STOP_NAMES.get(int(stop), "unknown")
+    return {"iterations": int(iterations), "evaluations": int(evaluations), "stop": name,
+            "converged": name in CONVERGED_STOPS}
 _OBJECTIVE = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                          C.POINTER(C.c_double), C.c_void_p)
 
@@ -243,8 +256,7 @@ class BranchSiteEngine:
     def fit(self, x0, lo=None, hi=None, e=1e-8, max_iter=2000, max_step=5.0,
             gradient="adjoint"):
         """Bounded BFGS maximum-likelihood fit from x0 (replaces ming2 at
-        codeml.c:688).  Returns (x, -lnL, {"iterations", "evaluations",
-        "converged"})."""
+        codeml.c:688).  Returns (x, -lnL, fit_info dict)."""
         x = np.array(x0, dtype=np.float64)
         if len(x) != self.np:
             raise BsaError(f"x has {len(x)} values, the engine expects {self.np}")
@@ -257,8 +269,7 @@ class BranchSiteEngine:
         info = (C.c_int * 3)()
         _check(lib().bsa_fit(self._h, _dptr(x), len(x), _dptr(lo), _dptr(hi), C.byref(o),
                              C.byref(f), info))
-        return x, f.value, {"iterations": info[0], "evaluations": info[1],
-                            "converged": bool(info[2])}
+        return x, f.value, fit_info(info[0], info[1], info[2])
 
     def beb(self, x, sets=False):
         """Bayes empirical Bayes at x (bsa_beb): dict with the class
@@ -417,19 +428,30 @@ class BranchSiteBatch:
         _check(lib().bsa_batch_default_bounds(self._h, gene, _dptr(lo), _dptr(hi)))
         return lo, hi
 
-    def fit(self, x0s, e=1e-8, max_iter=2000, max_step=5.0):
+    def fit(self, x0s, e=1e-8, max_iter=2000, max_step=5.0, lo=None, hi=None):
         """ML fits of every gene from x0s[gene] (bounded BFGS, adjoint
-        gradient, codeml's bounds), one optimiser thread per gene, served in
-        rounds of the genes whose points are ready.  Returns (list of x, -lnL per
+        gradient), one optimiser thread per gene, served in rounds of the
+        genes whose points are ready.  lo / hi: per-gene bound vectors (None,
+        or None entries: codeml's bounds).  Returns (list of x, -lnL per
         gene, list of info dicts)."""
         xs = [np.array(x, dtype=np.float64) for x in self._xs(x0s)]
         o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
         out = np.zeros(len(xs))
         info = np.zeros(3 * len(xs), dtype=np.int32)
-        _check(lib().bsa_batch_fit(self._h, _ptrs(xs), None, None, C.byref(o), _dptr(out),
+
+        def bound_ptrs(b):
+            if b is None:
+                return None
+            arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in b]
+            keep.append(arrs)
+            dp = C.POINTER(C.c_double)
+            return (dp * len(arrs))(*[dp() if a is None else _dptr(a) for a in arrs])
+        keep = []
+        _check(lib().bsa_batch_fit(self._h, _ptrs(xs), bound_ptrs(lo), bound_ptrs(hi),
+                                   C.byref(o), _dptr(out),
                                    info.ctypes.data_as(C.POINTER(C.c_int))))
-        infos = [{"iterations": int(info[3 * i]), "evaluations": int(info[3 * i + 1]),
-                  "converged": bool(info[3 * i + 2])} for i in range(len(xs))]
+        infos = [fit_info(info[3 * i], info[3 * i + 1], info[3 * i + 2])
+                 for i in range(len(xs))]
         return xs, out, infos
 
     def fit_lrt(self, x1s, e=1e-8, max_iter=2000, max_step=5.0):
@@ -446,9 +468,14 @@ class BranchSiteBatch:
         info = np.zeros(7 * len(x1), dtype=np.int32)
         _check(lib().bsa_batch_fit_lrt(self._h, _ptrs(x1), _ptrs(x0), C.byref(o), _dptr(out),
                                        info.ctypes.data_as(C.POINTER(C.c_int))))
-        keys = ["h1_iterations", "h1_evaluations", "h1_converged", "h0_iterations",
-                "h0_evaluations", "h0_converged", "refit"]
-        infos = [{k: int(info[7 * i + j]) for j, k in enumerate(keys)} for i in range(len(x1))]
+        infos = []
+        for i in range(len(x1)):
+            o = info[7 * i:7 * i + 7]
+            d = {"refit": int(o[6])}
+            for hyp, k in (("h1", 0), ("h0", 3)):
+                for key, v in fit_info(o[k], o[k + 1], o[k + 2]).items():
+                    d[f"{hyp}_{key}"] = v
+            infos.append(d)
         return x1, x0, out.reshape(-1, 2), infos
 
     def fhK(self, gene):
@@ -516,7 +543,7 @@ def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0):
     info = (C.c_int * 3)()
     _check(lib().bsa_debug_minimize(n, cfun, None, _dptr(x), _dptr(lo), _dptr(hi), C.byref(o),
                                     C.byref(f), info))
-    return x, f.value, {"iterations": info[0], "evaluations": info[1], "converged": bool(info[2])}
+    return x, f.value, fit_info(info[0], info[1], info[2])
 
 
 def model_a_setup(kappa, p0, p1, w0, w2, pi):

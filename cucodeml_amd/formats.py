@@ -263,11 +263,10 @@ class CodonData:
         return m
 
 
-def site_amino_acid(data, species, site):
-    """Amino acid of `species` at `site` as codeml prints it (GetAASiteSpecies,
+def code_amino_acid(data, c):
+    """Amino acid of codon code c as codeml prints it (GetAASiteSpecies,
     codeml.c:4941-4971): '-' for a codon that allows every sense codon, '*'
     when the codons it allows code for more than one amino acid."""
-    c = int(data.z[species, data.pose[site]])
     if c < codons.NSENSE:
         return str(codons.AA61[c])
     allowed = data.amb_sets[c - codons.NSENSE]
@@ -275,6 +274,17 @@ def site_amino_acid(data, species, site):
         return "-"
     aas = {str(codons.AA61[k]) for k in allowed}
     return aas.pop() if len(aas) == 1 else "*"
+
+
+def code_triplet(data, c):
+    """Text of codon code c: the sense codon, or the ambiguous triplet as read
+    (codeml's CODONs table, EncodeSeqs treesub.c:1097-1131)."""
+    return codons.codon_str(c) if c < codons.NSENSE else data.amb_triplets[c - codons.NSENSE]
+
+
+def site_amino_acid(data, species, site):
+    """Amino acid of `species` at `site` as codeml prints it (code_amino_acid)."""
+    return code_amino_acid(data, int(data.z[species, data.pose[site]]))
 
 
 def encode_codons(names, seqs):

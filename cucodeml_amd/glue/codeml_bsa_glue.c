@@ -39,6 +39,14 @@ static void die(const char *what)
     exit(-1);
 }
 
+/* a control-file setting the engine does not compute: stop before any
+ * device call, naming the key */
+static void refuse(const char *key, int value, const char *why)
+{
+    fprintf(stderr, "\nbsa: %s = %d is not offloaded: %s\n", key, value, why);
+    exit(-1);
+}
+
 static void sync_opts(void)
 {
     bsa_model_opts o;
@@ -54,12 +62,22 @@ void CUDA_Init(double *x)
     int nn = tree.nnode, ns = com.ns, npatt = com.npatt;
     int *father = malloc(sizeof(int) * nn), *label = malloc(sizeof(int) * nn);
     int *ibranch = malloc(sizeof(int) * nn);
-    unsigned char *z = malloc((size_t)ns * npatt), *mask;
+    unsigned char *z, *mask;
     bsa_problem p;
     const char *prec = getenv("BSA_PRECISION");
     (void)x;
     if (com.seqtype != 1 || com.model != 2 || com.NSsites != 2)
         die("only branch-site model A (seqtype=1, model=2, NSsites=2) is offloaded");
+    /* the engine's states are the 61 sense codons of the universal code in
+     * TCAG order; another code numbers its codons differently */
+    if (com.icode != 0)
+        refuse("icode", com.icode, "only the universal genetic code (icode = 0) is implemented");
+    if (com.ncode != 61) refuse("ncode", com.ncode, "the engine has 61 sense codons");
+    if (com.ngene > 1) refuse("ngene", com.ngene, "one partition (Mgene = 0, no G option)");
+    if (com.clock) refuse("clock", com.clock, "only clock = 0 is implemented");
+    if (com.fix_blength == 2)
+        refuse("fix_blength", 2, "x carries no branch lengths; the engine expects them");
+    z = malloc((size_t)ns * npatt);
     for (i = 0; i < nn; i++) {
         father[i] = nodes[i].father;
         label[i] = i == tree.root ? 0 : (int)nodes[i].label;

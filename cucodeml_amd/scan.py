@@ -97,7 +97,8 @@ def fit_gene(gene, precision="fp64", gradient="adjoint", device=0, e=1e-8, graph
             "params1": [h1.kappa, h1.p0, h1.p1, h1.w0, h1.w2],
             "iterations": h1.iterations + h0.iterations,
             "evaluations": h1.evaluations + h0.evaluations,
-            "converged": h1.converged and h0.converged, "seconds": time.perf_counter() - t0}
+            "x0": h0.x.tolist(), "converged": h1.converged and h0.converged,
+            "stop": (h1.stop, h0.stop), "seconds": time.perf_counter() - t0}
 
 
 def scan(genes, workers=8, precision="fp64", gradient="adjoint", device=0, e=1e-8, graph=False):
@@ -136,13 +137,48 @@ def batch_groups(genes, batch_genes=64, max_waste=0.25):
     return groups
 
 
+def _finish_fp64(genes, x1, x0, device, e):
+    """FP64 finish of FP32 batch fits: H1 from the FP32 H1 estimates and H0
+    from the FP32 H0 estimates, on an FP64 batch engine, then H1 again from
+    the H0 estimates where H0 comes out above H1 (as bsa_batch_fit_lrt
+    does).  FP32 partials carry rounding noise of ~1e-9 |lnL| that ends a
+    fit at that floor; the finish takes the estimates to the FP64 maximum.
+    Returns (x1, x0, -lnL (n, 2), H1 params, per-gene finish info)."""
+    n = len(genes)
+    b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision="fp64",
+                               device=device)
+    try:
+        b.set_model(-1, True, 1.0)
+        x0f, f0, i0 = b.fit(x0, e=e)
+        b.set_model(-1, False)
+        x1f, f1, i1 = b.fit(x1, e=e)
+        refit = [i for i in range(n) if -f0[i] > -f1[i] + 1e-6]
+        if refit:
+            # one batch fit; the other genes are held at their maximum (lo = hi)
+            xs = [np.append(x0f[i], 1.0) if i in refit else x1f[i] for i in range(n)]
+            held = [None if i in refit else x1f[i] for i in range(n)]
+            xb, fb, ib = b.fit(xs, e=e, lo=held, hi=held)
+            for i in refit:
+                i1[i] = codeml._merge_info(i1[i], ib[i])
+                if fb[i] < f1[i]:
+                    x1f[i], f1[i] = xb[i], fb[i]
+        b.lnl(x1f)   # every gene's last evaluation at its H1 estimates
+        par1 = [b.last_params(i) for i in range(n)]
+    finally:
+        b.close()
+    info = [{"h1": i1[i], "h0": i0[i], "refit": i in refit} for i in range(n)]
+    return x1f, x0f, np.stack([f1, f0], axis=1), par1, info
+
+
 def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
     """H1 and H0 fits and the LRT of a list of genes on one batch engine
     (bsa_batch_fit_lrt): the sequence of ``fit_gene`` (H1, H0 from the H1
     estimates, H1 again from the H0 estimates where H0 lies above H1), each
     gene's chain in its own optimiser thread, so no gene waits for another
-    gene's fit.  Returns the per-gene dicts of ``fit_gene`` (seconds = the
-    batch's wall time)."""
+    gene's fit.  With precision "fp32" the chains run on FP32 partials and
+    are finished on FP64 ones (``_finish_fp64``); lnL, estimates and the
+    statistic are then the FP64 ones, as in ``fit_gene``.  Returns the
+    per-gene dicts of ``fit_gene`` (seconds = the batch's wall time)."""
     t0 = time.perf_counter()
     n = len(genes)
     b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision=precision,
@@ -158,25 +194,38 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
         par1 = [b.last_params(i) for i in range(n)]
     finally:
         b.close()
+    stages = [{"h1": {k[3:]: v for k, v in inf.items() if k.startswith("h1_")},
+               "h0": {k[3:]: v for k, v in inf.items() if k.startswith("h0_")},
+               "refit": bool(inf["refit"])} for inf in info]
+    first = stages
+    if precision == "fp32":
+        x1, x0, negl, par1, fin = _finish_fp64(genes, x1, x0, device, e)
+        stages = [{"h1": codeml._merge_info(a["h1"], c["h1"]),
+                   "h0": codeml._merge_info(a["h0"], c["h0"]),
+                   "refit": a["refit"] or c["refit"]} for a, c in zip(stages, fin)]
     wall = time.perf_counter() - t0
     out = []
     for i, g in enumerate(genes):
-        inf = info[i]
+        st = stages[i]
         h1 = codeml.Fit("H1", -negl[i, 0], x1[i], *par1[i][:5], par1[i][7:11].copy(),
-                        x1[i][:g.tree.nbranch].copy(), len(x1[i]), inf["h1_iterations"],
-                        inf["h1_evaluations"], bool(inf["h1_converged"]), wall, {})
+                        x1[i][:g.tree.nbranch].copy(), len(x1[i]), st["h1"]["iterations"],
+                        st["h1"]["evaluations"], st["h1"]["converged"], wall, {},
+                        st["h1"]["stop"])
         h0 = codeml.Fit("H0", -negl[i, 1], x0[i], 0, 0, 0, 0, 1.0, None, None, len(x0[i]),
-                        inf["h0_iterations"], inf["h0_evaluations"], bool(inf["h0_converged"]),
-                        wall, {})
+                        st["h0"]["iterations"], st["h0"]["evaluations"], st["h0"]["converged"],
+                        wall, {}, st["h0"]["stop"])
         r = codeml.lrt(h0, h1)
-        out.append({"gene": g.name, "ns": g.tree.ns, "npatt": g.data.npatt,
-                    "lnL1": h1.lnL, "lnL0": h0.lnL, "stat": r["stat"], "p": r["p_mixture"],
-                    "w2": h1.w2, "x1": h1.x.tolist(),
-                    "params1": [h1.kappa, h1.p0, h1.p1, h1.w0, h1.w2],
-                    "iterations": h1.iterations + h0.iterations,
-                    "evaluations": h1.evaluations + h0.evaluations,
-                    "converged": h1.converged and h0.converged, "refit": bool(inf["refit"]),
-                    "seconds": wall})
+        d = {"gene": g.name, "ns": g.tree.ns, "npatt": g.data.npatt,
+             "lnL1": h1.lnL, "lnL0": h0.lnL, "stat": r["stat"], "p": r["p_mixture"],
+             "w2": h1.w2, "x1": h1.x.tolist(), "x0": h0.x.tolist(),
+             "params1": [h1.kappa, h1.p0, h1.p1, h1.w0, h1.w2],
+             "iterations": h1.iterations + h0.iterations,
+             "evaluations": h1.evaluations + h0.evaluations,
+             "converged": h1.converged and h0.converged, "stop": (h1.stop, h0.stop),
+             "refit": st["refit"], "precision": precision, "seconds": wall}
+        if precision == "fp32":
+            d["fp32_evaluations"] = first[i]["h1"]["evaluations"] + first[i]["h0"]["evaluations"]
+        out.append(d)
     return out
 
 

@@ -210,13 +210,26 @@ typedef struct {
     int    gradient;          /* BSA_GRAD_ADJOINT or BSA_GRAD_FD             */
 } bsa_fit_opts;
 
+/* why a fit stopped (info[2] of bsa_fit); only BSA_STOP_HEND and
+ * BSA_STOP_PGRAD are convergence */
+#define BSA_STOP_LIMIT     0  /* iteration limit                              */
+#define BSA_STOP_HEND      1  /* two successive steps changed -lnL and x by
+                                 less than e relative, confirmed after a
+                                 restart of the Hessian approximation (the
+                                 reference's H_end rule, tools.c:5724-5740)   */
+#define BSA_STOP_PGRAD     2  /* projected gradient below 1e-10               */
+#define BSA_STOP_STALL     3  /* -lnL fell by less than 1e-3 e |lnL| over the
+                                 last 50 iterations (flat ridge)              */
+#define BSA_STOP_NODESCENT 4  /* no descent step along steepest descent       */
+
 /* Maximum-likelihood fit from the start x (np values, moved into [lo, hi]):
  * bounded BFGS (active set, projected line search).  Replaces
  * ming2(fout, &lnL, CUDA_lfundG, NULL, x, xb, space, e, np) at codeml.c:688
  * (ming2: tools.c:6361-6545).  On return x holds the estimates, *neg_lnl
- * = -lnL there, and info (may be NULL) = {iterations, evaluations,
- * converged (1) or iteration limit (0)}; the engine's last evaluation is
- * at x, so bsa_class_lnl / bsa_last_params describe the fit. */
+ * = -lnL there, and info (may be NULL) = {iterations, evaluations, stop
+ * reason BSA_STOP_*}; the engine's last evaluation is at x, so
+ * bsa_class_lnl / bsa_last_params describe the fit.  A variable with
+ * lo == hi stays fixed at that value. */
 int  bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, const double *hi,
              const bsa_fit_opts *opts, double *neg_lnl, int *info3);
 
@@ -281,8 +294,9 @@ int  bsa_batch_fit(bsa_batch *batch, double *const *x, const double *const *lo,
  * x1[gene] (np of H1): start in, H1 estimates out; x0[gene] (np - 1): H0
  * estimates out.  neg_lnl[2*gene] = -lnL at the H1 estimates,
  * neg_lnl[2*gene+1] at the H0 ones.  info[7*gene ..] (may be NULL): H1
- * iterations, evaluations, converged (both H1 fits summed), H0 iterations,
- * evaluations, converged, and 1 where H1 was fitted again.  Every gene's
+ * iterations and evaluations (both H1 fits summed), the stop reason of the
+ * H1 fit whose maximum was kept, H0 iterations, evaluations, stop reason,
+ * and 1 where H1 was fitted again.  Every gene's
  * last evaluation is left at its H1 estimates. */
 int  bsa_batch_fit_lrt(bsa_batch *batch, double *const *x1, double *const *x0,
                        const bsa_fit_opts *opts, double *neg_lnl, int *info);

@@ -8,7 +8,11 @@ import numpy as np
 from cucodeml_amd import formats
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NAMES = ["toy_amb", "cfg0", "cfg1", "cfg2s", "cfg4s"]
+NAMES = ["toy_amb", "cfg0", "cfg1", "cfg2s", "cfg4s", "ml16"]
+# two sister tips on branches of length 0 with different codons at some
+# sites: no class can produce those sites (8 taxa: no node scaling in the
+# reference; 40 taxa: the reference scales nodes)
+IMPOSSIBLE = ["imposs8", "imposs40"]
 
 
 def load(name):
@@ -31,6 +35,11 @@ def site_to_pattern(fx, data):
     for h in range(ls):
         col[z[:, h].tobytes()] = fx["fixed_site_lnf"][h]
     return np.array([col[data.z[:, h].tobytes()] for h in range(data.npatt)])
+
+
+def patterns_of_sites(fx, data, sites):
+    """Our pattern indices of the given alignment sites (0-based)."""
+    return sorted({int(data.pose[h]) for h in sites})
 
 
 def params(fx):

@@ -34,7 +34,8 @@ def test_check_ctl_accepts_model_a_rejects_others():
         good = codeml.read_ctl(write(d, "g.ctl", fixtures.CTL.format(fix_omega=0, omega=1.5)))
     codeml.check_ctl(good)
     for k, v in [("model", "0"), ("NSsites", "1"), ("CodonFreq", "3"), ("clock", "1"),
-                 ("cleandata", "1"), ("seqtype", "2"), ("fix_kappa", "1")]:
+                 ("cleandata", "1"), ("seqtype", "2"), ("fix_kappa", "1"), ("icode", "1"),
+                 ("fix_blength", "-1"), ("fix_blength", "3")]:
         bad = dict(good, **{k: v})
         with pytest.raises(ValueError):
             codeml.check_ctl(bad)
@@ -101,3 +102,48 @@ def test_load_problems_reads_ndata_sets_on_one_tree():
     # as a multiset and the same compressed site counts
     assert np.allclose(np.sort(t1.branch_vector()), np.sort(t2.branch_vector()))
     assert np.array_equal(np.sort(d1.fpatt), np.sort(d2.fpatt))
+
+
+def test_fix_blength_2_needs_branch_lengths():
+    fx, _, _ = fixtures.load("cfg0")
+    with tempfile.TemporaryDirectory() as d:
+        write(d, "seq.phy", fx["phylip"])
+        from cucodeml_amd import formats
+        names, _ = formats.read_phylip(fx["phylip"])
+        t = formats.parse_newick(fx["newick"], names)
+        write(d, "tree.nwk", t.newick(lengths=False))
+        ctl = fixtures.CTL.format(fix_omega=0, omega=1.5).replace("fix_blength = 0",
+                                                                  "fix_blength = 2")
+        c = codeml.read_ctl(write(d, "codeml.ctl", ctl))
+        codeml.check_ctl(c)
+        with pytest.raises(ValueError, match="no branch lengths"):
+            codeml.load_problems(c, d)
+        c["fix_blength"] = "1"
+        assert len(codeml.load_problems(c, d)) == 1
+
+
+def _lnf_rows(text):
+    rows = [ln.split() for ln in text.splitlines()]
+    return rows[0], [r for r in rows[1:] if len(r) > 5]
+
+
+def test_lnf_file_matches_reference_layout_and_order():
+    """The driver's lnf text (format_lnf) from the oracle's per-site values
+    against the reference's own lnf file for the same data and parameters
+    (ml16 fixed-parameter run): header, pattern order (PatternWeight's
+    sorted column text, treesub.c:1302-1435), counts and codon/amino-acid
+    columns exactly, the numbers to their print precision."""
+    from oracle import oracle
+    fx, tree, data = fixtures.load("ml16")
+    _, _, site = oracle.lnl(tree, data, *fixtures.params(fx))
+    got = codeml.format_lnf(data, site)
+    ref = fx["fixed_lnf_text"]
+    hg, rg = _lnf_rows(got)
+    hr, rr = _lnf_rows(ref)
+    assert hg == hr and len(rg) == len(rr) == data.npatt
+    assert got.count("\n") == ref.count("\n")
+    for a, b in zip(rg, rr):
+        assert a[0] == b[0] and a[1] == b[1] and a[5:] == b[5:]
+        assert abs(float(a[2]) - float(b[2])) <= 1e-10 * abs(float(b[2])) + 2e-10
+        assert abs(float(a[3]) - float(b[3])) <= 2e-12 + 1e-9 * float(b[3])
+        assert abs(float(a[4]) - float(b[4])) <= 1.5e-4

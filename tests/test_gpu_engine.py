@@ -214,3 +214,44 @@ def test_full_size_properties():
     c = e32.lnl(x)
     say(f"  fp32 - fp64 = {c - a:.3e}")
     assert abs(c - a) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", fixtures.IMPOSSIBLE)
+def test_impossible_sites(name, precision):
+    """Sites no class can produce (sister tips on zero-length branches with
+    different codons).  Every class's value there is log(1e-300), the
+    reference's floor (fx_r treesub.c:7524-7534), and the site's mixture is
+    log(1e-300) too.  Where the reference scales no node (8 taxa) that is
+    what it prints.  Where it scales nodes (40 taxa) it adds NodeScale's
+    scale logs to the floor, -800 for a scaled node whose partials are all
+    below 1e-300 (treesub.c:6956-6959), so its value depends on where it
+    scales; the engine's does not.  The difference at those sites is exactly
+    the oracle's restatement of those scale logs (the oracle with and
+    without the reference's node scaling), and every other site equals the
+    reference's."""
+    say(f"case: impossible sites {name} {precision}")
+    fx, tree, data = fixtures.load(name)
+    eng = make_engine(tree, data, precision=precision)
+    nl = eng.lnl(x_fixed(tree, fx))
+    fh, site = eng.fhK()
+    ref = fixtures.site_to_pattern(fx, data)
+    bad = np.zeros(data.npatt, dtype=bool)
+    bad[fixtures.patterns_of_sites(fx, data, fx["impossible_sites"])] = True
+    floor = np.log(1e-300)
+    assert np.all(fh[:, bad] == floor)
+    assert np.all(np.abs(site[bad] - floor) < 1e-12)
+    tol = 1e-10 if precision == "fp64" else 5e-6
+    ok = ~bad
+    assert np.all(np.abs(site[ok] - ref[ok]) <= tol * np.abs(ref[ok]) + tol)
+    _, _, scaled = oracle.lnl(tree, data, *fixtures.params(fx))
+    _, _, plain = oracle.lnl(tree, data, *fixtures.params(fx), use_scaling=False)
+    scale_logs = scaled[bad] - plain[bad]
+    say(f"  reference - engine at impossible sites: {ref[bad] - site[bad]}; "
+        f"oracle scale logs {scale_logs}")
+    assert np.all(np.abs((ref[bad] - site[bad]) - scale_logs) < 1e-9)
+    if name == "imposs8":
+        assert np.all(scale_logs == 0)
+    # lnL = the reference's, less the scale logs of the impossible sites
+    want = fx["fixed_lnL"] - np.dot(data.fpatt[bad], scale_logs)
+    assert abs(-nl - want) <= (6e-7 if precision == "fp64" else 2e-4)

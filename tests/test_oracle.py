@@ -191,3 +191,29 @@ def test_class_limits():
     L, fh, site = oracle.lnl(tree, data, k, 0.3, 0.7 - 1e-14, w0, w2)
     mix = np.log(0.3 * np.exp(fh[0]) + 0.7 * np.exp(fh[1]))
     assert np.max(np.abs(site - mix)) < 1e-9
+
+
+@pytest.mark.parametrize("name", fixtures.IMPOSSIBLE)
+def test_oracle_matches_reference_on_impossible_sites(name):
+    """Sites no class can produce (sister tips on zero-length branches with
+    different codons): the reference floors each class's f at 1e-300
+    (fx_r treesub.c:7524-7534) and, where it scales nodes, adds the scale
+    logs of NodeScale (treesub.c:6940-6968), -800 at a scaled node whose
+    partials are all below 1e-300 (which it then resets to 1).  The oracle
+    restates both and matches every site the reference prints; without
+    node scaling those sites are log(1e-300) exactly."""
+    fx, tree, data = fixtures.load(name)
+    ref = fixtures.site_to_pattern(fx, data)
+    L, _, site = oracle.lnl(tree, data, *fixtures.params(fx))
+    assert abs(L - fx["fixed_lnL"]) <= 6e-7 + 1e-12 * abs(L)
+    assert np.all(np.abs(site - ref) <= 1e-10 * np.abs(ref) + 1e-10)
+    bad = np.isin(np.arange(data.npatt), fixtures.patterns_of_sites(fx, data,
+                                                                    fx["impossible_sites"]))
+    assert bad.sum() >= 1
+    _, _, plain = oracle.lnl(tree, data, *fixtures.params(fx), use_scaling=False)
+    assert np.all(np.abs(plain[bad] - np.log(1e-300)) < 1e-12)
+    if name == "imposs8":
+        assert np.all(np.abs(ref[bad] - np.log(1e-300)) < 1e-9)
+    else:
+        # the reference's scale logs make these sites ~-800 lower
+        assert np.all(ref[bad] < np.log(1e-300) - 100)
