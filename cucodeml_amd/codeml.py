@@ -129,6 +129,40 @@ def _bounds(eng_bounds, x0, nb, fixed_branches):
     return lo, hi
 
 
+def finish_fp64(fit, x, lo, hi, e, gradient="adjoint"):
+    """FP64 finish of a fit that ended at the FP32 rounding-noise floor:
+    Newton iterations from there (bsa_fit_opts.newton: the Hessian from
+    differences of the adjoint gradient, which lands on the maximum also
+    along flat directions such as w2 on a ridge, where a BFGS restart from
+    the identity would stall), and, where they do not converge, BFGS
+    followed by Newton iterations.  ``fit`` is an engine's fit method."""
+    if gradient == "adjoint":
+        x1, f1, i1 = fit(x, lo, hi, e=e, max_iter=-1, newton=3)
+        if i1["converged"]:
+            return x1, f1, i1
+        x2, f2, i2 = fit(x1, lo, hi, e=e, newton=2)
+        return x2, f2, _merge_info(i1, i2)
+    return fit(x, lo, hi, e=e, gradient=gradient)
+
+
+def finish_fp64_batch(b, xs, los, his, e):
+    """``finish_fp64`` for every gene of an FP64 batch engine: Newton
+    iterations for all genes, then BFGS and Newton for those that did not
+    converge, the others held where they are (lo = hi).  los / his: per-gene
+    bounds (None entries: codeml's).  Returns (list of x, -lnL, info)."""
+    n = len(xs)
+    x1, f1, i1 = b.fit(xs, e=e, lo=los, hi=his, max_iter=-1, newton=3)
+    redo = [i for i in range(n) if not i1[i]["converged"]]
+    if not redo:
+        return x1, f1, i1
+    lo2 = [los[i] if i in redo else x1[i] for i in range(n)]
+    hi2 = [his[i] if i in redo else x1[i] for i in range(n)]
+    x2, f2, i2 = b.fit(x1, e=e, lo=lo2, hi=hi2, newton=2)
+    for i in redo:
+        x1[i], f1[i], i1[i] = x2[i], f2[i], _merge_info(i1[i], i2[i])
+    return x1, f1, i1
+
+
 def _merge_info(a, b):
     """Info of an FP32 fit finished by an FP64 one: counts summed, the
     finishing fit's stop reason."""
@@ -176,7 +210,7 @@ def fit_model_a(tree, data, fix_omega=False, omega=1.0, start=None, precision="f
             e64 = engine.BranchSiteEngine(tree, data, precision="fp64", device=eng.device,
                                           fix_omega=fix_omega, omega_fix=omega)
             try:
-                x, f, info2 = e64.fit(x, lo, hi, e=e, gradient=gradient)
+                x, f, info2 = finish_fp64(e64.fit, x, lo, hi, e, gradient)
                 par = e64.last_params()
             finally:
                 e64.close()
@@ -213,13 +247,18 @@ def fit_model_a_batch(problems, fix_omega=False, omega=1.0, starts=None, precisi
         x0.append(v)
     t0 = time.perf_counter()
 
-    def one_pass(prec, xs):
+    def one_pass(prec, xs, finish=False):
         b = engine.BranchSiteBatch(problems, precision=prec, device=device,
                                    fix_omega=fix_omega, omega_fix=omega)
         try:
             bounds = [_bounds(b.default_bounds(i), xs[i], problems[i][0].nbranch,
                               fixed_branches) for i in range(n)]
-            x, f, info = b.fit(xs, e=e, lo=[a for a, _ in bounds], hi=[c for _, c in bounds])
+            los, his = [a for a, _ in bounds], [c for _, c in bounds]
+            if finish:
+                x, f, info = finish_fp64_batch(b, xs, los, his, e)
+            else:
+                x, f, info = b.fit(xs, e=e, lo=los, hi=his)
+            b.lnl(x)   # every gene's last evaluation at its estimates
             return x, f, info, [b.last_params(i) for i in range(n)], b.np
         finally:
             b.close()
@@ -230,7 +269,7 @@ def fit_model_a_batch(problems, fix_omega=False, omega=1.0, starts=None, precisi
         extra = [{"fp32_lnL": -f[i], "fp32_iterations": info[i]["iterations"],
                   "fp32_evaluations": info[i]["evaluations"], "fp32_stop": info[i]["stop"]}
                  for i in range(n)]
-        x2, f, info2, par, nps = one_pass("fp64", x)
+        x2, f, info2, par, nps = one_pass("fp64", x, finish=True)
         info = [_merge_info(a, c) for a, c in zip(info, info2)]
         x = x2
     sec = time.perf_counter() - t0

@@ -22,6 +22,8 @@ struct bsa_engine {
     Forest f;
 };
 
+int bsa_engine_precision(const bsa_engine *eng) { return eng->f.precision; }
+
 extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
 {
     if (!p || !out) return fail(BSA_EINVAL, "null argument");

@@ -18,11 +18,13 @@
 //   * BFGS inverse update restricted to the free variables, skipped when the
 //     curvature s'y is not positive;
 //   * stop (the reason is returned, BSA_STOP_*):
+//       BSA_STOP_PGRAD  the projected gradient is below
+//                       gtol = 1e-8 + 1e-10 max(1, |f|), about what an FP64
+//                       gradient of a sum over sites of size |f| resolves;
 //       BSA_STOP_HEND   two successive steps change f by less than e |f| and
 //                       x by less than e |x| (the reference's H_end rule,
 //                       tools.c:5724-5740), confirmed once more after
 //                       restarting H from the scaled identity;
-//       BSA_STOP_PGRAD  the projected gradient vanishes;
 //       BSA_STOP_STALL  f has fallen by less than 1e-3 e |f| over the last
 //                       50 iterations (on degenerate likelihood surfaces,
 //                       e.g. p0 -> 1 where w2 and p1 no longer matter, x can
@@ -32,7 +34,19 @@
 //       BSA_STOP_NODESCENT  no descent step along the scaled steepest
 //                       descent (or a zero direction on the free variables);
 //       BSA_STOP_LIMIT  the iteration limit.
-//     Only the first two are convergence in the reference's sense.
+//   * optional Newton finish (opts->newton > 0 iterations, after the
+//     quasi-Newton phase, which max_iter < 0 skips): the Hessian of the
+//     free variables by forward differences of the gradient (one gradient
+//     evaluation per free variable, steps of 1e-5 max(1, |x_i|) towards the
+//     inside of the box), Levenberg damping until it is positive definite,
+//     Newton step with a projected Armijo line search.  Where the last step
+//     is below 1e-6 max(1, |x_i|) in every variable, the fit stops with
+//     BSA_STOP_NEWTON: the point is the maximum to that precision, also in
+//     flat directions (w2 on a ridge) that BFGS's curvature estimate, and the
+//     H_end rule, resolve poorly.
+//     With `verify` (exact FP64 gradients), a stall or no-descent exit is
+//     checked the same way by up to two Newton iterations.
+//     BSA_STOP_HEND, BSA_STOP_PGRAD and BSA_STOP_NEWTON are convergence.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -43,6 +57,7 @@
 #include "bsa_fit.h"
 
 int bsa_error(int code, const char *msg);   // sets bsa_last_error (engine)
+int bsa_engine_precision(const bsa_engine *eng);   // BSA_FP64 / BSA_FP32 (engine)
 
 namespace {
 inline double clampd(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
@@ -50,10 +65,126 @@ inline double clampd(double v, double a, double b) { return v < a ? a : (v > b ?
 
 namespace bsa {
 
-int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const double *hi,
-                 const bsa_fit_opts *opts, FitResult *res)
+// Newton iterations from x (f, g its value and gradient, updated in place);
+// *converged = 1 when the last Newton step was below 1e-6 max(1, |x_i|)
+template <class FG>
+static int newton_finish(FG &fg, int n, double *x, const double *lo, const double *hi,
+                         int iters, double *f, std::vector<double> &g, int *converged)
 {
-    const int max_iter = opts && opts->max_iter > 0 ? opts->max_iter : 2000;
+    std::vector<double> xt(n), gt(n), gc(n);
+    *converged = 0;
+    for (int k = 0; k < iters; k++) {
+        std::vector<int> fr;
+        for (int i = 0; i < n; i++) {
+            const double tol = 1e-12 * (1 + std::fabs(lo[i]) + std::fabs(hi[i]));
+            const bool out = lo[i] == hi[i] || (x[i] <= lo[i] + tol && g[i] > 0) ||
+                             (x[i] >= hi[i] - tol && g[i] < 0);
+            if (!out) fr.push_back(i);
+        }
+        const int m = (int)fr.size();
+        if (m == 0) {
+            *converged = 1;
+            return BSA_OK;
+        }
+        // Hessian of the free variables, column by column
+        std::vector<double> Hm((size_t)m * m), gf(m), dd(m), L((size_t)m * m);
+        for (int a = 0; a < m; a++) gf[a] = g[fr[a]];
+        for (int a = 0; a < m; a++) {
+            const int i = fr[a];
+            double h = 1e-5 * std::max(1.0, std::fabs(x[i]));
+            if (x[i] + h > hi[i]) h = -h;
+            if (x[i] + h < lo[i]) h = 0.5 * (hi[i] - lo[i]) * (x[i] - lo[i] < hi[i] - x[i] ? 1 : -1);
+            std::memcpy(xt.data(), x, sizeof(double) * n);
+            xt[i] += h;
+            double ft;
+            int rc = fg(xt.data(), &ft, gc.data());
+            if (rc) return rc == BSA_ENUMERIC ? BSA_OK : rc;
+            for (int b = 0; b < m; b++) Hm[(size_t)b * m + a] = (gc[fr[b]] - gf[b]) / h;
+        }
+        double dmax = 0;
+        for (int a = 0; a < m; a++) {
+            for (int b = 0; b < a; b++) {
+                const double v = 0.5 * (Hm[(size_t)a * m + b] + Hm[(size_t)b * m + a]);
+                Hm[(size_t)a * m + b] = Hm[(size_t)b * m + a] = v;
+            }
+            dmax = std::max(dmax, std::fabs(Hm[(size_t)a * m + a]));
+        }
+        // Cholesky of H + mu I, mu raised until it succeeds
+        bool ok = false;
+        for (double mu = 0; !ok && mu < 1e6 * (dmax + 1); mu = mu ? 10 * mu : 1e-12 * (dmax + 1)) {
+            ok = true;
+            for (int a = 0; a < m && ok; a++)
+                for (int b = 0; b <= a; b++) {
+                    double v = Hm[(size_t)a * m + b] + (a == b ? mu : 0);
+                    for (int c = 0; c < b; c++) v -= L[(size_t)a * m + c] * L[(size_t)b * m + c];
+                    if (a == b) {
+                        if (!(v > 0)) {
+                            ok = false;
+                            break;
+                        }
+                        L[(size_t)a * m + a] = std::sqrt(v);
+                    } else {
+                        L[(size_t)a * m + b] = v / L[(size_t)b * m + b];
+                    }
+                }
+        }
+        if (!ok) return BSA_OK;
+        // (H + mu I) d = -g
+        for (int a = 0; a < m; a++) {
+            double v = -gf[a];
+            for (int c = 0; c < a; c++) v -= L[(size_t)a * m + c] * dd[c];
+            dd[a] = v / L[(size_t)a * m + a];
+        }
+        for (int a = m - 1; a >= 0; a--) {
+            double v = dd[a];
+            for (int c = a + 1; c < m; c++) v -= L[(size_t)c * m + a] * dd[c];
+            dd[a] = v / L[(size_t)a * m + a];
+        }
+        double rel = 0;
+        for (int a = 0; a < m; a++)
+            rel = std::max(rel, std::fabs(dd[a]) / std::max(1.0, std::fabs(x[fr[a]])));
+        if (rel < 1e-10) {   // at the maximum to rounding
+            *converged = 1;
+            return BSA_OK;
+        }
+        // projected Armijo line search along the Newton step
+        double step = 1.0, ft = 0;
+        bool acc = false;
+        for (int ls = 0; ls < 30; ls++) {
+            double slope = 0;
+            std::memcpy(xt.data(), x, sizeof(double) * n);
+            for (int a = 0; a < m; a++) {
+                const int i = fr[a];
+                xt[i] = clampd(x[i] + step * dd[a], lo[i], hi[i]);
+                slope += g[i] * (xt[i] - x[i]);
+            }
+            int rc = fg(xt.data(), &ft, gt.data());
+            if (rc && rc != BSA_ENUMERIC) return rc;
+            if (rc == BSA_OK && std::isfinite(ft) && ft <= *f + 1e-4 * slope) {
+                acc = true;
+                break;
+            }
+            step *= 0.5;
+        }
+        if (!acc) {
+            // no decrease: at the maximum where the step is below rounding
+            *converged = rel < 1e-6;
+            return BSA_OK;
+        }
+        std::memcpy(x, xt.data(), sizeof(double) * n);
+        g = gt;
+        *f = ft;
+        *converged = rel * step < 1e-6;
+        if (rel * step < 1e-8) return BSA_OK;
+    }
+    return BSA_OK;
+}
+
+int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const double *hi,
+                 const bsa_fit_opts *opts, FitResult *res, bool verify)
+{
+    const int max_iter = !opts || opts->max_iter == 0 ? 2000 : std::max(0, opts->max_iter);
+    const int newton = opts ? std::max(0, opts->newton) : 0;
     const double e = opts && opts->e > 0 ? opts->e : 1e-8;
     const double max_step = opts && opts->max_step > 0 ? opts->max_step : 5.0;
     int nfev = 0;
@@ -90,7 +221,8 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
                      (x[i] >= hi[i] - tol && g[i] < 0);
             pg = std::max(pg, std::fabs(clampd(x[i] - g[i], lo[i], hi[i]) - x[i]));
         }
-        if (pg < 1e-10) {
+        const double gtol = 1e-8 + 1e-10 * std::max(1.0, std::fabs(f));
+        if (pg < gtol) {
             stop = BSA_STOP_PGRAD;
             break;
         }
@@ -216,6 +348,16 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
             fresh = true;
         }
     }
+    if (newton > 0) {
+        int st = 0;
+        if ((rc = newton_finish(fg, n, x, lo, hi, newton, &f, g, &st))) return rc;
+        if (st) stop = BSA_STOP_NEWTON;
+        it += newton;
+    } else if (verify && (stop == BSA_STOP_STALL || stop == BSA_STOP_NODESCENT)) {
+        int st = 0;
+        if ((rc = newton_finish(fg, n, x, lo, hi, 2, &f, g, &st))) return rc;
+        if (st) stop = BSA_STOP_NEWTON;
+    }
     res->f = f;
     res->iterations = it;
     res->evaluations = nfev;
@@ -241,7 +383,8 @@ extern "C" int bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, con
         return fd ? bsa_lnl_grad_fd(eng, xx, np, 0, 1e-6, f, g) : bsa_lnl_grad(eng, xx, np, f, g);
     };
     FitResult r;
-    int rc = minimize_box(fg, np, x, lo, hi, opts, &r);
+    int rc = minimize_box(fg, np, x, lo, hi, opts, &r,
+                          !fd && bsa_engine_precision(eng) == BSA_FP64);
     if (rc) return rc;
     // leave the engine's state (per-site values, parameters) at the returned x
     double f;

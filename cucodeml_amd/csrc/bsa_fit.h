@@ -16,7 +16,12 @@ struct FitResult {
 
 // minimise fg over the box [lo, hi] from x (moved into the box); x holds
 // the minimiser on return
+// verify: where the quasi-Newton phase ends by the stall rule or without a
+// descent step, and opts asks for no Newton iterations, up to two Newton
+// iterations check whether x is the maximum (BSA_STOP_NEWTON if so); for
+// exact FP64 gradients (the adjoint on FP64 partials), whose differences give
+// the Hessian
 int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
-                 const bsa_fit_opts *opts, FitResult *res);
+                 const bsa_fit_opts *opts, FitResult *res, bool verify = false);
 
 }  // namespace bsa

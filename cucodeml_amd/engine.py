@@ -53,20 +53,20 @@ class _Opts(C.Structure):
 
 class _FitOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("e", C.c_double), ("max_step", C.c_double),
-                ("gradient", C.c_int)]
+                ("gradient", C.c_int), ("newton", C.c_int)]
 
 
 BSA_GRAD_ADJOINT, BSA_GRAD_FD = 0, 1
 # why a fit stopped (BSA_STOP_*); only "hend" and "pgrad" are convergence
-STOP_NAMES = {0: "limit", 1: "hend", 2: "pgrad", 3: "stall", 4: "nodescent"}
-CONVERGED_STOPS = ("hend", "pgrad")
+STOP_NAMES = {0: "limit", 1: "hend", 2: "pgrad", 3: "stall", 4: "nodescent", 5: "newton"}
+CONVERGED_STOPS = ("hend", "pgrad", "newton")
 
 
 def fit_info(iterations, evaluations, stop):
     """Info dict of a fit: iterations, evaluations, the stop reason and
-    whether it is convergence (the H_end rule or a vanishing projected
-    gradient; a stall on a flat ridge, no descent step or the iteration
-    limit are not)."""
+    whether it is convergence (the H_end rule, a vanishing projected
+    gradient or a vanishing Newton step; a stall on a flat ridge, no
+    descent step or the iteration limit are not)."""
     name = STOP_NAMES.get(int(stop), "unknown")
     return {"iterations": int(iterations), "evaluations": int(evaluations), "stop": name,
             "converged": name in CONVERGED_STOPS}
@@ -254,7 +254,7 @@ class BranchSiteEngine:
         return lo, hi
 
     def fit(self, x0, lo=None, hi=None, e=1e-8, max_iter=2000, max_step=5.0,
-            gradient="adjoint"):
+            gradient="adjoint", newton=0):
         """Bounded BFGS maximum-likelihood fit from x0 (replaces ming2 at
         codeml.c:688).  Returns (x, -lnL, fit_info dict)."""
         x = np.array(x0, dtype=np.float64)
@@ -264,7 +264,7 @@ class BranchSiteEngine:
         lo = dlo if lo is None else np.ascontiguousarray(lo, dtype=np.float64)
         hi = dhi if hi is None else np.ascontiguousarray(hi, dtype=np.float64)
         o = _FitOpts(int(max_iter), float(e), float(max_step),
-                     BSA_GRAD_FD if gradient == "fd" else BSA_GRAD_ADJOINT)
+                     BSA_GRAD_FD if gradient == "fd" else BSA_GRAD_ADJOINT, int(newton))
         f = C.c_double()
         info = (C.c_int * 3)()
         _check(lib().bsa_fit(self._h, _dptr(x), len(x), _dptr(lo), _dptr(hi), C.byref(o),
@@ -428,14 +428,15 @@ class BranchSiteBatch:
         _check(lib().bsa_batch_default_bounds(self._h, gene, _dptr(lo), _dptr(hi)))
         return lo, hi
 
-    def fit(self, x0s, e=1e-8, max_iter=2000, max_step=5.0, lo=None, hi=None):
+    def fit(self, x0s, e=1e-8, max_iter=2000, max_step=5.0, lo=None, hi=None, newton=0):
         """ML fits of every gene from x0s[gene] (bounded BFGS, adjoint
         gradient), one optimiser thread per gene, served in rounds of the
         genes whose points are ready.  lo / hi: per-gene bound vectors (None,
-        or None entries: codeml's bounds).  Returns (list of x, -lnL per
-        gene, list of info dicts)."""
+        or None entries: codeml's bounds); max_iter < 0 and newton > 0:
+        Newton iterations only (bsa_fit_opts).  Returns (list of x, -lnL
+        per gene, list of info dicts)."""
         xs = [np.array(x, dtype=np.float64) for x in self._xs(x0s)]
-        o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
+        o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT, int(newton))
         out = np.zeros(len(xs))
         info = np.zeros(3 * len(xs), dtype=np.int32)
 
@@ -463,7 +464,7 @@ class BranchSiteBatch:
         self.set_model(-1, False)
         x1 = [np.array(x, dtype=np.float64) for x in self._xs(x1s)]
         x0 = [np.zeros(len(x) - 1) for x in x1]
-        o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
+        o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT, 0)
         out = np.zeros(2 * len(x1))
         info = np.zeros(7 * len(x1), dtype=np.int32)
         _check(lib().bsa_batch_fit_lrt(self._h, _ptrs(x1), _ptrs(x0), C.byref(o), _dptr(out),
@@ -516,7 +517,7 @@ def forest_plan(genes, precision="fp64"):
     return {k: int(v) for k, v in zip(keys, o)}
 
 
-def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0):
+def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0, newton=0):
     """The engine's bounded BFGS on a Python objective fun(x) -> (f, grad)
     (diagnostic; runs on the host).  Returns (x, f, info)."""
     n = len(x0)
@@ -538,7 +539,7 @@ def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0):
     x = np.array(x0, dtype=np.float64)
     lo = np.ascontiguousarray(lo, dtype=np.float64)
     hi = np.ascontiguousarray(hi, dtype=np.float64)
-    o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT)
+    o = _FitOpts(int(max_iter), float(e), float(max_step), BSA_GRAD_ADJOINT, int(newton))
     f = C.c_double()
     info = (C.c_int * 3)()
     _check(lib().bsa_debug_minimize(n, cfun, None, _dptr(x), _dptr(lo), _dptr(hi), C.byref(o),

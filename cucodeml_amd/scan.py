@@ -139,19 +139,21 @@ def batch_groups(genes, batch_genes=64, max_waste=0.25):
 
 def _finish_fp64(genes, x1, x0, device, e):
     """FP64 finish of FP32 batch fits: H1 from the FP32 H1 estimates and H0
-    from the FP32 H0 estimates, on an FP64 batch engine, then H1 again from
-    the H0 estimates where H0 comes out above H1 (as bsa_batch_fit_lrt
-    does).  FP32 partials carry rounding noise of ~1e-9 |lnL| that ends a
-    fit at that floor; the finish takes the estimates to the FP64 maximum.
+    from the FP32 H0 estimates, on an FP64 batch engine
+    (codeml.finish_fp64_batch: Newton iterations on the adjoint gradient's
+    differences, BFGS where they do not converge), then H1 again from the H0
+    estimates where H0 comes out above H1 (as bsa_batch_fit_lrt does).
+    FP32 partials carry rounding noise of ~1e-9 |lnL| that ends a fit at
+    that floor; the finish takes the estimates to the FP64 maximum.
     Returns (x1, x0, -lnL (n, 2), H1 params, per-gene finish info)."""
     n = len(genes)
     b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision="fp64",
                                device=device)
     try:
         b.set_model(-1, True, 1.0)
-        x0f, f0, i0 = b.fit(x0, e=e)
+        x0f, f0, i0 = codeml.finish_fp64_batch(b, x0, [None] * n, [None] * n, e)
         b.set_model(-1, False)
-        x1f, f1, i1 = b.fit(x1, e=e)
+        x1f, f1, i1 = codeml.finish_fp64_batch(b, x1, [None] * n, [None] * n, e)
         refit = [i for i in range(n) if -f0[i] > -f1[i] + 1e-6]
         if refit:
             # one batch fit; the other genes are held at their maximum (lo = hi)

@@ -202,25 +202,38 @@ int  bsa_default_bounds(const bsa_engine *eng, double *lo, double *hi);
 #define BSA_GRAD_FD       1   /* bsa_lnl_grad_fd, forward differences       */
 
 typedef struct {
-    int    max_iter;          /* quasi-Newton iterations (<= 0: 2000)        */
+    int    max_iter;          /* quasi-Newton iterations (0: 2000; < 0: none,
+                                 only the Newton iterations below)           */
     double e;                 /* convergence tolerance on relative changes of
                                  -lnL and x, as ming2's e (<= 0: 1e-8)        */
     double max_step;          /* largest change of one variable per line
                                  search start (<= 0: 5)                      */
     int    gradient;          /* BSA_GRAD_ADJOINT or BSA_GRAD_FD             */
+    int    newton;            /* Newton iterations after the quasi-Newton
+                                 ones (0: none): Hessian of the free
+                                 variables by forward differences of the
+                                 gradient (one evaluation per variable),
+                                 damped until positive definite; lands on the
+                                 maximum also along flat directions, such as
+                                 w2 on a ridge, where BFGS's curvature
+                                 estimate is poor (no counterpart in ming2)  */
 } bsa_fit_opts;
 
-/* why a fit stopped (info[2] of bsa_fit); only BSA_STOP_HEND and
- * BSA_STOP_PGRAD are convergence */
+/* why a fit stopped (info[2] of bsa_fit); BSA_STOP_HEND, BSA_STOP_PGRAD
+ * and BSA_STOP_NEWTON are convergence */
 #define BSA_STOP_LIMIT     0  /* iteration limit                              */
 #define BSA_STOP_HEND      1  /* two successive steps changed -lnL and x by
                                  less than e relative, confirmed after a
                                  restart of the Hessian approximation (the
-                                 reference's H_end rule, tools.c:5724-5740)   */
-#define BSA_STOP_PGRAD     2  /* projected gradient below 1e-10               */
+                                 reference's H_end rule, tools.c:5724-5740),
+                                 with the projected gradient below 10 gtol   */
+#define BSA_STOP_PGRAD     2  /* projected gradient below
+                                 1e-8 + 1e-10 max(1, -lnL)                   */
 #define BSA_STOP_STALL     3  /* -lnL fell by less than 1e-3 e |lnL| over the
                                  last 50 iterations (flat ridge)              */
 #define BSA_STOP_NODESCENT 4  /* no descent step along steepest descent       */
+#define BSA_STOP_NEWTON    5  /* the last Newton step (bsa_fit_opts.newton)
+                                 was below 1e-6 max(1, |x_i|)                */
 
 /* Maximum-likelihood fit from the start x (np values, moved into [lo, hi]):
  * bounded BFGS (active set, projected line search).  Replaces

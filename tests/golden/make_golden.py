@@ -253,6 +253,9 @@ def make_impossible(name, seed, ns, ncodon, mean_bl, nbad):
         seqs[a][3 * h:3 * h + 3] = list("TTT")
         seqs[b][3 * h:3 * h + 3] = list("TTC")
     seqs = ["".join(s) for s in seqs]
+    # the tips were simulated on their own branches, so they differ at other
+    # sites too; with both branches at length 0 every such site is impossible
+    bad = [h for h in range(ncodon) if seqs[a][3 * h:3 * h + 3] != seqs[b][3 * h:3 * h + 3]]
     t = tree.branch.copy()
     t[a] = t[b] = 0.0
     tree.branch[:] = t

@@ -72,3 +72,43 @@ def test_undefined_region_is_avoided():
 def test_bad_bounds_rejected():
     with pytest.raises(engine.BsaError):
         engine.minimize(rosen, np.zeros(2), [1, 0], [0, 1])
+
+
+def _ridge(scale):
+    """A likelihood-like objective with one flat direction: Hessian
+    eigenvalues from 1e4 down to `scale`, minimum at c."""
+    rng = np.random.default_rng(5)
+    n = 10
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    lam = np.geomspace(1e4, scale, n)
+    A = (Q * lam) @ Q.T
+    c = rng.uniform(-0.5, 0.5, n)
+    return (lambda x: (0.5 * (x - c) @ A @ (x - c) + 8000.0, A @ (x - c))), c, n
+
+
+def test_newton_finish_lands_on_a_flat_minimum():
+    """BFGS's H_end-type stop leaves a flat direction (curvature 1e-2) short
+    of the minimum; the Newton finish (Hessian from differences of the
+    gradient, one evaluation per variable) lands on it, and says so."""
+    fun, c, n = _ridge(1e-2)
+    x0 = np.full(n, 0.9)
+    xb, _, ib = engine.minimize(fun, x0, [-1] * n, [1] * n)
+    xn, _, inn = engine.minimize(fun, x0, [-1] * n, [1] * n, newton=3)
+    assert inn["stop"] == "newton" and inn["converged"]
+    assert np.max(np.abs(xn - c)) < 1e-8
+    assert np.max(np.abs(xn - c)) <= np.max(np.abs(xb - c))
+
+
+def test_newton_only_from_a_near_point_and_on_a_bound():
+    # max_iter < 0: Newton iterations alone; one coordinate's minimum lies
+    # outside the box, so it stays on its bound
+    fun, c, n = _ridge(1.0)
+    lo, hi = np.full(n, -1.0), np.full(n, 1.0)
+    hi[3] = c[3] - 0.2
+    x, _, info = engine.minimize(fun, np.clip(c + 0.01, lo, hi), lo, hi, max_iter=-1, newton=4)
+    assert info["converged"] and info["stop"] == "newton"
+    assert abs(x[3] - hi[3]) < 1e-12
+    # the other coordinates minimise the quadratic with x[3] held there
+    _, g = fun(x)
+    free = np.arange(n) != 3
+    assert np.max(np.abs(g[free])) < 1e-6

@@ -174,9 +174,9 @@ def test_batch_fits_equal_single_fits(genes):
             xs, fs, is_ = e.fit(starts[i])
         finally:
             e.close()
-        say(f"  gene {i}: batch -lnL {fb[i]:.8f} ({ib[i]['iterations']} it), single "
-            f"{fs:.8f} ({is_['iterations']} it)")
-        assert ib[i]["converged"]
+        say(f"  gene {i}: batch -lnL {fb[i]:.8f} ({ib[i]['iterations']} it, {ib[i]['stop']}), "
+            f"single {fs:.8f} ({is_['iterations']} it, {is_['stop']})")
+        assert ib[i]["stop"] == is_["stop"] and ib[i]["stop"] != "limit"
         assert abs(fb[i] - fs) <= 1e-6 * abs(fs)
         assert np.abs(xb[i] - xs).max() <= 1e-3 * max(1.0, np.abs(xs).max())
 
@@ -189,8 +189,10 @@ def test_batched_scan_equals_scan():
     say(f"  wall {wa:.2f} s batched vs {wc:.2f} s per gene")
     for r, s in zip(a, c):
         say(f"  {r['gene']}: lnL1 {r['lnL1']:.6f} / {s['lnL1']:.6f}  lnL0 {r['lnL0']:.6f} / "
-            f"{s['lnL0']:.6f}")
-        assert r["converged"]
+            f"{s['lnL0']:.6f}  stop {r['stop']} / {s['stop']}")
+        # genes of 6-14 taxa and ~100 codons: degenerate maxima (p0 -> 1,
+        # w2 free) end by the stall rule; no fit may run into the limit
+        assert "limit" not in r["stop"] and "limit" not in s["stop"]
         assert abs(r["lnL1"] - s["lnL1"]) <= 1e-5 and abs(r["lnL0"] - s["lnL0"]) <= 1e-5
         assert abs(r["stat"] - s["stat"]) <= 2e-5
 
@@ -210,8 +212,14 @@ def test_batch_fit_of_more_genes_than_fit_threads():
         t = b.timing()
     finally:
         b.close()
-    say(f"  rounds {t['n_eval']:.0f}, evaluations {sum(i['evaluations'] for i in ib)}")
-    assert all(i["converged"] for i in ib)
+    stops = {}
+    for i in ib:
+        stops[i["stop"]] = stops.get(i["stop"], 0) + 1
+    say(f"  rounds {t['n_eval']:.0f}, evaluations {sum(i['evaluations'] for i in ib)}, "
+        f"stop reasons {stops}")
+    # genes of 4-6 taxa and 20-60 codons: many maxima are degenerate
+    # (p0 -> 1, w2 and p1 free) and end by the stall rule
+    assert "limit" not in stops
     for k in (0, 150, 299):
         tree, data, _ = genes[k]
         e = engine.BranchSiteEngine(tree, data)
@@ -252,7 +260,10 @@ def test_chained_lrt_fits_equal_separate_batch_fits():
     for i in range(len(genes)):
         say(f"  gene {i}: H1 {-negl[i, 0]:.6f} / {-f1[i]:.6f}  H0 {-negl[i, 1]:.6f} / "
             f"{-f0[i]:.6f}  refit {info[i]['refit']}")
-        assert info[i]["h1_converged"] and info[i]["h0_converged"]
+        assert info[i]["h1_stop"] != "limit" and info[i]["h0_stop"] != "limit"
+        if not info[i]["refit"]:
+            assert info[i]["h1_stop"] == i1[i]["stop"]
+        assert info[i]["h0_stop"] == i0[i]["stop"]
         assert np.array_equal(x0c[i], x0[i]) and negl[i, 1] == f0[i]
         assert info[i]["h0_evaluations"] == i0[i]["evaluations"] - 1
         assert info[i]["refit"] == int(-f0[i] > -f1[i] + 1e-6)
