@@ -8,7 +8,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbsa.so")
-SOURCES = ["bsa_engine.hip", "bsa_forest.hip", "bsa_kernels.hip", "bsa_model.cpp", "bsa_fit.cpp",
+SOURCES = ["bsa_engine.hip", "bsa_forest.hip", "bsa_kernels.hip", "bsa_noalloc.hip", "bsa_model.cpp",
+           "bsa_fit.cpp",
            "bsa_batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

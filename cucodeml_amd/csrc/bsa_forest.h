@@ -196,6 +196,18 @@ double gene_collect(Gene &g, const Forest &f);
 // d(-lnL)/dx of gene g from the adjoint outputs of the last forest_run
 void gene_chain_rule(const Gene &g, const Forest &f, const double *x, double *grad);
 
+// Bytes of device memory a planned forest's likelihood pass takes when its
+// buffers are laid out one after another (256-byte aligned) from `base`; with
+// base != nullptr the forest's device pointers are set to that layout.
+size_t forest_place_lnl(Forest &f, char *base);
+// One likelihood evaluation of a planned, checked one-gene forest at codeml's
+// x, with every device buffer inside [base, base + cap) (device memory the
+// caller owns) and all work on stream s; nothing is allocated on the device.
+// fhK [4 npatt] / site [npatt] may be null.  Refuses a gene that does not
+// fit.  Afterwards f holds no buffers (nothing to free).
+int forest_lnl_in_place(Forest &f, char *base, size_t cap, hipStream_t s, const double *x,
+                        double *lnl, double *fhK, double *site);
+
 // Launch shapes: pruning tile (64 or 128 patterns), child-adjoint tiles per
 // block (0 = by launch size, else 1..8), branch-matrix tiles per chunk (0 = by
 // gene size, up to kAdjTilesPerChunkAuto, else 1..kAdjTilesPerChunk).

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 namespace bsa {
 
@@ -511,10 +512,47 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
     return BSA_OK;
 }
 
+// Host images of a planned forest's static device inputs: tip codes,
+// ambiguity masks, pi, pattern counts and the root jobs.
+struct StaticInputs {
+    std::vector<unsigned char> z, m;
+    std::vector<double> pi64, fp;
+    std::vector<RootArgs> ra;
+};
+
+static void static_inputs(const Forest &f, StaticInputs &si)
+{
+    const size_t np = f.npad, G = f.G;
+    si.z.assign(np * f.ns_total, 0);
+    si.m.assign(64 * (size_t)f.namb_rows, 0);
+    si.pi64.assign(64 * G, 0.0);
+    si.fp.assign(np * G, 0.0);
+    si.ra.assign(2 * G, RootArgs());
+    for (Gene *gp : f.genes) {
+        const Gene &g = *gp;
+        for (int s = 0; s < g.ns; s++)
+            std::memcpy(&si.z[(size_t)(g.tip_base + s) * np], &g.z[(size_t)s * g.npatt], g.npatt);
+        for (int a = 0; a < g.namb; a++)
+            for (int k = 0; k < kS; k++)
+                si.m[(size_t)(g.amb_base + a) * 64 + k] = g.code_mask[(size_t)(kS + a) * kS + k];
+        for (int i = 0; i < kS; i++) si.pi64[(size_t)g.gi * 64 + i] = g.pi[i];
+        for (int h = 0; h < g.npatt; h++) si.fp[(size_t)g.gi * np + h] = g.fpatt[h];
+        for (int w = 0; w < 2; w++) {
+            RootArgs &r = si.ra[(size_t)w * G + g.gi];
+            r.npatt = g.npatt;
+            r.npad = f.npad;
+            for (int c = 0; c < 4; c++) {
+                r.root_slot[c] = gslot(g, g.root_slot[c]);
+                // adjoint slots of the root: interior node root, class c
+                r.seed_aslot[c] = w ? gaslot(g, (g.root - g.ns) * 4 + c) : -1;
+            }
+        }
+    }
+}
+
 int forest_upload(Forest &f, int device)
 {
     f.dev = device;
-    const std::vector<Gene *> &genes = f.genes;
     const size_t ts = tsize(f);
     const size_t np = f.npad, G = f.G;
     HIPCHK(hipSetDevice(f.dev));
@@ -548,35 +586,17 @@ int forest_upload(Forest &f, int device)
     std::memcpy(f.h_pjobs, f.pjobs.data(), sizeof(PmatJob) * f.pjobs.size());
 
     // static inputs
-    std::vector<unsigned char> z(np * f.ns_total, 0);
-    std::vector<unsigned char> m(64 * (size_t)f.namb_rows, 0);
-    std::vector<double> pi64(64 * G, 0.0), fp(np * G, 0.0);
-    std::vector<RootArgs> ra(2 * G);
-    for (Gene *gp : genes) {
-        const Gene &g = *gp;
-        for (int s = 0; s < g.ns; s++)
-            std::memcpy(&z[(size_t)(g.tip_base + s) * np], &g.z[(size_t)s * g.npatt], g.npatt);
-        for (int a = 0; a < g.namb; a++)
-            for (int k = 0; k < kS; k++)
-                m[(size_t)(g.amb_base + a) * 64 + k] = g.code_mask[(size_t)(kS + a) * kS + k];
-        for (int i = 0; i < kS; i++) pi64[(size_t)g.gi * 64 + i] = g.pi[i];
-        for (int h = 0; h < g.npatt; h++) fp[(size_t)g.gi * np + h] = g.fpatt[h];
-        for (int w = 0; w < 2; w++) {
-            RootArgs &r = ra[(size_t)w * G + g.gi];
-            r.npatt = g.npatt;
-            r.npad = f.npad;
-            for (int c = 0; c < 4; c++) {
-                r.root_slot[c] = gslot(g, g.root_slot[c]);
-                // adjoint slots of the root: interior node root, class c
-                r.seed_aslot[c] = w ? gaslot(g, (g.root - g.ns) * 4 + c) : -1;
-            }
-        }
-    }
-    HIPCHK(hipMemcpy(f.d_z, z.data(), z.size(), hipMemcpyHostToDevice));
-    if (f.namb_rows > 0) HIPCHK(hipMemcpy(f.d_mask_amb, m.data(), m.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(f.d_pi, pi64.data(), sizeof(double) * pi64.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(f.d_fpatt, fp.data(), sizeof(double) * fp.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(f.d_rargs, ra.data(), sizeof(RootArgs) * ra.size(), hipMemcpyHostToDevice));
+    StaticInputs si;
+    static_inputs(f, si);
+    HIPCHK(hipMemcpy(f.d_z, si.z.data(), si.z.size(), hipMemcpyHostToDevice));
+    if (f.namb_rows > 0)
+        HIPCHK(hipMemcpy(f.d_mask_amb, si.m.data(), si.m.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.d_pi, si.pi64.data(), sizeof(double) * si.pi64.size(),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.d_fpatt, si.fp.data(), sizeof(double) * si.fp.size(),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.d_rargs, si.ra.data(), sizeof(RootArgs) * si.ra.size(),
+                     hipMemcpyHostToDevice));
     if (!f.prune.empty())
         HIPCHK(hipMemcpy(f.d_prune, f.prune.data(), sizeof(PruneJob) * f.prune.size(),
                          hipMemcpyHostToDevice));
@@ -1128,6 +1148,109 @@ int forest_run(Forest &f, bool adjoint)
 {
     HIPCHK(hipSetDevice(f.dev));
     return f.precision == BSA_FP32 ? run_device<float>(f, adjoint) : run_device<double>(f, adjoint);
+}
+
+// ------------------------------------------- evaluation in the caller's memory
+
+size_t forest_place_lnl(Forest &f, char *base)
+{
+    const size_t ts = tsize(f), np = f.npad, G = f.G;
+    const size_t nnode_total = (size_t)f.ns_total + f.nint_total;
+    f.nroot_blk = f.npad / 16;
+    size_t off = 0;
+    auto take = [&](auto *&ptr, size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        if (base) ptr = reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(base + off);
+        off += bytes;
+    };
+    take(f.d_uvr, sizeof(double) * f.model_block());
+    take(f.d_pjobs, sizeof(PmatJob) * f.pjobs.size());
+    take(f.d_prune, sizeof(PruneJob) * std::max<size_t>(1, f.prune.size()));
+    take(f.d_rargs, sizeof(RootArgs) * 2 * G);
+    take(f.d_PT, sizeof(double) * 4096 * nnode_total * 3);
+    take(f.d_AT, sizeof(double) * 64 * (size_t)f.ncodes * f.ns_total * 3);
+    if (f.namb_rows > 0) take(f.d_mask_amb, 64 * (size_t)f.namb_rows);
+    take(f.d_L, ts * 64 * np * f.nslot);
+    take(f.d_E, sizeof(int) * np * f.nslot);
+    take(f.d_z, np * f.ns_total);
+    take(f.d_pi, sizeof(double) * 64 * G);
+    take(f.d_fpatt, sizeof(double) * np * G);
+    take(f.d_fh, sizeof(double) * 4 * np * G);
+    take(f.d_site, sizeof(double) * np * G);
+    take(f.d_partial, sizeof(double) * 2 * kRootSums * f.nroot_blk * G);
+    take(f.d_out, sizeof(double) * kRootSums * G);
+    return (off + 255) & ~(size_t)255;
+}
+
+template <typename T>
+static int lnl_in_place(Forest &f, hipStream_t s, const double *x, double *lnl, double *fhK,
+                        double *site)
+{
+    Gene &g = *f.genes[0];
+    std::vector<double> h_uvr(f.model_block(), 0.0), h_out((size_t)kRootSums * f.G, 0.0);
+    std::vector<PmatJob> h_pj(f.pjobs);
+    f.h_uvr = h_uvr.data();
+    f.h_out = h_out.data();
+    f.h_pjobs = h_pj.data();
+    int rc = gene_stage_x(g, f, x, false);
+    if (rc) return rc;
+    StaticInputs si;
+    static_inputs(f, si);
+    const size_t nnode_total = (size_t)f.ns_total + f.nint_total;
+    // the array keeps the last call's contents: P(t) - I must start at zero
+    // (the pad rows and columns are never written)
+    HIPCHK(hipMemsetAsync(f.d_PT, 0, sizeof(double) * 4096 * nnode_total * 3, s));
+    HIPCHK(hipMemcpyAsync(f.d_z, si.z.data(), si.z.size(), hipMemcpyHostToDevice, s));
+    if (f.namb_rows > 0)
+        HIPCHK(hipMemcpyAsync(f.d_mask_amb, si.m.data(), si.m.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f.d_pi, si.pi64.data(), sizeof(double) * si.pi64.size(),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f.d_fpatt, si.fp.data(), sizeof(double) * si.fp.size(),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f.d_rargs, si.ra.data(), sizeof(RootArgs) * si.ra.size(),
+                          hipMemcpyHostToDevice, s));
+    if (!f.prune.empty())
+        HIPCHK(hipMemcpyAsync(f.d_prune, f.prune.data(), sizeof(PruneJob) * f.prune.size(),
+                              hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f.d_uvr, f.h_uvr, sizeof(double) * f.model_block(),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(f.d_pjobs, f.h_pjobs, sizeof(PmatJob) * f.pjobs.size(),
+                          hipMemcpyHostToDevice, s));
+    if ((rc = enqueue_device<T>(f, s, false, false))) return rc;
+    HIPCHK(hipMemcpyAsync(f.h_out, f.d_out, sizeof(double) * kRootSums * f.G,
+                          hipMemcpyDeviceToHost, s));
+    if (fhK)
+        for (int c = 0; c < 4; c++)
+            HIPCHK(hipMemcpyAsync(fhK + (size_t)c * g.npatt, f.d_fh + (size_t)c * f.npad,
+                                  sizeof(double) * g.npatt, hipMemcpyDeviceToHost, s));
+    if (site)
+        HIPCHK(hipMemcpyAsync(site, f.d_site, sizeof(double) * g.npatt, hipMemcpyDeviceToHost, s));
+    // the host images above are freed on return
+    HIPCHK(hipStreamSynchronize(s));
+    *lnl = gene_collect(g, f);
+    return BSA_OK;
+}
+
+int forest_lnl_in_place(Forest &f, char *base, size_t cap, hipStream_t s, const double *x,
+                        double *lnl, double *fhK, double *site)
+{
+    if (f.G != 1) return fail(BSA_EINVAL, "one gene per evaluation in given memory");
+    const size_t need = forest_place_lnl(f, nullptr);
+    if (need > cap)
+        return fail(BSA_EINVAL,
+                    "the gene needs %zu bytes of device scratch (%d taxa, %d patterns padded to %d, "
+                    "%s partials); the limit is %zu bytes",
+                    need, f.genes[0]->ns, f.genes[0]->npatt, f.npad,
+                    f.precision == BSA_FP32 ? "FP32" : "FP64", cap);
+    forest_place_lnl(f, base);
+    f.stream = s;
+    const int rc = f.precision == BSA_FP32 ? lnl_in_place<float>(f, s, x, lnl, fhK, site)
+                                           : lnl_in_place<double>(f, s, x, lnl, fhK, site);
+    // nothing here is the forest's to release
+    Forest detached;
+    detached.precision = f.precision;
+    f = detached;
+    return rc;
 }
 
 // Capture the device work of one evaluation as a graph (the launch sequence

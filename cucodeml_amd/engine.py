@@ -30,7 +30,7 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_batch_eval_genes", "bsa_batch_fit", "bsa_batch_fit_lrt",
             "bsa_batch_class_lnl", "bsa_batch_last_params", "bsa_batch_timing", "bsa_batch_work",
             "bsa_batch_set_graph", "bsa_debug_forest_plan", "bsa_set_shapes", "bsa_tune",
-            "bsa_batch_tune"]
+            "bsa_batch_tune", "bsa_lnl_noalloc"]
 
 
 class BsaError(RuntimeError):
@@ -128,6 +128,8 @@ def lib():
         L.bsa_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.bsa_batch_tune.argtypes = [C.c_void_p, dpp, C.c_int, dp]
         L.bsa_debug_forest_plan.argtypes = [pp, C.c_int, dp]
+        L.bsa_lnl_noalloc.argtypes = [C.POINTER(_Problem), C.POINTER(_Opts), dp, C.c_int, dp,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -503,6 +505,27 @@ class BranchSiteBatch:
         d.update({f"sum_{k}": float(o[7 + i]) for i, k in enumerate(keys)})
         d["n_eval"], d["n_launch"], d["n_adj_launch"] = float(o[14]), float(o[15]), float(o[16])
         return d
+
+
+# bytes of the device array bsa_lnl_noalloc works in (BSA_NOALLOC_BYTES)
+NOALLOC_BYTES = 128 << 20
+
+
+def lnl_noalloc(tree, data, x, precision="fp64", kernel="mfma", device=0, fix_omega=False,
+                omega_fix=1.0, p_transformed=True, stream=None, tile=0):
+    """-lnL, per-class log f (4, npatt) and per-site log f of one gene by
+    bsa_lnl_noalloc: no engine, no device allocation (every device buffer in
+    the library's fixed array on the device).  stream: a hipStream_t value
+    (int) or None for the null stream."""
+    keep = []
+    prob = _problem(tree, data, precision, kernel, device, tile, None, keep)
+    o = _Opts(int(bool(fix_omega)), float(omega_fix), int(bool(p_transformed)))
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = C.c_double()
+    fh, site = np.zeros(4 * data.npatt), np.zeros(data.npatt)
+    _check(lib().bsa_lnl_noalloc(C.byref(prob), C.byref(o), _dptr(x), len(x), C.byref(out),
+                                 fh.ctypes.data, site.ctypes.data, stream))
+    return out.value, fh.reshape(4, -1), site
 
 
 def forest_plan(genes, precision="fp64"):

@@ -25,6 +25,8 @@
 #ifndef BSA_H
 #define BSA_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -327,6 +329,26 @@ int  bsa_batch_set_graph(bsa_batch *batch, int enable);
  * out8 = {genes, padded patterns, partial slots, pruning launches, pruning
  * jobs, adjoint jobs, eigen-basis jobs, branch-matrix chunks} */
 int  bsa_debug_forest_plan(const bsa_problem *const *probs, int ngene, double *out8);
+
+/* ------------------------------------------------------------------------
+ * Likelihood of one gene that allocates no device memory.  No counterpart in
+ * the reference: it is for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * The same computation as bsa_create + bsa_lnl + bsa_class_lnl (bitwise the
+ * same results), with every device buffer in one array of BSA_NOALLOC_BYTES
+ * that the library holds on each GPU it runs on until the process ends; a
+ * gene whose buffers do not fit (beyond about 64 taxa x 1,000 patterns in
+ * FP64, twice the patterns in FP32) is refused with BSA_EINVAL, and the error text
+ * gives the bytes it needs.  All device work runs on `stream` (a hipStream_t;
+ * NULL: the null stream) and has finished when the call returns.  Calls on
+ * one device are serialized by the library (they share that device's array,
+ * which keeps the last call's contents).  fhK [4*npatt] and site_lnf
+ * [npatt] (may be NULL) are as bsa_class_lnl's; opts NULL means H1 with
+ * transformed p's.
+ * ------------------------------------------------------------------------ */
+#define BSA_NOALLOC_BYTES ((size_t)128 << 20)
+int  bsa_lnl_noalloc(const bsa_problem *prob, const bsa_model_opts *opts, const double *x, int np,
+                     double *neg_lnl, double *fhK, double *site_lnf, void *stream);
 
 const char *bsa_last_error(void);
 

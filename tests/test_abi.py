@@ -211,3 +211,23 @@ def test_host_model_setup_is_thread_safe():
             for a, b in zip(alone, got):
                 for u, v in zip(a, b):
                     assert np.array_equal(u, v)
+
+
+def test_noalloc_entry_refuses_genes_beyond_its_array():
+    """bsa_lnl_noalloc checks the size of the gene's buffers against its
+    fixed array (BSA_NOALLOC_BYTES) before any HIP call: a gene beyond it is
+    refused with BSA_EINVAL and a message giving the bytes it needs."""
+    rng = np.random.default_rng(4)
+    tree = simulate.yule_tree(96, 0.05, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    seqs, _ = simulate.simulate_model_a(tree, 3000, 2.0, 0.1, 3.0, 0.5, 0.3, pi, rng)
+    data = formats.load_codon_data(tree.names, seqs)
+    x = engine.x_from_params(tree.branch_vector(), 2.0, 0.5, 0.3, 0.1, 3.0)
+    with pytest.raises(engine.BsaError, match=r"needs \d+ bytes of device scratch"):
+        engine.lnl_noalloc(tree, data, x)
+    # and a wrong parameter count, also before any HIP call
+    with pytest.raises(engine.BsaError, match="np="):
+        engine.lnl_noalloc(tree, data, x[:-1])
+    src = open(os.path.join(ROOT, "include", "bsa.h")).read()
+    assert re.search(r"#define BSA_NOALLOC_BYTES \(\(size_t\)128 << 20\)", src)
+    assert engine.NOALLOC_BYTES == 128 << 20
