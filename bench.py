@@ -256,7 +256,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
     ap.add_argument("--gradient", choices=["adjoint", "fd"], default="adjoint")
-    ap.add_argument("--kernel", choices=["mfma", "valu"], default="mfma")
+    ap.add_argument("--kernel", choices=["mfma", "mfma32", "valu"], default=None,
+                    help="default: mfma32 with fp32 partials (FP32 matrix-core child adjoints), "
+                         "mfma with fp64")
     ap.add_argument("--tile", type=int, default=0, help="patterns per pruning block (64/128)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the default launch shapes (no autotuning)")
@@ -285,6 +287,7 @@ def main():
               "is described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
 
+    args.kernel = args.kernel or engine.default_kernel(args.precision)
     tree, data, p, seqs = make_workload(args.taxa, args.codons, args.seed)
     x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"],
                              p["w2"])
@@ -312,13 +315,15 @@ def main():
     sto = "double" if args.precision == "fp64" else "float"
     prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
                         (f"k_prune_res/k_prune_mfma<{sto}> (pruning levels, FP64 arithmetic, "
-                         f"{sto} storage)") if args.kernel == "mfma" else
+                         f"{sto} storage)") if args.kernel != "valu" else
                         f"k_prune_valu<{sto}> (FP64 arithmetic, {sto} storage)")
     rls = [(tm["sum_prune"], prune_rl)]
     adj_rl = br_rl = None
     if args.gradient == "adjoint":
         nlev = tm["n_adj_launch"] - nev   # child-adjoint launches (one branch-matrix launch each)
         adj_rl = roofline(work[7], work[6], tm["sum_adjoint"], nev, nlev,
+                          "k_adj_a32 (child adjoints, FP32 matrix-core products, identity term "
+                          "in FP64)" if args.kernel == "mfma32" else
                           f"k_adj_a<{sto}> (child adjoints, FP64 arithmetic)")
         br_rl = roofline(work[9], work[8], tm["sum_branch"], nev, nev,
                          f"k_adj_m<{sto}> (branch matrices, FP64 arithmetic)")
@@ -326,7 +331,7 @@ def main():
     rl = max(rls, key=lambda r: r[0])[1]
     default_workload = (args.taxa, args.codons, args.seed, args.precision, args.gradient,
                         args.kernel, args.tile, args.tune) == (128, 10000, 2, "fp32", "adjoint",
-                                                               "mfma", 0, True)
+                                                               "mfma32", 0, True)
     tr = measured_traffic(rl["kernel"]) if default_workload else None
     if tr:
         rl["traffic"], rl["traffic_source"] = tr
@@ -342,7 +347,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-storage/f64-arith",
+        "dtype": "f64" if args.precision == "fp64" else (
+            "f32-storage/f64-arith (child adjoints f32 matrix cores)" if args.kernel == "mfma32"
+            else "f32-storage/f64-arith"),
         "data": "synthetic: seeded model-A simulation (Yule tree, F3x4 Dirichlet(1))",
         "config": {"workload": f"branch-site model A, {args.taxa} taxa x {args.codons} codons "
                                f"({data.npatt} patterns), {args.precision} partials, "
@@ -367,7 +374,8 @@ def main():
                       "what": "Bayes empirical Bayes site posteriors at x: 100 forward passes "
                               "(121 w sets) + 10^4-point grid integration"}
     if args.fd_steps > 0:
-        e64 = engine.BranchSiteEngine(tree, data, precision="fp64", kernel=args.kernel)
+        e64 = engine.BranchSiteEngine(tree, data, precision="fp64",
+                                      kernel="valu" if args.kernel == "valu" else "mfma")
         e64.gradientB(x)
         t1 = time.perf_counter()
         for _ in range(args.fd_steps):

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -420,8 +421,10 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
     if (genes.empty()) return fail(BSA_EINVAL, "no genes");
     if (precision != BSA_FP64 && precision != BSA_FP32)
         return fail(BSA_EINVAL, "unknown precision %d", precision);
-    if (kernel != BSA_KERNEL_MFMA && kernel != BSA_KERNEL_VALU)
+    if (kernel != BSA_KERNEL_MFMA && kernel != BSA_KERNEL_VALU && kernel != BSA_KERNEL_MFMA32)
         return fail(BSA_EINVAL, "unknown kernel %d", kernel);
+    if (kernel == BSA_KERNEL_MFMA32 && precision != BSA_FP32)
+        return fail(BSA_EINVAL, "the FP32 matrix-core kernel needs FP32 partials (BSA_FP32)");
     tile = tile == 0 ? 64 : tile;
     if (tile != 64 && tile != 128) return fail(BSA_EINVAL, "tile %d: 64 or 128 patterns per block", tile);
     f.precision = precision;
@@ -887,11 +890,14 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
                 (double *)f.d_PT, f.d_mask_amb, f.ncodes, (double *)f.d_AT);
     HIPCHK(hipGetLastError());
     REC(1);
+    // the pruning keeps FP64 products (BSA_KERNEL_MFMA32 moves only the child
+    // adjoints to the FP32 matrix cores)
+    const bool mfma32 = adjoint && f.kernel == BSA_KERNEL_MFMA32 && sizeof(T) == 4;
     for (size_t l = 0; l + 1 < f.level_off.size(); l++) {
         const int a = f.level_off[l], b = f.level_off[l + 1];
-        launch_prune<T>(s, f.kernel, f.tile, rs ? f.d_prune : f.d_prune + a,
-                        rs ? rs->lev[l] : nullptr, rs ? rs->nlev[l] : b - a, f.level_maxint[l],
-                        f.npad, f.d_z,
+        launch_prune<T>(s, f.kernel == BSA_KERNEL_VALU ? 1 : 0, f.tile,
+                        rs ? f.d_prune : f.d_prune + a, rs ? rs->lev[l] : nullptr,
+                        rs ? rs->nlev[l] : b - a, f.level_maxint[l], f.npad, f.d_z,
                         (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes, (T *)f.d_L,
                         f.d_E, adjoint ? (T *)f.d_C : nullptr);
     }
@@ -903,12 +909,21 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
     HIPCHK(hipGetLastError());
     REC(3);
     if (adjoint) {
-        for (size_t d = 0; d + 1 < f.adj_off.size(); d++)
-            launch_adjoint_level<T>(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
-                                    rs ? rs->adl[d] : nullptr, rs ? rs->nadl[d] : f.adj_nint[d],
-                                    f.npad, f.d_z, (const double *)f.d_PT, (const double *)f.d_AT,
-                                    f.ncodes, f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA,
-                                    f.tpb_set);
+        for (size_t d = 0; d + 1 < f.adj_off.size(); d++) {
+            if (mfma32)
+                launch_adjoint_level32(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
+                                       rs ? rs->adl[d] : nullptr,
+                                       rs ? rs->nadl[d] : f.adj_nint[d], f.npad, f.d_z,
+                                       (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
+                                       f.d_E, (const float *)f.d_C, (float *)f.d_A, f.d_EA,
+                                       f.tpb_set);
+            else
+                launch_adjoint_level<T>(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
+                                        rs ? rs->adl[d] : nullptr,
+                                        rs ? rs->nadl[d] : f.adj_nint[d], f.npad, f.d_z,
+                                        (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
+                                        f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA, f.tpb_set);
+        }
         REC(6);
         launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
                             f.npad, f.nchunk, f.tpc, f.d_z, (const double *)f.d_PT,
@@ -1403,7 +1418,7 @@ int forest_set_shapes(Forest &f, int tile, int tpb, int tpc)
 {
     HIP_SHARED;
     if (tile != 64 && tile != 128) return fail(BSA_EINVAL, "tile %d: 64 or 128 patterns per block", tile);
-    if (tile == 128 && f.kernel != BSA_KERNEL_MFMA)
+    if (tile == 128 && f.kernel == BSA_KERNEL_VALU)
         return fail(BSA_EINVAL, "128-pattern tiles need the MFMA kernel");
     if (tpb < 0 || tpb > 8) return fail(BSA_EINVAL, "tiles per child-adjoint block %d: 0..8", tpb);
     if (tpc < 0 || tpc > kAdjTilesPerChunk)
@@ -1459,7 +1474,7 @@ int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *o
     if ((rc = measure(&best))) return rc;
     const double start = best;
     const std::vector<int> cand[3] = {
-        f.kernel == BSA_KERNEL_MFMA ? std::vector<int>{64, 128} : std::vector<int>{64},
+        f.kernel != BSA_KERNEL_VALU ? std::vector<int>{64, 128} : std::vector<int>{64},
         {0, 1, 2, 4, 8},
         {0, 4, 8, 16, 32, 40, 64}};
     for (int k = 0; k < 3; k++) {

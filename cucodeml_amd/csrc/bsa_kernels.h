@@ -91,6 +91,11 @@ constexpr int kAdjTilesPerChunk = 64;   // most 64-pattern tiles per branch-matr
 constexpr int kAdjTilesPerChunkAuto = 32;   // most of them when chosen by the gene's size
 constexpr int kWParts = 8;              // partial sums per eigen set in the W reduction
 
+// the same on the FP32 matrix cores (float storage, BSA_KERNEL_MFMA32)
+void launch_adjoint_level32(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
+                            const unsigned char *z, const double *PT, const double *TT, int ncodes,
+                            const int *E, const float *Cb, float *A, int *EA, int tpb);
+
 // adjoints of one depth's interior children (jobs with v_tip < 0 only);
 // tpb: 64-pattern tiles per block, 0 for a choice by the launch's size
 template <typename T>

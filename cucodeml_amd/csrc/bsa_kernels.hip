@@ -6,18 +6,22 @@
 //   k_prune_valu  the same on plain vector FMA
 //   k_root        per-pattern class log-likelihoods, mixture and weighted sum
 //   k_adj_a       adjoints of one depth's interior children (preorder sweep)
+//   k_adj_a32     the same on the FP32 matrix cores (BSA_KERNEL_MFMA32)
 //   k_adj_m       branch matrices of every (branch, class) job
 //   k_eigbranch, k_wreduce, k_zform   eigen-basis derivatives and their sums
 //   k_beb_*       Bayes empirical Bayes over the 10^4-point grid
 //
-// Arithmetic is FP64 throughout; T (double or float) is only the storage type
-// of the conditional likelihoods.  Layouts:
+// Arithmetic is FP64 (k_adj_a32 aside); T (double or float) is only the
+// storage type of the conditional likelihoods.  Layouts:
 //   PT[pm][k][j]    = P_{j,k}(t) - delta_{jk} (double) for matrix pm = node*3 + set,
 //                     64 x 64, rows and columns >= 61 zero (k: child state,
-//                     j: parent state).  The identity part is added exactly
-//                     after the product, so small off-diagonal terms are
-//                     summed on their own scale instead of being swamped by
-//                     the near-1 diagonal.
+//                     j: parent state).  P - I is computed without the
+//                     cancellation that P itself would suffer for short
+//                     branches.  The FP64 matrix-core kernels add the identity
+//                     when they stage P in LDS and multiply the full P;
+//                     k_prune_valu and k_adj_a32 add it after the product, so
+//                     that the small off-diagonal terms are summed on their
+//                     own scale (in FP32 the near-1 diagonal would swamp them).
 //   L[slot][h][j]   (T) conditional likelihood of a node / site class, pattern
 //                   h, parent-side state j (64, lanes >= 61 stay exactly zero)
 //   E[slot][h]      accumulated power-of-two exponent: true value = L * 2^E
@@ -452,6 +456,15 @@ __global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_res(const PruneJob 
     }
 }
 
+// ------------------------------------------------------------------ pruning (FP32 MFMA)
+
+__device__ inline f4 mfma32(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int kT32 = 65;   // floats per row of a wave's transpose tile in LDS
+
 // ------------------------------------------------------------------ pruning (VALU)
 
 // Same job on vector FMA, tile of 64 patterns: thread = parent state j (lane)
@@ -806,6 +819,89 @@ __global__ __launch_bounds__(256, 3) void k_adj_a(const AdjJob *__restrict__ job
     }
 }
 
+// Adjoints of one depth's interior children with FP32 matrix-core products
+// (float storage, BSA_KERNEL_MFMA32):
+//   Abar_v(h, k) = X(h, k) + sum_j X(h, j) fl32(P_v - I)_{j,k}
+// X formed in FP64 as in k_adj_a, rounded to FP32 (to nearest) as the
+// matrix-core operand and for the identity term, which is added in FP64 from
+// a transpose tile of the wave's X rows in LDS; the off-diagonal terms are
+// summed on their own scale by the FP32 accumulators (16x16x4, 16 k-steps x
+// 4 column blocks).  The accumulator element i of lane (g, r) is pattern
+// 4g + i of the wave's 16, state 16b + r.  An adjoint feeds only the
+// gradient, where this rounding stays below 1e-6 relative; the likelihood
+// itself keeps FP64 products.
+__global__ __launch_bounds__(256, 4) void k_adj_a32(const AdjJob *__restrict__ jobs,
+                                                    const int *__restrict__ sel, int npad,
+                                                    const unsigned char *__restrict__ z,
+                                                    const double *__restrict__ PT,
+                                                    const double *__restrict__ TT, int ncodes,
+                                                    const int *__restrict__ E,
+                                                    const float *__restrict__ Cb,
+                                                    float *__restrict__ A, int *__restrict__ EA,
+                                                    int tpb)
+{
+    __shared__ float sP[64 * kT32];            // fl32(P_v - I), [k][j]
+    __shared__ float sX[4][16 * kT32];         // per wave: X of its 16 patterns
+    const AdjJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    {
+        const double *P = PT + (size_t)jb.v_pmat * 4096;
+        for (int q = tid; q < 4096; q += 256) sP[(q >> 6) * kT32 + (q & 63)] = (float)P[q];
+    }
+    float *xw = sX[w];
+    const int ntile = npad / 64;
+    for (int t = 0; t < tpb; t++) {
+        const int tile = blockIdx.x * tpb + t;
+        if (tile >= ntile) break;
+        const int hw = tile * 64 + 16 * w;
+        double x[16];
+        const int ex = x_sum16<float>(jb, hw + r, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
+        float xf[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            xf[kk] = (float)x[kk];
+            xw[r * kT32 + 16 * g + kk] = xf[kk];
+        }
+        if (t == 0) __syncthreads();
+        f4 Y[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) Y[b] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                Y[b] = mfma32(xf[kk], sP[(16 * b + r) * kT32 + 16 * g + kk], Y[b]);
+        float *Av = A + (size_t)jb.v_aslot * npad * 64;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            // output row 4g + i of the wave; its X exponent sits in lane 4g + i
+            const int exr = __shfl(ex, 4 * g + i, 64);
+            double yv[4];
+            double m = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                yv[b] = (double)xw[(4 * g + i) * kT32 + 16 * b + r] + (double)Y[b][i];
+                m = yv[b] > m ? yv[b] : m;
+            }
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                double o = __shfl_xor(m, off, 64);
+                m = o > m ? o : m;
+            }
+            int e = 0;
+            if (m > 0) frexp(m, &e);
+            const int h = hw + 4 * g + i;
+            uint32_t rb = row_seed((uint32_t)jb.v_aslot ^ 0xA5A5A5A5u, (uint32_t)h, (uint32_t)r);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                Store<float>::put(Av + (size_t)h * 64 + 16 * b + r, ldexp(yv[b], -e), rb);
+                rb = xorshift32(rb);
+            }
+            if (r == 0) EA[(size_t)jb.v_aslot * npad + h] = exr + e;
+        }
+    }
+}
+
 // Branch matrices of every adjoint job, one launch after the sweep:
 // M_v[j][k] += sum_h X(h, j) 2^{ex(h) + E_v(h)} I_v(h, k) over a chunk of tiles.
 // Per tile the block forms the scaled X tile in LDS (each lane one pattern and
@@ -1130,6 +1226,17 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
     else
         hipLaunchKernelGGL(k_prune_valu<T>, dim3(npad / 64, njobs), dim3(256), 0, s, jobs, sel,
                            npad, z, PT, TT, ncodes, L, E, Cb);
+}
+
+void launch_adjoint_level32(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
+                            const unsigned char *z, const double *PT, const double *TT, int ncodes,
+                            const int *E, const float *Cb, float *A, int *EA, int tpb)
+{
+    if (njobs <= 0) return;
+    const int ntile = npad / 64;
+    if (tpb <= 0) tpb = std::max(1, std::min(4, njobs * ntile / 512));
+    hipLaunchKernelGGL(k_adj_a32, dim3((ntile + tpb - 1) / tpb, njobs), dim3(256), 0, s, jobs, sel,
+                       npad, z, PT, TT, ncodes, E, Cb, A, EA, tpb);
 }
 
 template <typename T>

@@ -18,7 +18,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libbsa.so")
 
 BSA_FP64, BSA_FP32 = 0, 1
-BSA_KERNEL_MFMA, BSA_KERNEL_VALU = 0, 1
+BSA_KERNEL_MFMA, BSA_KERNEL_VALU, BSA_KERNEL_MFMA32 = 0, 1, 2
 
 EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
@@ -172,13 +172,21 @@ def _problem(tree, data, precision, kernel, device, tile, pi, keep):
                     z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
                     code_mask=mask.ctypes.data_as(C.POINTER(C.c_ubyte)), fpatt=_dptr(fp),
                     pi=_dptr(pv), precision=BSA_FP32 if precision == "fp32" else BSA_FP64,
-                    kernel=BSA_KERNEL_VALU if kernel == "valu" else BSA_KERNEL_MFMA,
+                    kernel={"valu": BSA_KERNEL_VALU, "mfma32": BSA_KERNEL_MFMA32}.get(kernel, BSA_KERNEL_MFMA),
                     device=device, tile=tile)
 
 
+def default_kernel(precision):
+    """FP32 partials: child adjoints on the FP32 matrix cores (mfma32);
+    FP64: every product on the FP64 matrix cores."""
+    return "mfma32" if precision == "fp32" else "mfma"
+
+
 class BranchSiteEngine:
-    def __init__(self, tree, data, precision="fp64", kernel="mfma", device=0,
+    def __init__(self, tree, data, precision="fp64", kernel=None, device=0,
                  fix_omega=False, omega_fix=1.0, p_transformed=True, pi=None, tile=0):
+        kernel = kernel or default_kernel(precision)
+        self.kernel = kernel
         self.tree, self.data = tree, data
         self.precision, self.device = precision, device
         self._keep = []
@@ -339,8 +347,10 @@ class BranchSiteBatch:
     padded to the largest gene's count, so genes of similar size batch best.
     ``genes`` is a list of (tree, data) pairs."""
 
-    def __init__(self, genes, precision="fp64", kernel="mfma", device=0, tile=0,
+    def __init__(self, genes, precision="fp64", kernel=None, device=0, tile=0,
                  fix_omega=False, omega_fix=1.0, p_transformed=True):
+        kernel = kernel or default_kernel(precision)
+        self.kernel = kernel
         self.genes = list(genes)
         self.precision, self.device = precision, device
         self._keep = []
@@ -511,14 +521,15 @@ class BranchSiteBatch:
 NOALLOC_BYTES = 128 << 20
 
 
-def lnl_noalloc(tree, data, x, precision="fp64", kernel="mfma", device=0, fix_omega=False,
+def lnl_noalloc(tree, data, x, precision="fp64", kernel=None, device=0, fix_omega=False,
                 omega_fix=1.0, p_transformed=True, stream=None, tile=0):
     """-lnL, per-class log f (4, npatt) and per-site log f of one gene by
     bsa_lnl_noalloc: no engine, no device allocation (every device buffer in
     the library's fixed array on the device).  stream: a hipStream_t value
     (int) or None for the null stream."""
     keep = []
-    prob = _problem(tree, data, precision, kernel, device, tile, None, keep)
+    prob = _problem(tree, data, precision, kernel or default_kernel(precision), device, tile,
+                    None, keep)
     o = _Opts(int(bool(fix_omega)), float(omega_fix), int(bool(p_transformed)))
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = C.c_double()

@@ -46,6 +46,11 @@ extern "C" {
 /* pruning kernel variant */
 #define BSA_KERNEL_MFMA   0   /* 16x16x4 matrix-core contraction (default) */
 #define BSA_KERNEL_VALU   1   /* plain vector FMA contraction              */
+#define BSA_KERNEL_MFMA32 2   /* with BSA_FP32 only: as BSA_KERNEL_MFMA, with
+                                 the child adjoints of the gradient sweep on
+                                 the FP32 matrix cores (P - I in FP32, the
+                                 identity term added in FP64); the pruning
+                                 keeps FP64 products                        */
 
 typedef struct bsa_engine bsa_engine;
 
