@@ -326,6 +326,9 @@ def main():
                           "in FP64)" if args.kernel == "mfma32" else
                           f"k_adj_a<{sto}> (child adjoints, FP64 arithmetic)")
         br_rl = roofline(work[9], work[8], tm["sum_branch"], nev, nev,
+                         "k_adj_m<float> (branch matrices: interior children on the FP32 matrix "
+                         "cores in 16-pattern groups summed in FP64, tips FP64)"
+                         if args.kernel == "mfma32" else
                          f"k_adj_m<{sto}> (branch matrices, FP64 arithmetic)")
         rls += [(tm["sum_adjoint"], adj_rl), (tm["sum_branch"], br_rl)]
     rl = max(rls, key=lambda r: r[0])[1]

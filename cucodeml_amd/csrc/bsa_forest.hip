@@ -890,8 +890,9 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
                 (double *)f.d_PT, f.d_mask_amb, f.ncodes, (double *)f.d_AT);
     HIPCHK(hipGetLastError());
     REC(1);
-    // the pruning keeps FP64 products (BSA_KERNEL_MFMA32 moves only the child
-    // adjoints to the FP32 matrix cores)
+    // the pruning keeps FP64 products (BSA_KERNEL_MFMA32 moves the child
+    // adjoints and the interior children's branch matrices to the FP32
+    // matrix cores)
     const bool mfma32 = adjoint && f.kernel == BSA_KERNEL_MFMA32 && sizeof(T) == 4;
     for (size_t l = 0; l + 1 < f.level_off.size(); l++) {
         const int a = f.level_off[l], b = f.level_off[l + 1];
@@ -926,7 +927,7 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
         }
         REC(6);
         launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
-                            f.npad, f.nchunk, f.tpc, f.d_z, (const double *)f.d_PT,
+                            f.npad, f.nchunk, f.tpc, mfma32, f.d_z, (const double *)f.d_PT,
                             (const double *)f.d_AT, f.ncodes, f.d_mask64, f.d_tperm,
                             (const T *)f.d_L, f.d_E, (const T *)f.d_C, (const T *)f.d_A, f.d_EA,
                             f.d_Mpart);

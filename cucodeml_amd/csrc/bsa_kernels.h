@@ -103,12 +103,13 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int
                           const unsigned char *z, const double *PT, const double *TT, int ncodes,
                           const int *E, const T *Cb, T *A, int *EA, int tpb);
 
-// branch-matrix chunk partials of all jobs (after the sweep); tperm: per tip
-// row, the order in which the tip's patterns are visited (a permutation of
-// 0..npad-1)
+// branch-matrix chunk partials of all jobs (after the sweep); m32 (float
+// storage only): interior children's products on the FP32 matrix cores;
+// tperm: per tip row, the order in which the tip's patterns are visited (a
+// permutation of 0..npad-1)
 template <typename T>
 void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
-                      int nchunk, int tpc, const unsigned char *z, const double *PT,
+                      int nchunk, int tpc, bool m32, const unsigned char *z, const double *PT,
                       const double *TT, int ncodes, const double *mask64, const int *tperm,
                       const T *L, const int *E, const T *Cb, const T *A, const int *EA,
                       double *Mpart);

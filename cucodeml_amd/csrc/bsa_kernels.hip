@@ -7,7 +7,8 @@
 //   k_root        per-pattern class log-likelihoods, mixture and weighted sum
 //   k_adj_a       adjoints of one depth's interior children (preorder sweep)
 //   k_adj_a32     the same on the FP32 matrix cores (BSA_KERNEL_MFMA32)
-//   k_adj_m       branch matrices of every (branch, class) job
+//   k_adj_m       branch matrices of every (branch, class) job (interior children on
+//                 the FP32 matrix cores with BSA_KERNEL_MFMA32)
 //   k_eigbranch, k_wreduce, k_zform   eigen-basis derivatives and their sums
 //   k_beb_*       Bayes empirical Bayes over the 10^4-point grid
 //
@@ -857,12 +858,15 @@ __global__ __launch_bounds__(256, 4) void k_adj_a32(const AdjJob *__restrict__ j
         double x[16];
         const int ex = x_sum16<float>(jb, hw + r, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
         float xf[16];
+        // the previous tile's reads of xw (the wave's own rows) come first
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             xf[kk] = (float)x[kk];
             xw[r * kT32 + 16 * g + kk] = xf[kk];
         }
         if (t == 0) __syncthreads();
+        else __builtin_amdgcn_wave_barrier();
         f4 Y[4];
 #pragma unroll
         for (int b = 0; b < 4; b++) Y[b] = f4{0, 0, 0, 0};
@@ -902,6 +906,50 @@ __global__ __launch_bounds__(256, 4) void k_adj_a32(const AdjJob *__restrict__ j
     }
 }
 
+// Branch matrices on the FP32 matrix cores (BSA_KERNEL_MFMA32): the product
+//   M_v[j][k] += sum_h X(h, j) 2^{s(h)} L_v(h, k),   s(h) = ex(h) + E_v(h),
+// over one 64-pattern tile, per 16-pattern group q (the rows one wave forms):
+// the group's X rows are stored in LDS as float(X(h, j) 2^{s(h) - s_q}), s_q
+// the largest s(h) of the group (so the values are <= 1 and keep their
+// relative sizes), its four FP32 16x16x4 steps are summed in an FP32 tile,
+// and the tile times 2^{s_q} is added to the FP64 accumulators.  sXm: the
+// tile's scaled rows (row stride kT32), sGrp[q] = 2^{s_q}, bv[kk] =
+// L_v(h0 + 4kk + g, 16w + r).  Wave w adds columns 16w..16w+15; Macc[a]
+// element i is row 16a + 4g + i (the FP32 accumulator layout).
+__device__ inline void m_tile32(const float *sXm, const double *sGrp, const float *bv, d4 Macc[4])
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        f4 Tq[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) Tq[a] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const int kk = 4 * q + st;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                Tq[a] = mfma32(sXm[(4 * kk + g) * kT32 + 16 * a + r], bv[kk], Tq[a]);
+        }
+        const double sc = sGrp[q];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) Macc[a][i] += (double)Tq[a][i] * sc;
+    }
+}
+
+// the largest s of the wave's 16 patterns (lane 16g + r holds pattern r's s)
+__device__ inline int group_max16(int s)
+{
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const int o = __shfl_xor(s, off, 64);
+        s = o > s ? o : s;
+    }
+    return s;
+}
+
 // Branch matrices of every adjoint job, one launch after the sweep:
 // M_v[j][k] += sum_h X(h, j) 2^{ex(h) + E_v(h)} I_v(h, k) over a chunk of tiles.
 // Per tile the block forms the scaled X tile in LDS (each lane one pattern and
@@ -915,8 +963,10 @@ __global__ __launch_bounds__(256, 4) void k_adj_a32(const AdjJob *__restrict__ j
 // w owns rows 16w..16w+15 of M instead, and multiplies only the 16-column
 // strips that the tile's codes reach (usually one of four).  tperm entries
 // hold the pattern in the low 24 bits and its code in the top 8.
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_adj_m(const AdjJob *__restrict__ jobs,
+// M32 (float storage, BSA_KERNEL_MFMA32): an interior child's product runs on
+// the FP32 matrix cores with m_tile32; tips keep the FP64 path.
+template <typename T, bool M32>
+__global__ __launch_bounds__(256, sizeof(T) == 4 && !M32 ? 4 : 3) void k_adj_m(const AdjJob *__restrict__ jobs,
                                                   const int *__restrict__ sel, int npad,
                                                   const unsigned char *__restrict__ z,
                                                   const double *__restrict__ TT, int ncodes,
@@ -929,7 +979,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_adj_m(const Adj
                                                   const int *__restrict__ EA,
                                                   double *__restrict__ Mpart, int nchunk, int tpc)
 {
+    static_assert(!M32 || sizeof(T) == 4, "FP32 branch-matrix products need float storage");
     __shared__ double sX[64 * kLdsStride];
+    __shared__ double sGrp[4];
     const AdjJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
     const int chunk = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
@@ -957,6 +1009,40 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_adj_m(const Adj
         // X in two halves of 8 states, so that at most 8 values of each
         // operand are live at a time (VGPRs for four blocks per CU)
         const int exv = vtip ? 0 : E[(size_t)jb.v_slot * npad + hx];
+        if constexpr (M32) {
+            if (!vtip) {
+                float *sXm = reinterpret_cast<float *>(sX);
+                float bv[16];
+                const T *Lv = L + ((size_t)jb.v_slot * npad + h0 + g) * 64 + col;
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) bv[kk] = Lv[(size_t)(4 * kk) * 64];
+                double sc;
+                int sg;
+                {
+                    double x[8];
+                    const int ex =
+                        x_sum16<T, 8>(jb, hx, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
+                    const int sh = ex + exv;
+                    sg = group_max16(sh);
+                    sc = pow2i(sh - sg);
+                    __syncthreads();   // the previous tile's reads of sX, sGrp are done
+#pragma unroll
+                    for (int kk = 0; kk < 8; kk++)
+                        sXm[(16 * w + r) * kT32 + 16 * g + kk] = (float)(x[kk] * sc);
+                }
+                {
+                    double x[8];
+                    x_sum16<T, 8>(jb, hx, 16 * g + 8, npad, z, TT, ncodes, E, Cb, A, EA, x);
+#pragma unroll
+                    for (int kk = 0; kk < 8; kk++)
+                        sXm[(16 * w + r) * kT32 + 16 * g + 8 + kk] = (float)(x[kk] * sc);
+                }
+                if (lane == 0) sGrp[w] = pow2i(sg);
+                __syncthreads();
+                m_tile32(sXm, sGrp, bv, Macc);
+                continue;
+            }
+        }
         {
             double x[8];
             const int ex = x_sum16<T, 8>(jb, hx, 16 * g, npad, z, TT, ncodes, E, Cb, A, EA, x);
@@ -1013,12 +1099,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 3) void k_adj_m(const Adj
     }
     // interior child: Macc[a] holds rows 16a.., columns 16w..; tip: rows
     // 16w.., columns 16a..
+    // (FP32 products of an interior child: rows 16a + 4g + i)
     double *Mo = Mpart + ((size_t)jb.m_index * nchunk + chunk) * 4096;
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int row = vtip ? 16 * w + g + 4 * i : 16 * a + g + 4 * i;
+            const int row = vtip ? 16 * w + g + 4 * i
+                                 : 16 * a + (M32 ? 4 * g + i : g + 4 * i);
             const int cl = vtip ? 16 * a + r : col;
             Mo[row * 64 + cl] = Macc[a][i];
         }
@@ -1268,15 +1356,23 @@ void launch_adjoint_level(hipStream_t s, const AdjJob *jobs, const int *sel, int
 
 template <typename T>
 void launch_adjoint_m(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
-                      int nchunk, int tpc, const unsigned char *z, const double *PT,
+                      int nchunk, int tpc, bool m32, const unsigned char *z, const double *PT,
                       const double *TT, int ncodes, const double *mask64, const int *tperm,
                       const T *L, const int *E, const T *Cb, const T *A, const int *EA,
                       double *Mpart)
 {
     if (njobs <= 0) return;
     (void)PT;   // tip rows come from TT, interior rows from L
-    hipLaunchKernelGGL(k_adj_m<T>, dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel, npad, z, TT,
-                       ncodes, mask64, tperm, L, E, Cb, A, EA, Mpart, nchunk, tpc);
+    if constexpr (sizeof(T) == 4) {
+        if (m32) {
+            hipLaunchKernelGGL((k_adj_m<T, true>), dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel,
+                               npad, z, TT, ncodes, mask64, tperm, L, E, Cb, A, EA, Mpart, nchunk,
+                               tpc);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_adj_m<T, false>), dim3(nchunk, njobs), dim3(256), 0, s, jobs, sel, npad,
+                       z, TT, ncodes, mask64, tperm, L, E, Cb, A, EA, Mpart, nchunk, tpc);
 }
 
 void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const EigJob *ejobs,
@@ -1299,12 +1395,13 @@ template void launch_adjoint_level<float>(hipStream_t, const AdjJob *, const int
                                           const unsigned char *, const double *, const double *,
                                           int, const int *, const float *, float *, int *, int);
 template void launch_adjoint_m<double>(hipStream_t, const AdjJob *, const int *, int, int, int,
-                                       int, const unsigned char *, const double *, const double *,
+                                       int, bool, const unsigned char *, const double *,
+                                       const double *,
                                        int, const double *, const int *, const double *,
                                        const int *, const double *, const double *, const int *,
                                        double *);
 template void launch_adjoint_m<float>(hipStream_t, const AdjJob *, const int *, int, int, int, int,
-                                      const unsigned char *, const double *, const double *, int,
+                                      bool, const unsigned char *, const double *, const double *, int,
                                       const double *, const int *, const float *, const int *,
                                       const float *, const float *, const int *, double *);
 

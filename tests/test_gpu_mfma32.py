@@ -1,7 +1,9 @@
-"""FP32 matrix-core products in the child adjoints (kernel "mfma32",
-BSA_KERNEL_MFMA32, FP32 partials): the gradient sweep's contractions
-Abar_v = X P_v run on the FP32 matrix cores with P - I in FP32 and the
-identity term added in FP64; the pruning keeps FP64 products.  Checks: lnL
+"""FP32 matrix-core products in the gradient sweep (kernel "mfma32",
+BSA_KERNEL_MFMA32, FP32 partials): the child adjoints Abar_v = X P_v run on
+the FP32 matrix cores with P - I in FP32 and the identity term added in
+FP64, and the interior children's branch matrices M_v = sum_h X_h^T L_h in
+FP32 per 16-pattern group, summed in FP64; the pruning keeps FP64 products.
+Checks: lnL
 and the per-class values bitwise those of the FP32 engine with FP64 products
 (same forward pass), the gradient within 1e-5 relative of FP64 on the
 fixtures and at 128 taxa x 10,000 codons, and the child adjoints no slower."""
