@@ -10,15 +10,18 @@ arithmetic, stochastic rounding) with the adjoint gradient
 the same run, the plain FP64 path with ming2's forward-difference gradient
 (gradientB: np+1 = 259 likelihood evaluations per step) is timed on the same
 GPU, and the reference's own CPU codeml (built from its sources into
-oracle/_ref/codeml) on a bounded sample of the same workload.
+oracle/_ref/codeml) on the same alignment at fixed parameters, its set-up
+timed apart and subtracted.
 
 Inputs are synthetic (seeded simulation under model A, parameter draws of
 the 128-taxon gradient-check configuration) and resident in HBM before
 timing starts.
 """
 import argparse
+import collections
 import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -34,6 +37,7 @@ from cucodeml_amd import engine, formats, simulate  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def make_workload(ns, ncodon, seed):
@@ -72,68 +76,152 @@ method = 0
 """
 
 
-def cpu_baseline(tree, seqs, p, ncodon_full, npatt_full, np_full, sample_codons):
-    """Reference CPU codeml at fixed parameters on the first `sample_codons`
-    codons; one run = 3 full likelihood evaluations (initial, NEB, final)."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "codeml")
-    if not os.path.exists(ref):
-        return None
-    sub = [s[:3 * sample_codons] for s in seqs]
-    d_sub = formats.load_codon_data(tree.names, sub)
+REF_CODEML = os.path.join(ROOT, "oracle", "_ref", "codeml")
+
+
+def ref_run(names, seqs, newick, ctl, xin=None, timeout=3600):
+    """One run of the reference CPU codeml in a scratch folder: (wall
+    seconds, stdout, `out` text), or None if it failed."""
     work = tempfile.mkdtemp(prefix="bsa_cpu_")
     try:
-        formats.write_phylip(os.path.join(work, "seq.phy"), tree.names, sub)
+        formats.write_phylip(os.path.join(work, "seq.phy"), names, seqs)
         with open(os.path.join(work, "tree.nwk"), "w") as f:
-            f.write(f"{tree.ns} 1\n{tree.newick(digits=10)}\n")
-        x = list(tree.branch_vector()) + [p["kappa"], p["p0"], p["p1"], p["w0"]]
-        with open(os.path.join(work, "in.codeml"), "w") as f:
-            f.write("-1 " + " ".join(f"{v:.12f}" for v in x) + "\n")
+            f.write(f"{len(names)} 1\n{newick}\n")
+        if xin is not None:
+            with open(os.path.join(work, "in.codeml"), "w") as f:
+                f.write(xin + "\n")
         with open(os.path.join(work, "codeml.ctl"), "w") as f:
-            f.write(CTL.format(w2=p["w2"]))
+            f.write(ctl)
         t0 = time.perf_counter()
-        r = subprocess.run([ref, "codeml.ctl"], cwd=work, capture_output=True, text=True,
-                           input="\n", timeout=600)
+        r = subprocess.run([REF_CODEML, "codeml.ctl"], cwd=work, capture_output=True, text=True,
+                           input="\n", timeout=timeout)
         wall = time.perf_counter() - t0
         if r.returncode != 0:
             return None
+        with open(os.path.join(work, "out")) as f:
+            out = f.read()
+        return wall, r.stdout, out
     finally:
         shutil.rmtree(work, ignore_errors=True)
-    per_lnl_sample = wall / 3.0
+
+
+def cpu_info():
+    """CPU model of this host (for the baseline records)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(tree, seqs, p, npatt_full, np_full, sample_codons):
+    """Reference CPU codeml (one thread) at fixed parameters (in.codeml -1).
+    One run makes 3 full likelihood evaluations (codeml.c:670, 5107, 833);
+    a second run on the first 10 codons of the same alignment gives the
+    set-up (input, distances, eigen systems, and its own three evaluations
+    of 10 codons), which is subtracted.  The sample is the first
+    `sample_codons` codons (the whole alignment by default), its per-lnL
+    time scaled by patterns to the full alignment when it is smaller."""
+    if not os.path.exists(REF_CODEML):
+        return None
+    x = list(tree.branch_vector()) + [p["kappa"], p["p0"], p["p1"], p["w0"]]
+    xin = "-1 " + " ".join(f"{v:.12f}" for v in x)
+    ctl = CTL.format(w2=p["w2"])
+    nwk = tree.newick(digits=10)
+    sub = [s[:3 * sample_codons] for s in seqs]
+    d_sub = formats.load_codon_data(tree.names, sub)
+    run = ref_run(tree.names, sub, nwk, ctl, xin)
+    setup = ref_run(tree.names, [s[:30] for s in seqs], nwk, ctl, xin)
+    if run is None or setup is None:
+        return None
+    wall, wall0 = run[0], setup[0]
+    per_lnl_sample = (wall - wall0) / 3.0
     per_lnl_full = per_lnl_sample * npatt_full / d_sub.npatt
     evals_per_s = 1.0 / ((np_full + 1) * per_lnl_full)
     return {"value": evals_per_s, "unit": "lnL+gradient evals/s", "cores": 1,
-            "kind": "reference",
-            "sample": (f"reference codeml (CPU, -O3, 1 thread) at fixed parameters on "
-                       f"{tree.ns} taxa x {sample_codons} codons ({d_sub.npatt} patterns): "
-                       f"{wall:.2f} s per run of 3 lnL evaluations incl. set-up -> "
-                       f"{per_lnl_sample:.3f} s/lnL, scaled by patterns to {npatt_full} "
-                       f"patterns ({per_lnl_full:.2f} s/lnL) x (np+1)={np_full + 1} "
-                       f"evaluations per forward-difference gradient")}
+            "kind": "reference", "cpu": cpu_info(),
+            "sample": (f"reference codeml (CPU, gcc -O3, 1 thread) at fixed parameters on "
+                       f"{tree.ns} taxa x {len(sub[0]) // 3} codons ({d_sub.npatt} patterns): "
+                       f"{wall:.2f} s per run of 3 lnL evaluations, {wall0:.2f} s for the "
+                       f"set-up run (10 codons) -> {per_lnl_sample:.3f} s/lnL"
+                       + (f", scaled by patterns to {npatt_full} ({per_lnl_full:.3f} s/lnL)"
+                          if d_sub.npatt != npatt_full else "")
+                       + f"; x (np+1) = {np_full + 1} evaluations per forward-difference "
+                         f"gradient (ming2's gradientB)")}
 
 
-def roofline(work_bytes, work_flops, ms_sum, nev, launches, kernel):
+def cpu_scan_baseline(ngenes, nproc):
+    """genes/hour of the reference CPU codeml on the genome-scan workload, as
+    a scan runs it: one single-thread process per gene and hypothesis (H1,
+    then H0), `nproc` genes at once on `nproc` cores, each from codeml's own
+    starting values (tree without branch lengths).  The sample is the nproc
+    genes of the first `ngenes` whose work (patterns x taxa) is nearest the
+    median: the workload's sizes are lognormal, so the mean gene costs more
+    and the figure favours the CPU."""
+    if not os.path.exists(REF_CODEML):
+        return None
+    from concurrent.futures import ThreadPoolExecutor
+    from cucodeml_amd import scan
+    genes = scan.simulate_genes(ngenes)
+    work = np.array([g.data.npatt * g.tree.ns for g in genes], dtype=float)
+    med = float(np.median(work))
+    pick = sorted(np.argsort(np.abs(work - med))[:nproc].tolist())
+    ctl1 = CTL.replace("fix_omega = 1", "fix_omega = 0").format(w2=1.5)
+    ctl0 = CTL.format(w2=1.0)
+
+    def fit(i):
+        g = genes[i]
+        seqs = formats.sequences_of(g.data)
+        nwk = g.tree.newick(lengths=False)
+        t0 = time.perf_counter()
+        h1 = ref_run(g.tree.names, seqs, nwk, ctl1, timeout=6 * 3600)
+        h0 = ref_run(g.tree.names, seqs, nwk, ctl0, timeout=6 * 3600)
+        el = time.perf_counter() - t0
+        print(f"reference fits of {g.name} ({g.tree.ns} taxa x {g.data.npatt} patterns): "
+              f"{el:.0f} s", file=sys.stderr, flush=True)
+        ok = h1 is not None and h0 is not None
+        return {"gene": g.name, "ns": g.tree.ns, "npatt": g.data.npatt, "seconds": el, "ok": ok}
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=nproc) as ex:
+        res = list(ex.map(fit, pick))
+    wall = time.perf_counter() - t0
+    return {"value": len(pick) / wall * 3600.0, "unit": "genes/hour", "cores": nproc,
+            "kind": "reference", "cpu": cpu_info(), "wall_s": wall,
+            "all_ok": all(r["ok"] for r in res), "genes": res,
+            "sample": (f"{len(pick)} genes of the first {ngenes} of the scan workload nearest "
+                       f"its median work ({med:.0f} patterns x taxa), one reference codeml "
+                       f"process per gene and hypothesis (H1, then H0) from codeml's own "
+                       f"starting values, {nproc} at once")}
+
+
+def roofline(work_bytes, work_flops, ms_sum, nev, launches, kernel, matrix="fp64"):
     """Roofline entry of one kernel from its algorithmic work per evaluation
     and its summed HIP-event time: bound = the larger of the HBM time
-    (bytes / 8 TB/s) and the FP64 matrix-core time (flops / 78.6 TFLOP/s)."""
+    (bytes / 8 TB/s) and the matrix-core time (flops / the dense peak of the
+    kernel's matrix-core arithmetic: FP64 78.6 or FP32 157.3 TFLOP/s)."""
+    peak_tf = FP32_PEAK_TFLOPS if matrix == "fp32" else FP64_PEAK_TFLOPS
     sec = ms_sum / 1e3 / nev
     gbs = work_bytes / sec / 1e9
     tf = work_flops / sec / 1e12
-    mfma = work_flops / (FP64_PEAK_TFLOPS * 1e12) > work_bytes / (HBM_PEAK_GBS * 1e9)
-    ach, peak, unit = (tf, FP64_PEAK_TFLOPS, "TFLOP/s") if mfma else (gbs, HBM_PEAK_GBS, "GB/s")
+    mfma = work_flops / (peak_tf * 1e12) > work_bytes / (HBM_PEAK_GBS * 1e9)
+    ach, peak, unit = (tf, peak_tf, "TFLOP/s") if mfma else (gbs, HBM_PEAK_GBS, "GB/s")
     return {"bound": "mfma" if mfma else "hbm", "achieved": ach, "peak": peak, "unit": unit,
             "frac": ach / peak, "traffic": None, "kernel": kernel,
             "bytes_per_eval": work_bytes, "flops_per_eval": work_flops,
             "avg_launch_us": ms_sum * 1e3 / max(launches, 1), "launches_per_eval": launches / nev,
-            "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
-            "fp64_tflops": tf, "fp64_frac": tf / FP64_PEAK_TFLOPS}
+            "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "matrix": matrix,
+            "matrix_tflops": tf, "matrix_frac": tf / peak_tf}
 
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` measured by the rocprofv3 PMC runs
     summarised under profiles/ for this same workload, or None."""
     import glob
-    import re
-    short = kernel.split("<")[0]
+    short = re.split(r"[<\s]", kernel)[0]
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as f:
             for line in f:
@@ -153,9 +241,17 @@ def genome_scan(ngenes, workers, precision, batch_genes, concurrent):
     if batch_genes > 0:
         scan.scan_batched(genes[:min(4, ngenes)], batch_genes=4, concurrent=1,
                           precision=precision)   # warm-up
+        stats = []
+        t0 = time.perf_counter()
+
+        def progress(k, n, m):
+            print(f"scan: batch {k}/{n} ({m} genes) done at {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+
         res, wall = scan.scan_batched(genes, batch_genes=batch_genes, concurrent=concurrent,
-                                      precision=precision)
-        how = {"mode": "batched", "batch_genes": batch_genes, "concurrent_batches": concurrent}
+                                      precision=precision, stats=stats, progress=progress)
+        how = {"mode": "batched", "batch_genes": batch_genes, "concurrent_batches": concurrent,
+               "device": scan_device_record(stats, wall)}
     else:
         scan.scan(genes[:min(2, ngenes)], workers=1, precision=precision)   # warm-up
         res, wall = scan.scan(genes, workers=workers, precision=precision)
@@ -168,7 +264,29 @@ def genome_scan(ngenes, workers, precision, batch_genes, concurrent):
             "per_gene": "H1 fit + H0 fit (bounded BFGS, adjoint gradient) + LRT",
             "precision": precision, **how, "wall_s": wall,
             "converged": int(sum(r["converged"] for r in res)),
+            "stops": dict(collections.Counter(f"{r['stop'][0]}/{r['stop'][1]}" for r in res)),
             "mean_evaluations": float(np.mean([r["evaluations"] for r in res]))}
+
+
+def scan_device_record(stats, wall):
+    """The batch engines' own event timers summed over a scan's batches:
+    each batch engine serves its genes' evaluation requests in rounds on its
+    own stream (batches run at once on separate streams)."""
+    rounds = sum(st["rounds"] for st in stats)
+    dev = sum(st["device_ms"] for st in stats)
+    bwall = sum(st["wall_ms"] for st in stats)
+    return {"batches": len(stats), "rounds": rounds,
+            "device_ms_sum": dev,
+            "device_busy_per_batch": dev / bwall if bwall > 0 else 0.0,
+            "stream_time_over_scan_wall": dev / (wall * 1e3) if wall > 0 else 0.0,
+            "device_ms_per_round": dev / max(rounds, 1),
+            "wall_ms_per_round": bwall / max(rounds, 1),
+            "host_ms_per_round": (bwall - dev) / max(rounds, 1),
+            "launches_per_round": sum(st["launches_per_round"] * st["rounds"] for st in stats)
+            / max(rounds, 1),
+            "genes_per_round": sum(st["evaluations"] for st in stats) / max(rounds, 1),
+            "phase_ms": {k: sum(st["phase_ms"][k] for st in stats)
+                         for k in stats[0]["phase_ms"]} if stats else {}}
 
 
 def large_fit(ns=512, ncodon=50000, gap=0.1, seed=4, e=1e-8):
@@ -269,7 +387,13 @@ def main():
                     help="relative convergence tolerance of the large fit")
     ap.add_argument("--fd-steps", type=int, default=1,
                     help="steps of the FP64 forward-difference path timed beside (0: skip)")
-    ap.add_argument("--cpu-sample-codons", type=int, default=1000)
+    ap.add_argument("--cpu-sample-codons", type=int, default=0,
+                    help="codons of the reference CPU baseline's sample (0: all)")
+    ap.add_argument("--scan-precision", choices=["fp64", "fp32"], default="fp64",
+                    help="partials of the scan's fits (fp32: FP32 chains finished on FP64)")
+    ap.add_argument("--scan-cpu", type=int, default=0,
+                    help="also time the reference CPU codeml on this many scan genes at once "
+                         "(one core each; full H1 + H0 fits: tens of minutes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scan-genes", type=int, default=256,
                     help="genes of the genome-scan sample timed beside (0: skip)")
@@ -321,15 +445,19 @@ def main():
     adj_rl = br_rl = None
     if args.gradient == "adjoint":
         nlev = tm["n_adj_launch"] - nev   # child-adjoint launches (one branch-matrix launch each)
+        m32 = args.kernel == "mfma32"
         adj_rl = roofline(work[7], work[6], tm["sum_adjoint"], nev, nlev,
                           "k_adj_a32 (child adjoints, FP32 matrix-core products, identity term "
-                          "in FP64)" if args.kernel == "mfma32" else
-                          f"k_adj_a<{sto}> (child adjoints, FP64 arithmetic)")
+                          "in FP64)" if m32 else
+                          f"k_adj_a<{sto}> (child adjoints, FP64 arithmetic)",
+                          "fp32" if m32 else "fp64")
+        # the branch matrices' flops are the interior children's products
         br_rl = roofline(work[9], work[8], tm["sum_branch"], nev, nev,
-                         "k_adj_m<float> (branch matrices: interior children on the FP32 matrix "
-                         "cores in 16-pattern groups summed in FP64, tips FP64)"
-                         if args.kernel == "mfma32" else
-                         f"k_adj_m<{sto}> (branch matrices, FP64 arithmetic)")
+                         "k_adj_m<float, true> (branch matrices: interior children on the FP32 "
+                         "matrix cores in 16-pattern groups summed in FP64, tips FP64)"
+                         if m32 else
+                         f"k_adj_m<{sto}> (branch matrices, FP64 arithmetic)",
+                         "fp32" if m32 else "fp64")
         rls += [(tm["sum_adjoint"], adj_rl), (tm["sum_branch"], br_rl)]
     rl = max(rls, key=lambda r: r[0])[1]
     default_workload = (args.taxa, args.codons, args.seed, args.precision, args.gradient,
@@ -390,13 +518,16 @@ def main():
         out["speedup_vs_fd64_gpu"] = value * fd_el
         out["lnL_error_vs_fp64"] = abs(f - f64)
     if args.scan_genes > 0:
-        out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, "fp64",
+        out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, args.scan_precision,
                                          args.scan_batch, args.scan_concurrent)
     if args.large_fit:
         out["large_fit"] = large_fit(e=args.large_fit_e)
+    if args.scan_cpu > 0:
+        out["genome_scan_cpu_baseline"] = cpu_scan_baseline(max(args.scan_genes, 256),
+                                                            args.scan_cpu)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(tree, seqs, p, args.codons, data.npatt, npar,
-                                           args.cpu_sample_codons)
+        out["cpu_baseline"] = cpu_baseline(tree, seqs, p, data.npatt, npar,
+                                           args.cpu_sample_codons or args.codons)
     print(json.dumps(out), flush=True)
 
 

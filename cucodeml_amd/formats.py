@@ -282,6 +282,14 @@ def code_triplet(data, c):
     return codons.codon_str(c) if c < codons.NSENSE else data.amb_triplets[c - codons.NSENSE]
 
 
+def sequences_of(data):
+    """Nucleotide sequences of `data` site by site (each code's triplet text),
+    e.g. to write the alignment back out as PHYLIP."""
+    tri = [code_triplet(data, c) for c in range(data.ncodes)]
+    cols = data.z[:, data.pose]
+    return ["".join(tri[c] for c in row) for row in cols]
+
+
 def site_amino_acid(data, species, site):
     """Amino acid of `species` at `site` as codeml prints it (code_amino_acid)."""
     return code_amino_acid(data, int(data.z[species, data.pose[site]]))

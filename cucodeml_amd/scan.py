@@ -25,7 +25,7 @@ median 400 and sigma 0.7, clipped to [50, 5000], parameters drawn as
 kappa ~ U(1, 4), w0 ~ U(0.01, 0.5), w2 ~ U(1.5, 6), p0 = 0.5, p1 = 0.3.
 """
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import ThreadPoolExecutor, as_completed
 from dataclasses import dataclass
 
 import numpy as np
@@ -172,7 +172,25 @@ def _finish_fp64(genes, x1, x0, device, e):
     return x1f, x0f, np.stack([f1, f0], axis=1), par1, info
 
 
-def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
+def batch_timing(b, ngenes):
+    """Device-side record of a batch engine's fits from its own event timers
+    (bsa_batch_timing): rounds served, device time per phase summed over
+    the rounds, wall time of the fits, launches per round."""
+    tm = b.timing()
+    rounds = max(tm["n_eval"], 1.0)
+    phases = ("pmat", "prune", "root", "adjoint", "branch", "finish")
+    dev = sum(tm["sum_" + k] for k in phases)
+    # per round: P(t) + tip tables (2), pruning levels, root + sums (2), child
+    # adjoints per depth + branch matrices, adjoint finish (3)
+    launches = 2 + tm["n_launch"] / rounds + 2 + tm["n_adj_launch"] / rounds + 3
+    return {"genes": ngenes, "rounds": int(tm["n_eval"]), "wall_ms": tm["sum_wall"],
+            "device_ms": dev, "device_busy": dev / tm["sum_wall"] if tm["sum_wall"] > 0 else 0.0,
+            "device_ms_per_round": dev / rounds, "wall_ms_per_round": tm["sum_wall"] / rounds,
+            "launches_per_round": launches,
+            "phase_ms": {k: tm["sum_" + k] for k in phases}}
+
+
+def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False, stats=None):
     """H1 and H0 fits and the LRT of a list of genes on one batch engine
     (bsa_batch_fit_lrt): the sequence of ``fit_gene`` (H1, H0 from the H1
     estimates, H1 again from the H0 estimates where H0 lies above H1), each
@@ -180,7 +198,9 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
     gene's fit.  With precision "fp32" the chains run on FP32 partials and
     are finished on FP64 ones (``_finish_fp64``); lnL, estimates and the
     statistic are then the FP64 ones, as in ``fit_gene``.  Returns the
-    per-gene dicts of ``fit_gene`` (seconds = the batch's wall time)."""
+    per-gene dicts of ``fit_gene`` (seconds = the batch's wall time).  With
+    a list `stats`, the batch's ``batch_timing`` record of its chained fits
+    is appended to it."""
     t0 = time.perf_counter()
     n = len(genes)
     b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision=precision,
@@ -192,7 +212,14 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
             s = codeml.default_start(g.tree, 2.0, 1.5, False)
             s[:g.tree.nbranch] = 0.1
             starts.append(s)
+        b.timing(reset=True)
         x1, x0, negl, info = b.fit_lrt(starts, e=e)
+        if stats is not None:
+            st = batch_timing(b, n)
+            st["evaluations"] = int(sum(inf["h1_evaluations"] + inf["h0_evaluations"]
+                                        for inf in info))
+            st["genes_per_round"] = st["evaluations"] / max(st["rounds"], 1)
+            stats.append(st)
         par1 = [b.last_params(i) for i in range(n)]
     finally:
         b.close()
@@ -232,17 +259,22 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False):
 
 
 def scan_batched(genes, batch_genes=64, concurrent=6, precision="fp64", device=0, e=1e-8,
-                 graph=False, max_waste=0.25):
+                 graph=False, max_waste=0.25, stats=None, progress=None):
     """Fit every gene (H1, H0, LRT) in batches of up to `batch_genes` genes
     of similar size, `concurrent` batches at a time.  Returns (results in
-    input order, wall seconds)."""
+    input order, wall seconds); `stats` (a list) receives each batch's
+    ``batch_timing`` record, and progress(batches done, batches, genes of
+    the batch) is called as each batch ends."""
     groups = batch_groups(genes, batch_genes, max_waste)
     out = [None] * len(genes)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=max(1, concurrent)) as ex:
-        futs = [(grp, ex.submit(fit_batch, [genes[i] for i in grp], precision, device, e, graph))
-                for grp in groups]
-        for grp, f in futs:
+        futs = {ex.submit(fit_batch, [genes[i] for i in grp], precision, device, e, graph,
+                          stats): grp for grp in groups}
+        for k, f in enumerate(as_completed(futs), 1):
+            grp = futs[f]
             for i, r in zip(grp, f.result()):
                 out[i] = r
+            if progress:
+                progress(k, len(groups), len(grp))
     return out, time.perf_counter() - t0

@@ -43,14 +43,16 @@ extern "C" {
 #define BSA_FP32          1   /* FP32 partials (FP64 arithmetic, stochastic
                                  rounding on store), power-of-two scaling  */
 
-/* pruning kernel variant */
-#define BSA_KERNEL_MFMA   0   /* 16x16x4 matrix-core contraction (default) */
-#define BSA_KERNEL_VALU   1   /* plain vector FMA contraction              */
+/* kernel variant */
+#define BSA_KERNEL_MFMA   0   /* 16x16x4 FP64 matrix-core contractions      */
+#define BSA_KERNEL_VALU   1   /* pruning by plain vector FMA contraction    */
 #define BSA_KERNEL_MFMA32 2   /* with BSA_FP32 only: as BSA_KERNEL_MFMA, with
-                                 the child adjoints of the gradient sweep on
-                                 the FP32 matrix cores (P - I in FP32, the
-                                 identity term added in FP64); the pruning
-                                 keeps FP64 products                        */
+                                 the gradient sweep's products on the FP32
+                                 matrix cores: the child adjoints (P - I in
+                                 FP32, the identity term added in FP64) and
+                                 the interior children's branch matrices
+                                 (FP32 per 16-pattern group, summed in
+                                 FP64); the pruning keeps FP64 products    */
 
 typedef struct bsa_engine bsa_engine;
 
@@ -76,7 +78,7 @@ typedef struct {
     const double *pi;         /* [61] codon frequencies                     */
     /* options */
     int precision;            /* BSA_FP64 or BSA_FP32                       */
-    int kernel;               /* BSA_KERNEL_MFMA or BSA_KERNEL_VALU         */
+    int kernel;               /* BSA_KERNEL_MFMA, _VALU or _MFMA32          */
     int device;               /* HIP device ordinal                         */
     int tile;                 /* site patterns per pruning block: 64, 128,
                                  or 0 for the default                       */
@@ -232,8 +234,7 @@ typedef struct {
 #define BSA_STOP_HEND      1  /* two successive steps changed -lnL and x by
                                  less than e relative, confirmed after a
                                  restart of the Hessian approximation (the
-                                 reference's H_end rule, tools.c:5724-5740),
-                                 with the projected gradient below 10 gtol   */
+                                 reference's H_end rule, tools.c:5724-5740) */
 #define BSA_STOP_PGRAD     2  /* projected gradient below
                                  1e-8 + 1e-10 max(1, -lnL)                   */
 #define BSA_STOP_STALL     3  /* -lnL fell by less than 1e-3 e |lnL| over the
