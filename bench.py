@@ -79,9 +79,10 @@ method = 0
 REF_CODEML = os.path.join(ROOT, "oracle", "_ref", "codeml")
 
 
-def ref_run(names, seqs, newick, ctl, xin=None, timeout=3600):
+def ref_run(names, seqs, newick, ctl, xin=None, timeout=3600, log=None):
     """One run of the reference CPU codeml in a scratch folder: (wall
-    seconds, stdout, `out` text), or None if it failed."""
+    seconds, stdout, `out` text), or None if it failed.  With `log` (a
+    path) its standard output goes to that file as it runs."""
     work = tempfile.mkdtemp(prefix="bsa_cpu_")
     try:
         formats.write_phylip(os.path.join(work, "seq.phy"), names, seqs)
@@ -93,14 +94,22 @@ def ref_run(names, seqs, newick, ctl, xin=None, timeout=3600):
         with open(os.path.join(work, "codeml.ctl"), "w") as f:
             f.write(ctl)
         t0 = time.perf_counter()
-        r = subprocess.run([REF_CODEML, "codeml.ctl"], cwd=work, capture_output=True, text=True,
-                           input="\n", timeout=timeout)
+        if log:
+            with open(log, "w") as lf:
+                r = subprocess.run([REF_CODEML, "codeml.ctl"], cwd=work, stdout=lf,
+                                   stderr=subprocess.STDOUT, text=True, input="\n",
+                                   timeout=timeout)
+            stdout = ""
+        else:
+            r = subprocess.run([REF_CODEML, "codeml.ctl"], cwd=work, capture_output=True,
+                               text=True, input="\n", timeout=timeout)
+            stdout = r.stdout
         wall = time.perf_counter() - t0
         if r.returncode != 0:
             return None
         with open(os.path.join(work, "out")) as f:
             out = f.read()
-        return wall, r.stdout, out
+        return wall, stdout, out
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -153,14 +162,15 @@ def cpu_baseline(tree, seqs, p, npatt_full, np_full, sample_codons):
                          f"gradient (ming2's gradientB)")}
 
 
-def cpu_scan_baseline(ngenes, nproc):
+def cpu_scan_baseline(ngenes, nproc, logdir=None):
     """genes/hour of the reference CPU codeml on the genome-scan workload, as
     a scan runs it: one single-thread process per gene and hypothesis (H1,
     then H0), `nproc` genes at once on `nproc` cores, each from codeml's own
     starting values (tree without branch lengths).  The sample is the nproc
     genes of the first `ngenes` whose work (patterns x taxa) is nearest the
     median: the workload's sizes are lognormal, so the mean gene costs more
-    and the figure favours the CPU."""
+    and the figure favours the CPU.  With `logdir` each run writes its
+    iteration log there (noisy = 3) as it goes."""
     if not os.path.exists(REF_CODEML):
         return None
     from concurrent.futures import ThreadPoolExecutor
@@ -171,14 +181,19 @@ def cpu_scan_baseline(ngenes, nproc):
     pick = sorted(np.argsort(np.abs(work - med))[:nproc].tolist())
     ctl1 = CTL.replace("fix_omega = 1", "fix_omega = 0").format(w2=1.5)
     ctl0 = CTL.format(w2=1.0)
+    if logdir:
+        os.makedirs(logdir, exist_ok=True)
+        ctl1, ctl0 = (c.replace("noisy = 0", "noisy = 3") for c in (ctl1, ctl0))
 
     def fit(i):
         g = genes[i]
         seqs = formats.sequences_of(g.data)
         nwk = g.tree.newick(lengths=False)
         t0 = time.perf_counter()
-        h1 = ref_run(g.tree.names, seqs, nwk, ctl1, timeout=6 * 3600)
-        h0 = ref_run(g.tree.names, seqs, nwk, ctl0, timeout=6 * 3600)
+        logs = [os.path.join(logdir, f"{g.name}_{h}.log") if logdir else None
+                for h in ("H1", "H0")]
+        h1 = ref_run(g.tree.names, seqs, nwk, ctl1, timeout=6 * 3600, log=logs[0])
+        h0 = ref_run(g.tree.names, seqs, nwk, ctl0, timeout=6 * 3600, log=logs[1])
         el = time.perf_counter() - t0
         print(f"reference fits of {g.name} ({g.tree.ns} taxa x {g.data.npatt} patterns): "
               f"{el:.0f} s", file=sys.stderr, flush=True)
@@ -394,6 +409,8 @@ def main():
     ap.add_argument("--scan-cpu", type=int, default=0,
                     help="also time the reference CPU codeml on this many scan genes at once "
                          "(one core each; full H1 + H0 fits: tens of minutes)")
+    ap.add_argument("--scan-cpu-log", default=None,
+                    help="folder for the reference runs' iteration logs (--scan-cpu)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scan-genes", type=int, default=256,
                     help="genes of the genome-scan sample timed beside (0: skip)")
@@ -524,7 +541,7 @@ def main():
         out["large_fit"] = large_fit(e=args.large_fit_e)
     if args.scan_cpu > 0:
         out["genome_scan_cpu_baseline"] = cpu_scan_baseline(max(args.scan_genes, 256),
-                                                            args.scan_cpu)
+                                                            args.scan_cpu, args.scan_cpu_log)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tree, seqs, p, data.npatt, npar,
                                            args.cpu_sample_codons or args.codons)
