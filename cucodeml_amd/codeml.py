@@ -129,26 +129,41 @@ def _bounds(eng_bounds, x0, nb, fixed_branches):
     return lo, hi
 
 
+# Newton iterations of the last phase of an FP64 finish
+FINISH_NEWTON = 8
+
+
 def finish_fp64(fit, x, lo, hi, e, gradient="adjoint"):
     """FP64 finish of a fit that ended at the FP32 rounding-noise floor:
     Newton iterations from there (bsa_fit_opts.newton: the Hessian from
     differences of the adjoint gradient, which lands on the maximum also
     along flat directions such as w2 on a ridge, where a BFGS restart from
     the identity would stall), and, where they do not converge, BFGS
-    followed by Newton iterations.  ``fit`` is an engine's fit method."""
+    followed by Newton iterations, and where those do not converge either,
+    FINISH_NEWTON more Newton iterations.  An FP32 chain can stop far along
+    a curved, nearly flat ridge (p2 w2 ~ const: one scan gene stopped at
+    w2 = 2.58 where the maximum has 1.91, 2e-4 below it in lnL); Newton
+    steps follow such a ridge at a linear rate and BFGS crawls, so the
+    last phase gives Newton the iterations to get there.  ``fit`` is an
+    engine's fit method."""
     if gradient == "adjoint":
         x1, f1, i1 = fit(x, lo, hi, e=e, max_iter=-1, newton=3)
         if i1["converged"]:
             return x1, f1, i1
-        x2, f2, i2 = fit(x1, lo, hi, e=e, newton=2)
-        return x2, f2, _merge_info(i1, i2)
+        x2, f2, i2 = fit(x1, lo, hi, e=e, newton=4)
+        i2 = _merge_info(i1, i2)
+        if i2["converged"]:
+            return x2, f2, i2
+        x3, f3, i3 = fit(x2, lo, hi, e=e, max_iter=-1, newton=FINISH_NEWTON)
+        return x3, f3, _merge_info(i2, i3)
     return fit(x, lo, hi, e=e, gradient=gradient)
 
 
 def finish_fp64_batch(b, xs, los, his, e):
     """``finish_fp64`` for every gene of an FP64 batch engine: Newton
     iterations for all genes, then BFGS and Newton for those that did not
-    converge, the others held where they are (lo = hi).  los / his: per-gene
+    converge, then Newton alone for those that still did not, the others
+    held where they are (lo = hi).  los / his: per-gene
     bounds (None entries: codeml's).  Returns (list of x, -lnL, info)."""
     n = len(xs)
     x1, f1, i1 = b.fit(xs, e=e, lo=los, hi=his, max_iter=-1, newton=3)
@@ -157,9 +172,17 @@ def finish_fp64_batch(b, xs, los, his, e):
         return x1, f1, i1
     lo2 = [los[i] if i in redo else x1[i] for i in range(n)]
     hi2 = [his[i] if i in redo else x1[i] for i in range(n)]
-    x2, f2, i2 = b.fit(x1, e=e, lo=lo2, hi=hi2, newton=2)
+    x2, f2, i2 = b.fit(x1, e=e, lo=lo2, hi=hi2, newton=4)
     for i in redo:
         x1[i], f1[i], i1[i] = x2[i], f2[i], _merge_info(i1[i], i2[i])
+    redo = [i for i in redo if not i1[i]["converged"]]
+    if not redo:
+        return x1, f1, i1
+    lo3 = [los[i] if i in redo else x1[i] for i in range(n)]
+    hi3 = [his[i] if i in redo else x1[i] for i in range(n)]
+    x3, f3, i3 = b.fit(x1, e=e, lo=lo3, hi=hi3, max_iter=-1, newton=FINISH_NEWTON)
+    for i in redo:
+        x1[i], f1[i], i1[i] = x3[i], f3[i], _merge_info(i1[i], i3[i])
     return x1, f1, i1
 
 

@@ -45,7 +45,7 @@
 //     flat directions (w2 on a ridge) that BFGS's curvature estimate, and the
 //     H_end rule, resolve poorly.
 //     With `verify` (exact FP64 gradients), a stall or no-descent exit is
-//     checked the same way by up to two Newton iterations.
+//     checked the same way by up to four Newton iterations.
 //     BSA_STOP_HEND, BSA_STOP_PGRAD and BSA_STOP_NEWTON are convergence.
 #include <algorithm>
 #include <cmath>
@@ -179,6 +179,12 @@ static int newton_finish(FG &fg, int n, double *x, const double *lo, const doubl
     }
     return BSA_OK;
 }
+
+// Newton iterations that check a stall or no-descent exit: a BFGS crawling
+// along a curved, nearly flat ridge (p2 w2 ~ const) near the maximum can end
+// by the stall rule a few 1e-6 of lnL short of it; from there the first two
+// Newton iterations close the gap and the later ones show the step vanish
+constexpr int kVerifyNewton = 4;
 
 int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const double *hi,
                  const bsa_fit_opts *opts, FitResult *res, bool verify)
@@ -355,7 +361,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
         it += newton;
     } else if (verify && (stop == BSA_STOP_STALL || stop == BSA_STOP_NODESCENT)) {
         int st = 0;
-        if ((rc = newton_finish(fg, n, x, lo, hi, 2, &f, g, &st))) return rc;
+        if ((rc = newton_finish(fg, n, x, lo, hi, kVerifyNewton, &f, g, &st))) return rc;
         if (st) stop = BSA_STOP_NEWTON;
     }
     res->f = f;

@@ -17,7 +17,7 @@ struct FitResult {
 // minimise fg over the box [lo, hi] from x (moved into the box); x holds
 // the minimiser on return
 // verify: where the quasi-Newton phase ends by the stall rule or without a
-// descent step, and opts asks for no Newton iterations, up to two Newton
+// descent step, and opts asks for no Newton iterations, up to four Newton
 // iterations check whether x is the maximum (BSA_STOP_NEWTON if so); for
 // exact FP64 gradients (the adjoint on FP64 partials), whose differences give
 // the Hessian

@@ -57,7 +57,7 @@ class _FitOpts(C.Structure):
 
 
 BSA_GRAD_ADJOINT, BSA_GRAD_FD = 0, 1
-# why a fit stopped (BSA_STOP_*); only "hend" and "pgrad" are convergence
+# why a fit stopped (BSA_STOP_*); "hend", "pgrad" and "newton" are convergence
 STOP_NAMES = {0: "limit", 1: "hend", 2: "pgrad", 3: "stall", 4: "nodescent", 5: "newton"}
 CONVERGED_STOPS = ("hend", "pgrad", "newton")
 
