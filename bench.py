@@ -389,9 +389,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
     ap.add_argument("--gradient", choices=["adjoint", "fd"], default="adjoint")
-    ap.add_argument("--kernel", choices=["mfma", "mfma32", "valu"], default=None,
+    ap.add_argument("--kernel", choices=["mfma", "mfma32", "fp32mc", "valu"], default=None,
                     help="default: mfma32 with fp32 partials (FP32 matrix-core child adjoints), "
-                         "mfma with fp64")
+                         "mfma with fp64; fp32mc: the pruning on the FP32 matrix cores too")
     ap.add_argument("--tile", type=int, default=0, help="patterns per pruning block (64/128)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the default launch shapes (no autotuning)")
@@ -454,15 +454,20 @@ def main():
     work = eng.work()
     nev = tm["n_eval"]
     sto = "double" if args.precision == "fp64" else "float"
-    prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
-                        (f"k_prune_res/k_prune_mfma<{sto}> (pruning levels, FP64 arithmetic, "
-                         f"{sto} storage)") if args.kernel != "valu" else
-                        f"k_prune_valu<{sto}> (FP64 arithmetic, {sto} storage)")
+    if args.kernel == "fp32mc":
+        prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
+                            "k_prune32 (pruning levels on the FP32 matrix cores, each row's two "
+                            "largest child terms and the identity term in FP64)", "fp32")
+    else:
+        prune_rl = roofline(work[1], work[0], tm["sum_prune"], nev, tm["n_launch"],
+                            (f"k_prune_res/k_prune_mfma<{sto}> (pruning levels, FP64 arithmetic, "
+                             f"{sto} storage)") if args.kernel != "valu" else
+                            f"k_prune_valu<{sto}> (FP64 arithmetic, {sto} storage)")
     rls = [(tm["sum_prune"], prune_rl)]
     adj_rl = br_rl = None
     if args.gradient == "adjoint":
         nlev = tm["n_adj_launch"] - nev   # child-adjoint launches (one branch-matrix launch each)
-        m32 = args.kernel == "mfma32"
+        m32 = args.kernel in ("mfma32", "fp32mc")
         adj_rl = roofline(work[7], work[6], tm["sum_adjoint"], nev, nlev,
                           "k_adj_a32 (child adjoints, FP32 matrix-core products, identity term "
                           "in FP64)" if m32 else
@@ -497,7 +502,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else (
             "f32-storage/f64-arith (child adjoints f32 matrix cores)" if args.kernel == "mfma32"
-            else "f32-storage/f64-arith"),
+            else "f32 (pruning and gradient sweep on f32 matrix cores, f64 terms)"
+            if args.kernel == "fp32mc" else "f32-storage/f64-arith"),
         "data": "synthetic: seeded model-A simulation (Yule tree, F3x4 Dirichlet(1))",
         "config": {"workload": f"branch-site model A, {args.taxa} taxa x {args.codons} codons "
                                f"({data.npatt} patterns), {args.precision} partials, "

@@ -421,10 +421,11 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
     if (genes.empty()) return fail(BSA_EINVAL, "no genes");
     if (precision != BSA_FP64 && precision != BSA_FP32)
         return fail(BSA_EINVAL, "unknown precision %d", precision);
-    if (kernel != BSA_KERNEL_MFMA && kernel != BSA_KERNEL_VALU && kernel != BSA_KERNEL_MFMA32)
+    if (kernel != BSA_KERNEL_MFMA && kernel != BSA_KERNEL_VALU && kernel != BSA_KERNEL_MFMA32 &&
+        kernel != BSA_KERNEL_FP32MC)
         return fail(BSA_EINVAL, "unknown kernel %d", kernel);
-    if (kernel == BSA_KERNEL_MFMA32 && precision != BSA_FP32)
-        return fail(BSA_EINVAL, "the FP32 matrix-core kernel needs FP32 partials (BSA_FP32)");
+    if ((kernel == BSA_KERNEL_MFMA32 || kernel == BSA_KERNEL_FP32MC) && precision != BSA_FP32)
+        return fail(BSA_EINVAL, "the FP32 matrix-core kernels need FP32 partials (BSA_FP32)");
     tile = tile == 0 ? 64 : tile;
     if (tile != 64 && tile != 128) return fail(BSA_EINVAL, "tile %d: 64 or 128 patterns per block", tile);
     f.precision = precision;
@@ -890,13 +891,15 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
                 (double *)f.d_PT, f.d_mask_amb, f.ncodes, (double *)f.d_AT);
     HIPCHK(hipGetLastError());
     REC(1);
-    // the pruning keeps FP64 products (BSA_KERNEL_MFMA32 moves the child
-    // adjoints and the interior children's branch matrices to the FP32
-    // matrix cores)
-    const bool mfma32 = adjoint && f.kernel == BSA_KERNEL_MFMA32 && sizeof(T) == 4;
+    // BSA_KERNEL_MFMA32 moves the child adjoints and the interior children's
+    // branch matrices to the FP32 matrix cores, BSA_KERNEL_FP32MC the
+    // pruning's products too
+    const bool mfma32 = adjoint && sizeof(T) == 4 &&
+                        (f.kernel == BSA_KERNEL_MFMA32 || f.kernel == BSA_KERNEL_FP32MC);
+    const int pk = f.kernel == BSA_KERNEL_VALU ? 1 : f.kernel == BSA_KERNEL_FP32MC ? 2 : 0;
     for (size_t l = 0; l + 1 < f.level_off.size(); l++) {
         const int a = f.level_off[l], b = f.level_off[l + 1];
-        launch_prune<T>(s, f.kernel == BSA_KERNEL_VALU ? 1 : 0, f.tile,
+        launch_prune<T>(s, pk, f.tile,
                         rs ? f.d_prune : f.d_prune + a, rs ? rs->lev[l] : nullptr,
                         rs ? rs->nlev[l] : b - a, f.level_maxint[l], f.npad, f.d_z,
                         (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes, (T *)f.d_L,

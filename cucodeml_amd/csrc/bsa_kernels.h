@@ -128,7 +128,8 @@ void launch_adjoint_finish(hipStream_t s, const double *Mpart, int nchunk, const
 void launch_pmat(hipStream_t s, const double *uvr, const PmatJob *jobs, const int *sel, int njobs,
                  double *PT, const unsigned char *mask_amb, int ncodes, double *TT);
 
-// kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64)
+// kernel 0 = MFMA (tile 64 or 128 patterns per block), 1 = VALU (tile 64),
+// 2 = FP32 matrix cores (float storage; 128-pattern blocks, tile ignored)
 // Cb: contribution buffer (interior children's P I per class) or nullptr;
 // maxint: the most interior children of any job in the launch
 template <typename T>

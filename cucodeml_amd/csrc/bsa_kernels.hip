@@ -4,6 +4,8 @@
 //   k_prune_mfma  one tree level of Felsenstein pruning on the FP64 matrix cores
 //   k_prune_res   the same with the children's P resident in LDS (big levels)
 //   k_prune_valu  the same on plain vector FMA
+//   k_prune32     the same on the FP32 matrix cores, each pattern row's two largest
+//                 child terms and the identity term in FP64 (BSA_KERNEL_FP32MC)
 //   k_root        per-pattern class log-likelihoods, mixture and weighted sum
 //   k_adj_a       adjoints of one depth's interior children (preorder sweep)
 //   k_adj_a32     the same on the FP32 matrix cores (BSA_KERNEL_MFMA32)
@@ -12,8 +14,9 @@
 //   k_eigbranch, k_wreduce, k_zform   eigen-basis derivatives and their sums
 //   k_beb_*       Bayes empirical Bayes over the 10^4-point grid
 //
-// Arithmetic is FP64 (k_adj_a32 aside); T (double or float) is only the
-// storage type of the conditional likelihoods.  Layouts:
+// Arithmetic is FP64 (k_adj_a32, k_prune32 and the FP32 branch-matrix path
+// aside); T (double or float) is only the storage type of the conditional
+// likelihoods.  Layouts:
 //   PT[pm][k][j]    = P_{j,k}(t) - delta_{jk} (double) for matrix pm = node*3 + set,
 //                     64 x 64, rows and columns >= 61 zero (k: child state,
 //                     j: parent state).  P - I is computed without the
@@ -465,6 +468,216 @@ __device__ inline f4 mfma32(float a, float b, f4 c)
 }
 
 constexpr int kT32 = 65;   // floats per row of a wave's transpose tile in LDS
+
+// Top two (value, state) of a pattern row whose 64 states are spread over
+// the 4 lanes 16g + r (g = 0..3) of a wave, 16 states per lane (a[kk] is
+// state 16g + kk): order by value, then by the lower state; afterwards every
+// one of the 4 lanes holds the row's pair.
+struct Top2 {
+    float v1, v2;
+    int c1, c2;
+};
+__device__ inline bool top_before(float va, int ca, float vb, int cb)
+{
+    return va > vb || (va == vb && ca < cb);
+}
+__device__ inline Top2 row_top2(const float *a, int g)
+{
+    Top2 t{-1.f, -1.f, 64, 64};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+        const float v = a[kk];
+        const int c = 16 * g + kk;
+        if (top_before(v, c, t.v1, t.c1)) {
+            t.v2 = t.v1;
+            t.c2 = t.c1;
+            t.v1 = v;
+            t.c1 = c;
+        } else if (top_before(v, c, t.v2, t.c2)) {
+            t.v2 = v;
+            t.c2 = c;
+        }
+    }
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+        const float pv1 = __shfl_xor(t.v1, off, 64), pv2 = __shfl_xor(t.v2, off, 64);
+        const int pc1 = __shfl_xor(t.c1, off, 64), pc2 = __shfl_xor(t.c2, off, 64);
+        if (top_before(pv1, pc1, t.v1, t.c1)) {
+            if (!top_before(t.v1, t.c1, pv2, pc2)) {
+                t.v2 = pv2;
+                t.c2 = pc2;
+            } else {
+                t.v2 = t.v1;
+                t.c2 = t.c1;
+            }
+            t.v1 = pv1;
+            t.c1 = pc1;
+        } else if (top_before(pv1, pc1, t.v2, t.c2)) {
+            t.v2 = pv1;
+            t.c2 = pc1;
+        }
+    }
+    return t;
+}
+
+// One tree level of pruning with the products on the FP32 matrix cores
+// (BSA_KERNEL_FP32MC, float storage), for jobs with at most two interior
+// children; blocks as k_prune_res (8 waves, 128 patterns per tile, both
+// children's fl32(P - I) resident in LDS).  Per interior child and pattern h
+//   C(h, j) = L(h, j) + sum_{k in top2(h)} L(h, k) (P - I)_{j,k}
+//                     + sum_{k not in top2(h)} L(h, k) fl32(P - I)_{j,k},
+// the first two terms in FP64, the last on the matrix cores (16x16x4, 16
+// k-steps x 4 column blocks).  The matrix core adds each product to its FP32
+// accumulator in turn with round-to-nearest, so a term below half an ulp of
+// the running sum is lost outright: with every off-diagonal term >= 0, the
+// plain contraction loses the smaller terms systematically (lnL 1.5e-3 too
+// low at 128 taxa x 10,000 codons).  The two largest L(h, k) of a row carry
+// almost all of a row's magnitude; without them the accumulator holds only
+// terms of comparable size.  L(h, j) comes from a transpose tile of the
+// wave's rows in LDS; accumulator element i of lane (g, r) is pattern
+// 4g + i of the wave's 16, parent state 16b + r.  Children, scaling and
+// stores as in k_prune_res.
+__global__ __launch_bounds__(64 * kResWaves, 4) void k_prune32(const PruneJob *__restrict__ jobs,
+                                                              const int *__restrict__ sel, int npad,
+                                                              const unsigned char *__restrict__ z,
+                                                              const double *__restrict__ PT,
+                                                              const double *__restrict__ TT,
+                                                              int ncodes, float *__restrict__ L,
+                                                              int *__restrict__ E,
+                                                              float *__restrict__ Cb, int tpb)
+{
+    __shared__ float sP[2][64 * kT32];          // fl32(P - I)[k][j] per interior child
+    __shared__ float sX[kResWaves][16 * kT32];  // per wave: its 16 child rows
+    __shared__ Top2 sT[kResWaves][16];           // per wave: each row's top two
+    const PruneJob &jb = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    const int nch = jb.nch;
+    {
+        int k = 0;
+        for (int c = 0; c < nch && k < 2; c++) {
+            if (jb.ch_tip[c] >= 0) continue;
+            const double *P = PT + (size_t)jb.ch_pmat[c] * 4096;
+            for (int q = tid; q < 4096; q += 64 * kResWaves)
+                sP[k][(q >> 6) * kT32 + (q & 63)] = (float)P[q];
+            k++;
+        }
+    }
+    __syncthreads();
+    float *xw = sX[w];
+    float *Lo = L + (size_t)jb.out_slot * npad * 64;
+    for (int t = 0; t < tpb; t++) {
+        const int hw = (blockIdx.x * tpb + t) * (16 * kResWaves) + 16 * w;
+        if (hw >= npad) break;   // the same for the whole wave
+        d4 acc[4];
+        int esum[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            acc[b] = d4{1, 1, 1, 1};
+            esum[b] = 0;
+        }
+        int k = 0;
+        for (int c = 0; c < nch; c++) {
+            const int pm = jb.ch_pmat[c];
+            const int tip = jb.ch_tip[c];
+            if (tip >= 0) {
+                const unsigned char *zt = z + (size_t)tip * npad + hw + 4 * g;
+                int code[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) code[i] = zt[i];
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        acc[b][i] *= tip_value(TT, pm, ncodes, code[i], 16 * b + r);
+                continue;
+            }
+            const int slot = jb.ch_slot[c];
+            const float *Lc = L + (size_t)slot * npad * 64;
+            const int *Ec = E + (size_t)slot * npad;
+            float a[16];
+            {
+                const f4 *q = reinterpret_cast<const f4 *>(Lc + (size_t)(hw + r) * 64 + 16 * g);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const f4 v = q[i];
+                    a[4 * i] = v.x;
+                    a[4 * i + 1] = v.y;
+                    a[4 * i + 2] = v.z;
+                    a[4 * i + 3] = v.w;
+                }
+            }
+            // the previous child's reads of xw (the wave's own rows) come first
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) xw[r * kT32 + 16 * g + kk] = a[kk];
+            const Top2 tp = row_top2(a, g);
+            if (g == 0) sT[w][r] = tp;
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++)
+                if (16 * g + kk == tp.c1 || 16 * g + kk == tp.c2) a[kk] = 0.f;
+            __builtin_amdgcn_wave_barrier();
+            const float *Pk = sP[k];
+            f4 Y[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) Y[b] = f4{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) {
+                const float *prow = Pk + (16 * g + kk) * kT32 + r;
+#pragma unroll
+                for (int b = 0; b < 4; b++) Y[b] = mfma32(a[kk], prow[16 * b], Y[b]);
+            }
+            float *Co = Cb ? Cb + (size_t)jb.ch_cslot[c] * npad * 64 : nullptr;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                // row 4g + i of the wave (its top two from LDS, one address
+                // per lane group: a broadcast)
+                const int src = 4 * g + i;
+                const Top2 tq = sT[w][src];
+                const float v1 = tq.v1, v2 = tq.v2;
+                const int c1 = tq.c1 & 63, c2 = tq.c2 & 63;
+                const int h = hw + src;
+                uint32_t rb = Co ? row_seed((uint32_t)jb.ch_cslot[c] ^ 0x3C6EF372u, (uint32_t)h,
+                                            (uint32_t)r)
+                                 : 0u;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int j = 16 * b + r;
+                    const double cv = (double)xw[src * kT32 + j] +
+                                      (double)v1 * (double)Pk[c1 * kT32 + j] +
+                                      (double)v2 * (double)Pk[c2 * kT32 + j] + (double)Y[b][i];
+                    if (Co) {
+                        Store<float>::put(Co + (size_t)h * 64 + j, cv, rb);
+                        rb = xorshift32(rb);
+                    }
+                    acc[b][i] *= cv;
+                }
+                esum[i] += Ec[h];
+            }
+            k++;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double m = acc[0][i];
+#pragma unroll
+            for (int b = 1; b < 4; b++) m = acc[b][i] > m ? acc[b][i] : m;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                double o = __shfl_xor(m, off, 64);
+                m = o > m ? o : m;
+            }
+            int e = 0;
+            if (m > 0) frexp(m, &e);
+            const int h = hw + 4 * g + i;
+            uint32_t rb = row_seed((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)r);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                Store<float>::put(Lo + (size_t)h * 64 + 16 * b + r, ldexp(acc[b][i], -e), rb);
+                rb = xorshift32(rb);
+            }
+            if (r == 0) E[(size_t)jb.out_slot * npad + h] = esum[i] + e;
+        }
+    }
+}
 
 // ------------------------------------------------------------------ pruning (VALU)
 
@@ -1300,6 +1513,16 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
     // and the launch has at least half as many blocks of 8 waves as CUs
     // (below that the 4-wave blocks spread the work better)
     const int nres = (npad + 16 * kResWaves - 1) / (16 * kResWaves);
+    if constexpr (sizeof(T) == 4) {
+        // FP32 matrix-core products (jobs with more than two interior
+        // children keep the FP64 kernel)
+        if (kernel == 2 && maxint <= 2) {
+            hipLaunchKernelGGL(k_prune32, dim3(nres, njobs), dim3(64 * kResWaves), 0, s, jobs, sel,
+                               npad, z, PT, TT, ncodes, L, E, Cb, 1);
+            return;
+        }
+    }
+    if (kernel == 2) kernel = 0;
     if (kernel == 0 && tile == 64 && maxint <= 2 && (long)nres * njobs >= 128) {
         hipLaunchKernelGGL(k_prune_res<T>, dim3(nres, njobs), dim3(64 * kResWaves), 0, s, jobs,
                            sel, npad, z, PT, TT, ncodes, L, E, Cb, 1);

@@ -18,7 +18,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libbsa.so")
 
 BSA_FP64, BSA_FP32 = 0, 1
-BSA_KERNEL_MFMA, BSA_KERNEL_VALU, BSA_KERNEL_MFMA32 = 0, 1, 2
+BSA_KERNEL_MFMA, BSA_KERNEL_VALU, BSA_KERNEL_MFMA32, BSA_KERNEL_FP32MC = 0, 1, 2, 3
 
 EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
@@ -172,7 +172,8 @@ def _problem(tree, data, precision, kernel, device, tile, pi, keep):
                     z=z.ctypes.data_as(C.POINTER(C.c_ubyte)), ncodes=data.ncodes,
                     code_mask=mask.ctypes.data_as(C.POINTER(C.c_ubyte)), fpatt=_dptr(fp),
                     pi=_dptr(pv), precision=BSA_FP32 if precision == "fp32" else BSA_FP64,
-                    kernel={"valu": BSA_KERNEL_VALU, "mfma32": BSA_KERNEL_MFMA32}.get(kernel, BSA_KERNEL_MFMA),
+                    kernel={"valu": BSA_KERNEL_VALU, "mfma32": BSA_KERNEL_MFMA32,
+                            "fp32mc": BSA_KERNEL_FP32MC}.get(kernel, BSA_KERNEL_MFMA),
                     device=device, tile=tile)
 
 

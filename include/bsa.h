@@ -53,6 +53,13 @@ extern "C" {
                                  the interior children's branch matrices
                                  (FP32 per 16-pattern group, summed in
                                  FP64); the pruning keeps FP64 products    */
+#define BSA_KERNEL_FP32MC 3   /* with BSA_FP32 only: as BSA_KERNEL_MFMA32,
+                                 with the pruning's products on the FP32
+                                 matrix cores too: per pattern row the two
+                                 largest child terms and the identity term
+                                 in FP64, the rest of P - I in FP32 (jobs
+                                 with more than two interior children keep
+                                 FP64 products)                            */
 
 typedef struct bsa_engine bsa_engine;
 
@@ -78,7 +85,7 @@ typedef struct {
     const double *pi;         /* [61] codon frequencies                     */
     /* options */
     int precision;            /* BSA_FP64 or BSA_FP32                       */
-    int kernel;               /* BSA_KERNEL_MFMA, _VALU or _MFMA32          */
+    int kernel;               /* BSA_KERNEL_MFMA, _VALU, _MFMA32, _FP32MC   */
     int device;               /* HIP device ordinal                         */
     int tile;                 /* site patterns per pruning block: 64, 128,
                                  or 0 for the default                       */

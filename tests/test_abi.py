@@ -67,6 +67,8 @@ def small():
     (dict(npatt=0), "npatt"),
     (dict(precision=7), "precision"),
     (dict(kernel=5), "kernel"),
+    (dict(kernel=2), "FP32 partials"),     # BSA_KERNEL_MFMA32 with FP64 partials
+    (dict(kernel=3), "FP32 partials"),     # BSA_KERNEL_FP32MC with FP64 partials
     (dict(ncodes=300), "ncodes"),
     (dict(tile=96), "tile"),
 ])
