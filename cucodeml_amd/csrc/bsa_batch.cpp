@@ -488,7 +488,7 @@ extern "C" int bsa_batch_fit_lrt(bsa_batch *b, double *const *x1, double *const 
     return BSA_OK;
 }
 
-extern "C" int bsa_batch_tune(bsa_batch *b, const double *const *x, int reps, double *out6)
+extern "C" int bsa_batch_tune(bsa_batch *b, const double *const *x, int reps, double *out7)
 {
     if (!b || !x) return fail(BSA_EINVAL, "null argument");
     const int G = (int)b->genes.size();
@@ -500,7 +500,7 @@ extern "C" int bsa_batch_tune(bsa_batch *b, const double *const *x, int reps, do
         gp[i] = gs[i].data();
     }
     return forest_tune(b->f, [&]() { return batch_eval(b, x, negl.data(), gp.data()); }, reps,
-                       out6);
+                       out7);
 }
 
 extern "C" int bsa_batch_class_lnl(bsa_batch *b, int gene, double *fhK, double *site)
@@ -568,6 +568,12 @@ extern "C" int bsa_batch_set_graph(bsa_batch *b, int enable)
     if (!b) return fail(BSA_EINVAL, "null batch");
     b->f.use_graph = enable != 0;
     return BSA_OK;
+}
+
+extern "C" int bsa_batch_set_prune(bsa_batch *b, int mode)
+{
+    if (!b) return fail(BSA_EINVAL, "null batch");
+    return forest_set_prune(b->f, mode);
 }
 
 // Plan a forest of the given genes on the host only (no device work) and

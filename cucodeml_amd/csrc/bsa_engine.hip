@@ -437,13 +437,19 @@ extern "C" int bsa_set_shapes(bsa_engine *e, int tile, int tpb, int tpc)
     return forest_set_shapes(e->f, tile, tpb, tpc);
 }
 
-extern "C" int bsa_tune(bsa_engine *e, const double *x, int np, int reps, double *out6)
+extern "C" int bsa_set_prune(bsa_engine *e, int mode)
+{
+    if (!e) return fail(BSA_EINVAL, "null engine");
+    return forest_set_prune(e->f, mode);
+}
+
+extern "C" int bsa_tune(bsa_engine *e, const double *x, int np, int reps, double *out7)
 {
     if (!e || !x) return fail(BSA_EINVAL, "null argument");
     if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
     std::vector<double> g(np);
     double f;
-    return forest_tune(e->f, [&]() { return bsa_lnl_grad(e, x, np, &f, g.data()); }, reps, out6);
+    return forest_tune(e->f, [&]() { return bsa_lnl_grad(e, x, np, &f, g.data()); }, reps, out7);
 }
 
 extern "C" int bsa_default_bounds(const bsa_engine *e, double *lo, double *hi)

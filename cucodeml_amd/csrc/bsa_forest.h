@@ -104,6 +104,13 @@ struct Forest {
     std::vector<PruneJob> prune;
     std::vector<int> level_off;
     std::vector<int> level_maxint;            // most interior children of a job, per level
+    // persistent pruning (one launch for every level, k_prune_walk): each
+    // gene's jobs in level order (wjobs[woff[g] .. woff[g + 1]), indices into
+    // prune), the genes by falling job count (walk_order), the items of the
+    // whole forest (gene * ntile + tile, genes in walk_order)
+    std::vector<int> wjobs, woff, walk_order, witems;
+    int prune_walk = 0;                       // 1: pruning by k_prune_walk
+    int walk_nblk = 0;                        // its grid: 2 blocks per CU
 
     // device
     hipStream_t stream = nullptr;
@@ -114,6 +121,7 @@ struct Forest {
     double *d_uvr = nullptr;
     PmatJob *d_pjobs = nullptr;
     PruneJob *d_prune = nullptr;
+    int *d_wjobs = nullptr, *d_woff = nullptr, *d_witems = nullptr;
     RootArgs *d_rargs = nullptr;                   // [2][G]: without, with adjoint seeds
     void *d_PT = nullptr, *d_AT = nullptr, *d_L = nullptr;
     int *d_E = nullptr;
@@ -213,11 +221,16 @@ int forest_lnl_in_place(Forest &f, char *base, size_t cap, hipStream_t s, const 
 // gene size, up to kAdjTilesPerChunkAuto, else 1..kAdjTilesPerChunk).
 // Captured graphs are dropped.
 int forest_set_shapes(Forest &f, int tile, int tpb, int tpc);
-// Autotuner: for each shape parameter in turn (pruning tile, child-adjoint
-// tiles per block, branch-matrix tiles per chunk) every candidate is timed
-// over `reps` calls of eval (one evaluation with gradient each) by the
-// device time of the evaluation, and the fastest is kept.  out[6] = {tile,
-// tpb, tpc, chunks, best ms, ms of the shapes it started from}.
-int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *out6);
+// Pruning by one launch per level (BSA_PRUNE_LEVELS) or by the persistent
+// walk (BSA_PRUNE_WALK, FP64-product kernels only); captured graphs are
+// dropped.
+int forest_set_prune(Forest &f, int mode);
+// Autotuner: for each parameter in turn (pruning mode, pruning tile,
+// child-adjoint tiles per block, branch-matrix tiles per chunk) every
+// candidate is timed over `reps` calls of eval (one evaluation with
+// gradient each) by the device time of the evaluation, and the fastest is
+// kept.  out[7] = {tile, tpb, tpc, chunks, best ms, ms of the shapes it
+// started from, pruning mode}.
+int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *out7);
 
 }  // namespace bsa

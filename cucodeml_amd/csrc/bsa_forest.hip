@@ -398,7 +398,7 @@ void forest_free(Forest &f)
 {
     // best-effort release: errors cannot be reported here and are ignored
     if (f.stream) (void)hipSetDevice(f.dev);
-    void *dev_ptrs[] = {f.d_uvr, f.d_pjobs, f.d_prune, f.d_rargs, f.d_PT, f.d_AT, f.d_L, f.d_E,
+    void *dev_ptrs[] = {f.d_uvr, f.d_pjobs, f.d_prune, f.d_wjobs, f.d_woff, f.d_witems, f.d_rargs, f.d_PT, f.d_AT, f.d_L, f.d_E,
                         f.d_z, f.d_mask_amb, f.d_pi, f.d_fpatt, f.d_fh, f.d_site, f.d_partial,
                         f.d_out, f.d_adj, f.d_ejobs, f.d_set_off, f.d_set_list, f.d_A, f.d_C,
                         f.d_EA, f.d_Mpart, f.d_W, f.d_G, f.d_dtau, f.d_Z, f.d_mask64, f.d_tperm,
@@ -508,6 +508,25 @@ int forest_plan(Forest &f, const std::vector<Gene *> &genes, int precision, int 
         f.level_maxint.push_back(maxint);
         f.level_off.push_back((int)f.prune.size());
     }
+    // persistent pruning: each gene's jobs in level order, genes by falling
+    // job count (the largest first, so the static round-robin of items over
+    // blocks ends evenly), every tile of a gene
+    f.wjobs.clear();
+    f.woff.assign(1, 0);
+    for (Gene *gp : genes) {
+        for (const auto &rg : gp->prune_rng)
+            for (int i = rg[0]; i < rg[1]; i++) f.wjobs.push_back(i);
+        f.woff.push_back((int)f.wjobs.size());
+    }
+    f.walk_order.resize(f.G);
+    for (int i = 0; i < f.G; i++) f.walk_order[i] = i;
+    std::stable_sort(f.walk_order.begin(), f.walk_order.end(), [&](int a, int b) {
+        return f.woff[a + 1] - f.woff[a] > f.woff[b + 1] - f.woff[b];
+    });
+    const int wt = f.npad / kPadP;
+    f.witems.clear();
+    for (int gi : f.walk_order)
+        for (int t = 0; t < wt; t++) f.witems.push_back(gi * wt + t);
     for (double &v : f.work) v = 0;
     for (Gene *gp : genes)
         for (int k = 0; k < 4; k++) f.work[k] += gp->work[k];
@@ -604,6 +623,19 @@ int forest_upload(Forest &f, int device)
     if (!f.prune.empty())
         HIPCHK(hipMemcpy(f.d_prune, f.prune.data(), sizeof(PruneJob) * f.prune.size(),
                          hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&f.d_wjobs, sizeof(int) * std::max<size_t>(1, f.wjobs.size())));
+    HIPCHK(hipMalloc(&f.d_woff, sizeof(int) * f.woff.size()));
+    HIPCHK(hipMalloc(&f.d_witems, sizeof(int) * std::max<size_t>(1, f.witems.size())));
+    if (!f.wjobs.empty())
+        HIPCHK(hipMemcpy(f.d_wjobs, f.wjobs.data(), sizeof(int) * f.wjobs.size(),
+                         hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f.d_woff, f.woff.data(), sizeof(int) * f.woff.size(), hipMemcpyHostToDevice));
+    if (!f.witems.empty())
+        HIPCHK(hipMemcpy(f.d_witems, f.witems.data(), sizeof(int) * f.witems.size(),
+                         hipMemcpyHostToDevice));
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, f.dev));
+    f.walk_nblk = 2 * std::max(1, ncu);
     return BSA_OK;
 }
 
@@ -799,6 +831,22 @@ int forest_check(const Forest &f)
             }
         }
     }
+    // persistent-pruning lists: each gene's jobs, in level order, exactly
+    // once; items within the forest's tiles
+    if ((long)f.woff.size() != f.G + 1 || f.woff[0] != 0 ||
+        f.woff.back() != (int)f.wjobs.size() || f.wjobs.size() != f.prune.size())
+        return bad("walk lists", -1);
+    for (const Gene *gp : f.genes) {
+        const Gene &g = *gp;
+        int q = f.woff[g.gi];
+        for (const auto &rg : g.prune_rng)
+            for (int i = rg[0]; i < rg[1]; i++, q++)
+                if (q >= f.woff[g.gi + 1] || f.wjobs[q] != i) return bad("walk jobs", g.gi);
+        if (q != f.woff[g.gi + 1]) return bad("walk jobs", g.gi);
+    }
+    if (f.witems.size() != (size_t)f.G * (f.npad / kPadP)) return bad("walk items", -1);
+    for (int v : f.witems)
+        if (!in(v, (long)f.G * (f.npad / kPadP))) return bad("walk item", v);
     for (const Gene *gp : f.genes) {
         const Gene &g = *gp;
         for (int c = 0; c < 4; c++)
@@ -878,6 +926,8 @@ struct RoundSel {
     int nej = 0;
     const int *sets = nullptr;
     int nsets = 0;
+    const int *witems = nullptr;        // persistent-pruning items of the round's genes
+    int nwitems = 0;
 };
 
 template <typename T>
@@ -897,7 +947,14 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
     const bool mfma32 = adjoint && sizeof(T) == 4 &&
                         (f.kernel == BSA_KERNEL_MFMA32 || f.kernel == BSA_KERNEL_FP32MC);
     const int pk = f.kernel == BSA_KERNEL_VALU ? 1 : f.kernel == BSA_KERNEL_FP32MC ? 2 : 0;
-    for (size_t l = 0; l + 1 < f.level_off.size(); l++) {
+    if (f.prune_walk) {
+        // every level in one persistent launch (FP64 matrix-core products)
+        launch_prune_walk<T>(s, f.d_prune, f.d_wjobs, f.d_woff, rs ? rs->witems : f.d_witems,
+                             rs ? rs->nwitems : (int)f.witems.size(), f.walk_nblk, f.npad, f.d_z,
+                             (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
+                             (T *)f.d_L, f.d_E, adjoint ? (T *)f.d_C : nullptr);
+    }
+    for (size_t l = 0; !f.prune_walk && l + 1 < f.level_off.size(); l++) {
         const int a = f.level_off[l], b = f.level_off[l + 1];
         launch_prune<T>(s, pk, f.tile,
                         rs ? f.d_prune : f.d_prune + a, rs ? rs->lev[l] : nullptr,
@@ -967,7 +1024,7 @@ static int read_timing(Forest &f, bool adjoint, int nadj_launch)
         f.n_adj_launch += nadj_launch;
     }
     for (int i = 0; i < 6; i++) f.timing_sum[i] += f.timing[i];
-    f.n_launch += (double)(f.level_off.size() - 1);
+    f.n_launch += f.prune_walk ? 1.0 : (double)(f.level_off.size() - 1);
     return BSA_OK;
 }
 
@@ -1046,7 +1103,7 @@ static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
     // job lists: P(t), pruning levels, genes, child adjoints per depth,
     // branch matrices, eigen-basis jobs, eigen sets
     size_t cap = f.pjobs.size() + f.prune.size() + f.G + 2 * f.adj.size() + f.ejobs.size() +
-                 3 * (size_t)f.G;
+                 3 * (size_t)f.G + f.witems.size();
     if (cap > f.sel_cap) {
         if (f.h_sel) HIPCHK(hipHostFree(f.h_sel));
         if (f.d_sel) HIPCHK(hipFree(f.d_sel));
@@ -1078,6 +1135,17 @@ static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
     off.push_back(n);
     for (int g : gl) h[n++] = g;
     rs.ngenes = (int)gl.size();
+    off.push_back(n);
+    if (f.prune_walk) {
+        // the round's genes in the forest's walk order, every tile
+        std::vector<char> in_round(f.G, 0);
+        for (int g : gl) in_round[g] = 1;
+        const int wt = f.npad / kPadP;
+        for (int g : f.walk_order)
+            if (in_round[g])
+                for (int t = 0; t < wt; t++) h[n++] = g * wt + t;
+    }
+    rs.nwitems = (int)(n - off.back());
     if (adjoint) {
         for (size_t d = 0; d < nd; d++) {
             off.push_back(n);
@@ -1109,6 +1177,7 @@ static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
     rs.pj = f.d_sel + off[k++];
     for (size_t l = 0; l < nlev; l++) rs.lev.push_back(f.d_sel + off[k++]);
     rs.genes = f.d_sel + off[k++];
+    rs.witems = f.d_sel + off[k++];
     if (adjoint) {
         for (size_t d = 0; d < nd; d++) rs.adl.push_back(f.d_sel + off[k++]);
         rs.adm = f.d_sel + off[k++];
@@ -1453,6 +1522,24 @@ int forest_set_shapes(Forest &f, int tile, int tpb, int tpc)
     return forest_check(f);
 }
 
+int forest_set_prune(Forest &f, int mode)
+{
+    HIP_SHARED;
+    if (mode != BSA_PRUNE_LEVELS && mode != BSA_PRUNE_WALK)
+        return fail(BSA_EINVAL, "pruning mode %d: BSA_PRUNE_LEVELS or BSA_PRUNE_WALK", mode);
+    if (mode == BSA_PRUNE_WALK && f.kernel != BSA_KERNEL_MFMA && f.kernel != BSA_KERNEL_MFMA32)
+        return fail(BSA_EINVAL, "the persistent pruning needs a kernel with FP64 matrix-core "
+                                "products (BSA_KERNEL_MFMA or BSA_KERNEL_MFMA32)");
+    if (mode != f.prune_walk)
+        for (auto &gr : f.graph)
+            if (gr) {
+                HIPCHK(hipGraphExecDestroy(gr));
+                gr = nullptr;
+            }
+    f.prune_walk = mode;
+    return BSA_OK;
+}
+
 int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *out)
 {
     if (reps < 1) reps = 3;
@@ -1477,6 +1564,18 @@ int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *o
     double best;
     if ((rc = measure(&best))) return rc;
     const double start = best;
+    // pruning by level launches or by the persistent walk (FP64-product
+    // kernels; the walk pays where many genes' tiles fill the GPU)
+    if (f.kernel == BSA_KERNEL_MFMA || f.kernel == BSA_KERNEL_MFMA32) {
+        const int keep = f.prune_walk;
+        if ((rc = forest_set_prune(f, 1 - keep))) return rc;
+        double ms;
+        if ((rc = measure(&ms))) return rc;
+        if (ms < best)
+            best = ms;
+        else if ((rc = forest_set_prune(f, keep)))
+            return rc;
+    }
     const std::vector<int> cand[3] = {
         f.kernel != BSA_KERNEL_VALU ? std::vector<int>{64, 128} : std::vector<int>{64},
         {0, 1, 2, 4, 8},
@@ -1504,6 +1603,7 @@ int forest_tune(Forest &f, const std::function<int()> &eval, int reps, double *o
         out[3] = f.nchunk;
         out[4] = best;
         out[5] = start;
+        out[6] = f.prune_walk;
     }
     return BSA_OK;
 }

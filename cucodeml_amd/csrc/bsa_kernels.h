@@ -137,6 +137,17 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
                   int njobs, int maxint, int npad, const unsigned char *z, const double *PT,
                   const double *TT, int ncodes, T *L, int *E, T *Cb);
 
+// every pruning level of a forest in one persistent launch: nblk blocks take
+// the items (gene * ntile + tile, 128-pattern tiles, ntile = npad / 128) in
+// turn, each walking its gene's jobs wjobs[woff[g] .. woff[g + 1]) (indices
+// into jobs, in level order) for its tile; FP64 matrix-core products, the
+// results of the level launches bitwise
+template <typename T>
+void launch_prune_walk(hipStream_t s, const PruneJob *jobs, const int *wjobs, const int *woff,
+                       const int *items, int nitems, int nblk, int npad,
+                       const unsigned char *z, const double *PT, const double *TT, int ncodes,
+                       T *L, int *E, T *Cb);
+
 // per-pattern class values, mixture, reduced sums (out[kRootSums]); with
 // seed_aslot[0] >= 0 also the adjoint seeds of the root (A, EA).  args:
 // device array of RootArgs per gene; gene g (gsel[i] for block row i, or i)

@@ -460,6 +460,147 @@ __global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_res(const PruneJob 
     }
 }
 
+// Persistent pruning of a forest of genes: one launch for every level.  An
+// item is (gene, 128-pattern tile); block b takes items b, b + gridDim.x,
+// ... of `items` (ordered by the host, genes with the most jobs first) and
+// walks the gene's jobs in level order (wjobs[woff[g] .. woff[g + 1]),
+// indices into jobs) for its tile.  Patterns are independent, so a tile's
+// jobs depend only on earlier jobs of the same tile, which the same block
+// ran: no flag or wait between blocks.  Interior children's P are staged
+// two at a time (the first two of every job as in k_prune_res; a third or
+// more after them), each pair after a __syncthreads() (a workgroup-scope
+// fence with the barrier): every job that reads another job's partials has
+// an interior child, so that barrier orders the loads after the child job's
+// stores, and the staging after the last reads of sP.  Per pattern
+// the arithmetic and the order of the children are those of k_prune_res
+// and k_prune_mfma, so the results are the same bitwise.
+template <typename T>
+__global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_walk(const PruneJob *__restrict__ jobs,
+                                                                 const int *__restrict__ wjobs,
+                                                                 const int *__restrict__ woff,
+                                                                 const int *__restrict__ items,
+                                                                 int nitems, int ntile, int npad,
+                                                                 const unsigned char *__restrict__ z,
+                                                                 const double *__restrict__ PT,
+                                                                 const double *__restrict__ TT,
+                                                                 int ncodes, T *__restrict__ L,
+                                                                 int *__restrict__ E,
+                                                                 T *__restrict__ Cb)
+{
+    __shared__ double sP[2][64 * 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, r = lane & 15;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int item = items[it];
+        const int gene = item / ntile, tile = item - gene * ntile;
+        const int hw = tile * (16 * kResWaves) + 16 * w;
+        const int hb = hw + g;
+        const bool live = hw < npad;   // the same for the whole wave
+        for (int q = woff[gene]; q < woff[gene + 1]; q++) {
+            const PruneJob &jb = jobs[wjobs[q]];
+            const int nch = jb.nch;
+            d4 acc[4];
+            int esum[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                acc[b] = d4{1, 1, 1, 1};
+                esum[b] = 0;
+            }
+            int k = 0;   // interior children seen
+            for (int c = 0; c < nch; c++) {
+                const int pm = jb.ch_pmat[c];
+                const int tip = jb.ch_tip[c];
+                if (tip >= 0) {
+                    if (!live) continue;
+                    const unsigned char *zt = z + (size_t)tip * npad;
+                    int code[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) code[i] = zt[hb + 4 * i];
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            acc[b][i] *= tip_value(TT, pm, ncodes, code[i], 16 * b + r);
+                    continue;
+                }
+                if ((k & 1) == 0) {
+                    // stage this interior child's P and the next one's: the
+                    // previous job's (or pair's) reads of sP and the previous
+                    // job's stores come first
+                    __syncthreads();
+                    int kk2 = 0;
+                    for (int c2 = c; c2 < nch && kk2 < 2; c2++) {
+                        if (jb.ch_tip[c2] >= 0) continue;
+                        const double *P = PT + (size_t)jb.ch_pmat[c2] * 4096;
+                        for (int e = tid; e < 4096; e += 64 * kResWaves)
+                            sP[kk2][e] = P[e] + ((e >> 6) == (e & 63) ? 1.0 : 0.0);
+                        kk2++;
+                    }
+                    __syncthreads();
+                }
+                const double *Pk = sP[k & 1];
+                k++;
+                if (!live) continue;
+                const int slot = jb.ch_slot[c];
+                const T *Lc = L + (size_t)slot * npad * 64;
+                const int *Ec = E + (size_t)slot * npad;
+                double a[16];
+                Store<T>::load16(Lc + (size_t)(hw + r) * 64 + 16 * g, a);
+                d4 tmp[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) tmp[b] = d4{0, 0, 0, 0};
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const double *prow = Pk + (16 * g + kk) * 64 + r;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) tmp[b] = mfma64(a[kk], prow[16 * b], tmp[b]);
+                }
+                if (Cb) {
+                    const int cs = jb.ch_cslot[c];
+                    T *Co = Cb + (size_t)cs * npad * 64;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        uint32_t rb = row_seed((uint32_t)cs ^ 0x3C6EF372u, (uint32_t)(hb + 4 * i),
+                                               (uint32_t)r);
+#pragma unroll
+                        for (int b = 0; b < 4; b++) {
+                            Store<T>::put(Co + (size_t)(hb + 4 * i) * 64 + 16 * b + r,
+                                          tmp[b][i], rb);
+                            rb = xorshift32(rb);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[b] *= tmp[b];
+#pragma unroll
+                for (int i = 0; i < 4; i++) esum[i] += Ec[hb + 4 * i];
+            }
+            if (!live) continue;
+            T *Lo = L + (size_t)jb.out_slot * npad * 64;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                double m = acc[0][i];
+#pragma unroll
+                for (int b = 1; b < 4; b++) m = acc[b][i] > m ? acc[b][i] : m;
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1) {
+                    double o = __shfl_xor(m, off, 64);
+                    m = o > m ? o : m;
+                }
+                int e = 0;
+                if (m > 0) frexp(m, &e);
+                const int h = hb + 4 * i;
+                uint32_t rb = row_seed((uint32_t)jb.out_slot, (uint32_t)h, (uint32_t)r);
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    Store<T>::put(Lo + (size_t)h * 64 + 16 * b + r, ldexp(acc[b][i], -e), rb);
+                    rb = xorshift32(rb);
+                }
+                if (r == 0) E[(size_t)jb.out_slot * npad + h] = esum[i] + e;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ pruning (FP32 MFMA)
 
 __device__ inline f4 mfma32(float a, float b, f4 c)
@@ -1539,6 +1680,19 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
                            npad, z, PT, TT, ncodes, L, E, Cb);
 }
 
+template <typename T>
+void launch_prune_walk(hipStream_t s, const PruneJob *jobs, const int *wjobs, const int *woff,
+                       const int *items, int nitems, int nblk, int npad,
+                       const unsigned char *z, const double *PT, const double *TT, int ncodes,
+                       T *L, int *E, T *Cb)
+{
+    if (nitems <= 0) return;
+    const int ntile = (npad + 16 * kResWaves - 1) / (16 * kResWaves);
+    hipLaunchKernelGGL(k_prune_walk<T>, dim3(std::max(1, std::min(nblk, nitems))),
+                       dim3(64 * kResWaves), 0, s, jobs, wjobs, woff, items, nitems, ntile, npad,
+                       z, PT, TT, ncodes, L, E, Cb);
+}
+
 void launch_adjoint_level32(hipStream_t s, const AdjJob *jobs, const int *sel, int njobs, int npad,
                             const unsigned char *z, const double *PT, const double *TT, int ncodes,
                             const int *E, const float *Cb, float *A, int *EA, int tpb)
@@ -1640,6 +1794,13 @@ template void launch_prune<double>(hipStream_t, int, int, const PruneJob *, cons
 template void launch_prune<float>(hipStream_t, int, int, const PruneJob *, const int *, int, int,
                                   int, const unsigned char *, const double *, const double *, int,
                                   float *, int *, float *);
+template void launch_prune_walk<double>(hipStream_t, const PruneJob *, const int *, const int *,
+                                        const int *, int, int, int, const unsigned char *,
+                                        const double *, const double *, int, double *, int *,
+                                        double *);
+template void launch_prune_walk<float>(hipStream_t, const PruneJob *, const int *, const int *,
+                                       const int *, int, int, int, const unsigned char *,
+                                       const double *, const double *, int, float *, int *, float *);
 template void launch_root<double>(hipStream_t, const RootArgs *, const int *, int, const double *,
                                   const int *, const double *, const double *, const double *,
                                   double *, double *, double *, int, double *, double *, int *);

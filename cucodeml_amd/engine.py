@@ -19,6 +19,8 @@ LIBPATH = os.path.join(PKG, "libbsa.so")
 
 BSA_FP64, BSA_FP32 = 0, 1
 BSA_KERNEL_MFMA, BSA_KERNEL_VALU, BSA_KERNEL_MFMA32, BSA_KERNEL_FP32MC = 0, 1, 2, 3
+BSA_PRUNE_LEVELS, BSA_PRUNE_WALK = 0, 1
+PRUNE_MODES = {"levels": BSA_PRUNE_LEVELS, "walk": BSA_PRUNE_WALK}
 
 EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa_lnl",
             "bsa_lnl_grad", "bsa_lnl_grad_fd", "bsa_class_lnl", "bsa_last_params", "bsa_timing",
@@ -30,7 +32,7 @@ EXPORTED = ["bsa_create", "bsa_destroy", "bsa_set_model", "bsa_num_params", "bsa
             "bsa_batch_eval_genes", "bsa_batch_fit", "bsa_batch_fit_lrt",
             "bsa_batch_class_lnl", "bsa_batch_last_params", "bsa_batch_timing", "bsa_batch_work",
             "bsa_batch_set_graph", "bsa_debug_forest_plan", "bsa_set_shapes", "bsa_tune",
-            "bsa_batch_tune", "bsa_lnl_noalloc"]
+            "bsa_batch_tune", "bsa_lnl_noalloc", "bsa_set_prune", "bsa_batch_set_prune"]
 
 
 class BsaError(RuntimeError):
@@ -102,6 +104,8 @@ def lib():
         L.bsa_default_bounds.argtypes = [C.c_void_p, dp, dp]
         L.bsa_beb.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp, dp, C.c_void_p]
         L.bsa_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.bsa_set_prune.argtypes = [C.c_void_p, C.c_int]
+        L.bsa_batch_set_prune.argtypes = [C.c_void_p, C.c_int]
         L.bsa_set_shapes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.bsa_tune.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, dp]
         L.bsa_fit.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, C.POINTER(_FitOpts), dp,
@@ -177,6 +181,12 @@ def _problem(tree, data, precision, kernel, device, tile, pi, keep):
                     device=device, tile=tile)
 
 
+def _tune_result(o):
+    return {"tile": int(o[0]), "tpb": int(o[1]), "tpc": int(o[2]), "chunks": int(o[3]),
+            "ms": float(o[4]), "ms_before": float(o[5]),
+            "prune": "walk" if int(o[6]) == BSA_PRUNE_WALK else "levels"}
+
+
 def default_kernel(precision):
     """FP32 partials: child adjoints on the FP32 matrix cores (mfma32);
     FP64: every product on the FP64 matrix cores."""
@@ -212,13 +222,17 @@ class BranchSiteEngine:
         """Launch shapes (bsa_set_shapes)."""
         _check(lib().bsa_set_shapes(self._h, tile, tpb, tpc))
 
+    def set_prune(self, mode="levels"):
+        """Pruning by one launch per level ("levels") or by the persistent
+        walk ("walk"), bsa_set_prune."""
+        _check(lib().bsa_set_prune(self._h, PRUNE_MODES[mode]))
+
     def tune(self, x, reps=3):
         """Autotune the launch shapes at x (bsa_tune); returns the choice."""
         x = np.ascontiguousarray(x, dtype=np.float64)
-        o = np.zeros(6)
+        o = np.zeros(7)
         _check(lib().bsa_tune(self._h, _dptr(x), len(x), reps, _dptr(o)))
-        return {"tile": int(o[0]), "tpb": int(o[1]), "tpc": int(o[2]), "chunks": int(o[3]),
-                "ms": float(o[4]), "ms_before": float(o[5])}
+        return _tune_result(o)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -381,6 +395,11 @@ class BranchSiteBatch:
     def use_graph(self, enable=True):
         _check(lib().bsa_batch_set_graph(self._h, int(bool(enable))))
 
+    def set_prune(self, mode="levels"):
+        """Pruning by level launches or by the persistent walk
+        (bsa_batch_set_prune)."""
+        _check(lib().bsa_batch_set_prune(self._h, PRUNE_MODES[mode]))
+
     def close(self):
         if getattr(self, "_h", None):
             lib().bsa_batch_destroy(self._h)
@@ -419,10 +438,9 @@ class BranchSiteBatch:
     def tune(self, xs, reps=3):
         """Autotune the batch's launch shapes at xs (bsa_batch_tune)."""
         xs = self._xs(xs)
-        o = np.zeros(6)
+        o = np.zeros(7)
         _check(lib().bsa_batch_tune(self._h, _ptrs(xs), reps, _dptr(o)))
-        return {"tile": int(o[0]), "tpb": int(o[1]), "tpc": int(o[2]), "chunks": int(o[3]),
-                "ms": float(o[4]), "ms_before": float(o[5])}
+        return _tune_result(o)
 
     def eval_genes(self, genes, xs, grad=True):
         """-lnL (and gradients) of the listed genes only (ascending indices)

@@ -199,13 +199,24 @@ int  bsa_work(bsa_engine *eng, double *out10);
  * the reference. */
 int  bsa_set_shapes(bsa_engine *eng, int tile, int tpb, int tpc);
 
-/* autotuner: times each candidate of each launch shape in turn (pruning
- * tile, child-adjoint tiles per block, branch-matrix tiles per chunk) by
- * the device time of `reps` lnL + gradient evaluations at x, and keeps the
- * fastest.  out6 (may be NULL) = {tile, tpb, tpc, chunks, best ms, ms of the
- * shapes it started from}.  The engine's last evaluation is at x.  No
- * counterpart in the reference. */
-int  bsa_tune(bsa_engine *eng, const double *x, int np, int reps, double *out6);
+/* pruning schedule: one launch per tree level (BSA_PRUNE_LEVELS, the
+ * default), or every level in one persistent launch (BSA_PRUNE_WALK: blocks
+ * take (gene, 128-pattern tile) items in turn and walk the gene's pruning
+ * jobs in level order for their tile; results bitwise those of the level
+ * launches; BSA_KERNEL_MFMA or BSA_KERNEL_MFMA32 only).  Replaces the
+ * reference's launch and host synchronisation per edge
+ * (cuda-codeml.cu:1306-1321). */
+#define BSA_PRUNE_LEVELS 0
+#define BSA_PRUNE_WALK   1
+int  bsa_set_prune(bsa_engine *eng, int mode);
+
+/* autotuner: times each candidate of each launch parameter in turn
+ * (pruning mode, pruning tile, child-adjoint tiles per block, branch-matrix
+ * tiles per chunk) by the device time of `reps` lnL + gradient evaluations
+ * at x, and keeps the fastest.  out7 (may be NULL) = {tile, tpb, tpc,
+ * chunks, best ms, ms of the shapes it started from, pruning mode}.  The
+ * engine's last evaluation is at x.  No counterpart in the reference. */
+int  bsa_tune(bsa_engine *eng, const double *x, int np, int reps, double *out7);
 
 /* codeml's parameter bounds for model A in the engine's x layout
  * (SetxBound codeml.c:1786-1828, SetxBoundTimes treesub.c:3872-3884):
@@ -329,14 +340,16 @@ int  bsa_batch_fit(bsa_batch *batch, double *const *x, const double *const *lo,
 int  bsa_batch_fit_lrt(bsa_batch *batch, double *const *x1, double *const *x0,
                        const bsa_fit_opts *opts, double *neg_lnl, int *info);
 /* bsa_tune for a batch, at x[gene] for every gene */
-int  bsa_batch_tune(bsa_batch *batch, const double *const *x, int reps, double *out6);
+int  bsa_batch_tune(bsa_batch *batch, const double *const *x, int reps, double *out7);
 /* per-gene outputs of the last evaluation, as bsa_class_lnl / bsa_last_params */
 int  bsa_batch_class_lnl(bsa_batch *batch, int gene, double *fhK, double *site_lnf);
 int  bsa_batch_last_params(const bsa_batch *batch, int gene, double *out11);
-/* as bsa_timing / bsa_work / bsa_set_graph, for the whole batch */
+/* as bsa_timing / bsa_work / bsa_set_graph / bsa_set_prune, for the whole
+ * batch */
 int  bsa_batch_timing(bsa_batch *batch, double *out17, int reset);
 int  bsa_batch_work(bsa_batch *batch, double *out10);
 int  bsa_batch_set_graph(bsa_batch *batch, int enable);
+int  bsa_batch_set_prune(bsa_batch *batch, int mode);
 /* diagnostic, host only: plan a batch of the given genes without a device
  * and check every index of its schedules against the buffers they index;
  * out8 = {genes, padded patterns, partial slots, pruning launches, pruning
