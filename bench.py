@@ -283,6 +283,44 @@ def genome_scan(ngenes, workers, precision, batch_genes, concurrent):
             "mean_evaluations": float(np.mean([r["evaluations"] for r in res]))}
 
 
+def batch_pruning(ngenes, taxa=24, codons=300, reps=3):
+    """Whole-batch lnL + gradient evaluations of `ngenes` synthetic genes
+    (seeded Yule trees of `taxa` taxa, `codons` codons, FP64 partials) with
+    the pruning by level launches and by the persistent walk: device ms per
+    evaluation (events), and whether the two give the same results bitwise."""
+    rng = np.random.default_rng(5)
+    genes, xs = [], []
+    for _ in range(ngenes):
+        tree = simulate.yule_tree(taxa, 0.05, rng)
+        _, pi = simulate.dirichlet_f3x4(rng)
+        seqs, _ = simulate.simulate_model_a(tree, codons, 2.0, 0.2, 3.0, 0.5, 0.3, pi, rng)
+        genes.append((tree, formats.load_codon_data(tree.names, seqs)))
+        xs.append(engine.x_from_params(tree.branch_vector(), 2.0, 0.5, 0.3, 0.2, 3.0))
+    out = {"workload": f"{ngenes} genes x {taxa} taxa x {codons} codons, fp64 partials, "
+                       f"lnL + adjoint gradient of every gene per evaluation"}
+    res = {}
+    phases = ("pmat", "prune", "root", "adjoint", "branch", "finish")
+    for mode in ("levels", "walk"):
+        b = engine.BranchSiteBatch(genes, precision="fp64")
+        try:
+            b.set_prune(mode)
+            b.lnl_grad(xs)
+            b.timing(reset=True)
+            for _ in range(reps):
+                f, g = b.lnl_grad(xs)
+            tm = b.timing()
+        finally:
+            b.close()
+        res[mode] = (f, g)
+        out[mode] = {"prune_ms": tm["sum_prune"] / reps,
+                     "device_ms": sum(tm["sum_" + k] for k in phases) / reps,
+                     "pruning_launches": tm["n_launch"] / reps}
+    out["bitwise_equal"] = bool(np.array_equal(res["levels"][0], res["walk"][0]) and
+                                all(np.array_equal(a, c) for a, c in
+                                    zip(res["levels"][1], res["walk"][1])))
+    return out
+
+
 def scan_device_record(stats, wall):
     """The batch engines' own event timers summed over a scan's batches:
     each batch engine serves its genes' evaluation requests in rounds on its
@@ -420,6 +458,9 @@ def main():
                     help="batches fitted at once in the scan")
     ap.add_argument("--scan-workers", type=int, default=8,
                     help="engines at once when the scan runs one engine per gene")
+    ap.add_argument("--batch-pruning", type=int, default=0,
+                    help="also time whole-batch evaluations of this many small genes with "
+                         "level launches and with the persistent pruning (0: skip)")
     ap.add_argument("--no-beb", dest="beb", action="store_false",
                     help="skip timing one Bayes empirical Bayes analysis beside")
     args = ap.parse_args()
@@ -543,6 +584,8 @@ def main():
     if args.scan_genes > 0:
         out["genome_scan"] = genome_scan(args.scan_genes, args.scan_workers, args.scan_precision,
                                          args.scan_batch, args.scan_concurrent)
+    if args.batch_pruning > 0:
+        out["batch_pruning"] = batch_pruning(args.batch_pruning)
     if args.large_fit:
         out["large_fit"] = large_fit(e=args.large_fit_e)
     if args.scan_cpu > 0:

@@ -336,14 +336,17 @@ __global__ __launch_bounds__(256, RG == 1 ? 4 : 2) void k_prune_mfma(const Prune
     }
 }
 
-// The same job with every interior child's P resident in LDS, for jobs with
-// at most two interior children: a block of 8 waves takes one (node, class)
-// job and tpb tiles of 128 patterns.  The P tiles are staged once; then each
-// wave runs its 16-pattern rows of every tile with no further barrier, so
-// the waves overlap one another's loads and products freely.  At 128 VGPRs
-// and 64 KB of LDS two such blocks share a CU, so one block's staging
-// overlaps the other's products.  Per pattern the arithmetic is that of
-// k_prune_mfma, so results are the same bitwise.
+// The same job with the first two interior children's P resident in LDS: a
+// block of 8 waves takes one (node, class) job and tpb tiles of 128
+// patterns.  The P tiles are staged once; then each wave runs its 16-pattern
+// rows of every tile with no further barrier, so the waves overlap one
+// another's loads and products freely.  A job with more interior children
+// (the unrooted root's three, multifurcations; launched with tpb = 1)
+// stages the later ones in pairs over the first, between barriers.  At 128
+// VGPRs and 64 KB of LDS two such blocks share a CU, so one block's staging
+// overlaps the other's products.  Per pattern the arithmetic and the order
+// of the children are those of k_prune_mfma, so results are the same
+// bitwise.
 constexpr int kResWaves = 8;
 
 template <typename T>
@@ -399,6 +402,21 @@ __global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_res(const PruneJob 
                     for (int i = 0; i < 4; i++)
                         acc[b][i] *= tip_value(TT, pm, ncodes, code[i], 16 * b + r);
             } else {
+                if (k >= 2 && (k & 1) == 0) {
+                    // a third or later interior child (the unrooted root,
+                    // multifurcations; tpb is 1 for such jobs): its P and the
+                    // next one's are staged over the first pair
+                    __syncthreads();
+                    int k2 = 0;
+                    for (int c2 = c; c2 < nch && k2 < 2; c2++) {
+                        if (jb.ch_tip[c2] >= 0) continue;
+                        const double *P = PT + (size_t)jb.ch_pmat[c2] * 4096;
+                        for (int e = tid; e < 4096; e += 64 * kResWaves)
+                            sP[k2][e] = P[e] + ((e >> 6) == (e & 63) ? 1.0 : 0.0);
+                        k2++;
+                    }
+                    __syncthreads();
+                }
                 const int slot = jb.ch_slot[c];
                 const T *Lc = L + (size_t)slot * npad * 64;
                 const int *Ec = E + (size_t)slot * npad;
@@ -407,7 +425,7 @@ __global__ __launch_bounds__(64 * kResWaves, 4) void k_prune_res(const PruneJob 
                 d4 tmp[4];
 #pragma unroll
                 for (int b = 0; b < 4; b++) tmp[b] = d4{0, 0, 0, 0};
-                const double *Pk = sP[k];
+                const double *Pk = sP[k & 1];
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
                     const double *prow = Pk + (16 * g + kk) * 64 + r;
@@ -1650,9 +1668,10 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
                   const double *TT, int ncodes, T *L, int *E, T *Cb)
 {
     if (njobs <= 0) return;
-    // P resident in LDS where every job has at most two interior children
-    // and the launch has at least half as many blocks of 8 waves as CUs
-    // (below that the 4-wave blocks spread the work better)
+    // P resident in LDS where the launch has at least half as many blocks of
+    // 8 waves as CUs (below that the 4-wave blocks spread the work better);
+    // one tile per block, so that jobs with more than two interior children
+    // can stage their later children's P over the first pair
     const int nres = (npad + 16 * kResWaves - 1) / (16 * kResWaves);
     if constexpr (sizeof(T) == 4) {
         // FP32 matrix-core products (jobs with more than two interior
@@ -1664,7 +1683,7 @@ void launch_prune(hipStream_t s, int kernel, int tile, const PruneJob *jobs, con
         }
     }
     if (kernel == 2) kernel = 0;
-    if (kernel == 0 && tile == 64 && maxint <= 2 && (long)nres * njobs >= 128) {
+    if (kernel == 0 && tile == 64 && (long)nres * njobs >= 128) {
         hipLaunchKernelGGL(k_prune_res<T>, dim3(nres, njobs), dim3(64 * kResWaves), 0, s, jobs,
                            sel, npad, z, PT, TT, ncodes, L, E, Cb, 1);
         return;
