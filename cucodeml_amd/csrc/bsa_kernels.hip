@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
 {
     __shared__ double Ut[64 * 64];   // Ut[m][j] = U[j][m]
     __shared__ double Ve[64 * 64];   // Ve[m][k] = exp(root_m t) V[m][k]
+    __shared__ double eR[64];        // exp(root_m t)
     const PmatJob jb = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
     const double *U = uvr + uvr_offset(jb.set);
     const double *V = U + 4096;
@@ -71,10 +72,12 @@ __global__ __launch_bounds__(256) void k_pmat(const double *__restrict__ uvr,
         for (int q = tid; q < 4096; q += 256) out[q] = 0.0;
         return;
     }
+    if (tid < 64) eR[tid] = exp(jb.t * R[tid]);   // 64 exponentials, not one per element
+    __syncthreads();
     for (int q = tid; q < 4096; q += 256) {
         int r = q >> 6, c = q & 63;
         Ut[c * 64 + r] = U[q];
-        Ve[q] = V[q] * exp(jb.t * R[r]);
+        Ve[q] = V[q] * eR[r];
     }
     __syncthreads();
     const int j = tid & 63, kq = tid >> 6;
