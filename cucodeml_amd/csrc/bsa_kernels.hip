@@ -645,39 +645,33 @@ __device__ inline bool top_before(float va, int ca, float vb, int cb)
 }
 __device__ inline Top2 row_top2(const float *a, int g)
 {
+    // branch-free (selects), so that the 16 updates and the 2 merges stay
+    // straight-line code
     Top2 t{-1.f, -1.f, 64, 64};
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
         const float v = a[kk];
         const int c = 16 * g + kk;
-        if (top_before(v, c, t.v1, t.c1)) {
-            t.v2 = t.v1;
-            t.c2 = t.c1;
-            t.v1 = v;
-            t.c1 = c;
-        } else if (top_before(v, c, t.v2, t.c2)) {
-            t.v2 = v;
-            t.c2 = c;
-        }
+        const bool b1 = top_before(v, c, t.v1, t.c1), b2 = top_before(v, c, t.v2, t.c2);
+        t.v2 = b1 ? t.v1 : (b2 ? v : t.v2);
+        t.c2 = b1 ? t.c1 : (b2 ? c : t.c2);
+        t.v1 = b1 ? v : t.v1;
+        t.c1 = b1 ? c : t.c1;
     }
 #pragma unroll
     for (int off = 16; off < 64; off <<= 1) {
         const float pv1 = __shfl_xor(t.v1, off, 64), pv2 = __shfl_xor(t.v2, off, 64);
         const int pc1 = __shfl_xor(t.c1, off, 64), pc2 = __shfl_xor(t.c2, off, 64);
-        if (top_before(pv1, pc1, t.v1, t.c1)) {
-            if (!top_before(t.v1, t.c1, pv2, pc2)) {
-                t.v2 = pv2;
-                t.c2 = pc2;
-            } else {
-                t.v2 = t.v1;
-                t.c2 = t.c1;
-            }
-            t.v1 = pv1;
-            t.c1 = pc1;
-        } else if (top_before(pv1, pc1, t.v2, t.c2)) {
-            t.v2 = pv1;
-            t.c2 = pc1;
-        }
+        // merged pair: the first of the two firsts, then the better of the
+        // other first and the winner's second
+        const bool pw = top_before(pv1, pc1, t.v1, t.c1);
+        const float ov = pw ? t.v1 : pv1, sv = pw ? pv2 : t.v2;
+        const int oc = pw ? t.c1 : pc1, sc = pw ? pc2 : t.c2;
+        const bool ob = top_before(ov, oc, sv, sc);
+        t.v1 = pw ? pv1 : t.v1;
+        t.c1 = pw ? pc1 : t.c1;
+        t.v2 = ob ? ov : sv;
+        t.c2 = ob ? oc : sc;
     }
     return t;
 }
@@ -698,8 +692,10 @@ __device__ inline Top2 row_top2(const float *a, int g)
 // terms of comparable size.  L(h, j) comes from a transpose tile of the
 // wave's rows in LDS; accumulator element i of lane (g, r) is pattern
 // 4g + i of the wave's 16, parent state 16b + r.  Children, scaling and
-// stores as in k_prune_res.
-__global__ __launch_bounds__(64 * kResWaves, 4) void k_prune32(const PruneJob *__restrict__ jobs,
+// stores as in k_prune_res.  At 156 VGPRs (three waves per SIMD, one block
+// per CU) it spills nothing; held to 128 for two blocks per CU it spilled
+// 24 VGPRs and ran 6 % slower.
+__global__ __launch_bounds__(64 * kResWaves, 2) void k_prune32(const PruneJob *__restrict__ jobs,
                                                               const int *__restrict__ sel, int npad,
                                                               const unsigned char *__restrict__ z,
                                                               const double *__restrict__ PT,
