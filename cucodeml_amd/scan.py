@@ -190,7 +190,8 @@ def batch_timing(b, ngenes):
             "phase_ms": {k: tm["sum_" + k] for k in phases}}
 
 
-def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False, stats=None):
+def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False, stats=None,
+              prune="levels"):
     """H1 and H0 fits and the LRT of a list of genes on one batch engine
     (bsa_batch_fit_lrt): the sequence of ``fit_gene`` (H1, H0 from the H1
     estimates, H1 again from the H0 estimates where H0 lies above H1), each
@@ -200,13 +201,15 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False, stats=None
     statistic are then the FP64 ones, as in ``fit_gene``.  Returns the
     per-gene dicts of ``fit_gene`` (seconds = the batch's wall time).  With
     a list `stats`, the batch's ``batch_timing`` record of its chained fits
-    is appended to it."""
+    is appended to it.  `prune`: the batch engine's pruning mode ("levels"
+    or "walk", bsa_batch_set_prune)."""
     t0 = time.perf_counter()
     n = len(genes)
     b = engine.BranchSiteBatch([(g.tree, g.data) for g in genes], precision=precision,
                                device=device)
     b.use_graph(graph)
     try:
+        b.set_prune(prune)
         starts = []
         for g in genes:
             s = codeml.default_start(g.tree, 2.0, 1.5, False)
@@ -259,10 +262,11 @@ def fit_batch(genes, precision="fp64", device=0, e=1e-8, graph=False, stats=None
 
 
 def scan_batched(genes, batch_genes=64, concurrent=6, precision="fp64", device=0, e=1e-8,
-                 graph=False, max_waste=0.25, stats=None, progress=None):
+                 graph=False, max_waste=0.25, stats=None, progress=None, prune="levels"):
     """Fit every gene (H1, H0, LRT) in batches of up to `batch_genes` genes
-    of similar size, `concurrent` batches at a time.  Returns (results in
-    input order, wall seconds); `stats` (a list) receives each batch's
+    of similar size, `concurrent` batches at a time, each batch engine
+    pruning by `prune` ("levels" or "walk").  Returns (results in input
+    order, wall seconds); `stats` (a list) receives each batch's
     ``batch_timing`` record, and progress(batches done, batches, genes of
     the batch) is called as each batch ends."""
     groups = batch_groups(genes, batch_genes, max_waste)
@@ -270,7 +274,7 @@ def scan_batched(genes, batch_genes=64, concurrent=6, precision="fp64", device=0
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=max(1, concurrent)) as ex:
         futs = {ex.submit(fit_batch, [genes[i] for i in grp], precision, device, e, graph,
-                          stats): grp for grp in groups}
+                          stats, prune): grp for grp in groups}
         for k, f in enumerate(as_completed(futs), 1):
             grp = futs[f]
             for i, r in zip(grp, f.result()):
