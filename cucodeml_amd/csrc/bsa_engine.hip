@@ -20,9 +20,12 @@ using namespace bsa;
 struct bsa_engine {
     Gene g;
     Forest f;
+    std::vector<double> part_x;   // x of the last part-1 evaluation
+    bool part_ready = false;      // its forward state is on the device
 };
 
 int bsa_engine_precision(const bsa_engine *eng) { return eng->f.precision; }
+bool bsa_engine_graph(const bsa_engine *eng) { return eng->f.use_graph; }
 
 extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
 {
@@ -76,6 +79,7 @@ static int evaluate_natural(bsa_engine *e, const double *x, double kappa, double
                             double w0, double w2, const double *qf_fixed, double *lnl,
                             bool adjoint)
 {
+    e->part_ready = false;   // the device buffers no longer hold a part-1 state
     int rc = gene_stage(e->g, e->f, x, kappa, p0, p1, w0, w2, qf_fixed, adjoint);
     if (rc) return rc;
     if ((rc = forest_run(e->f, adjoint))) return rc;
@@ -111,6 +115,50 @@ extern "C" int bsa_lnl(bsa_engine *e, const double *x, int np, double *neg_lnl)
     int rc = evaluate(e, x, &lnl);
     if (rc) return rc;
     *neg_lnl = -lnl;
+    return BSA_OK;
+}
+
+// An evaluation with gradient in two calls, for an optimiser that needs the
+// gradient only at the points it accepts: part 1 returns -lnL at x and keeps
+// the forward state the adjoint sweep needs; part 2 returns d(-lnL)/dx at
+// that x.  Together they give bsa_lnl_grad's values bitwise; a rejected
+// trial point costs the forward pass only.
+int bsa_engine_lnl_part1(bsa_engine *e, const double *x, int np, double *neg_lnl)
+{
+    if (!e || !x || !neg_lnl) return fail(BSA_EINVAL, "null argument");
+    if (np != bsa_num_params(e)) return fail(BSA_EINVAL, "np=%d, engine expects %d", np, bsa_num_params(e));
+    HIP_SHARED;
+    HIPCHK(hipSetDevice(e->f.dev));
+    e->part_ready = false;
+    int rc = forest_setup_adjoint(e->f);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    double kappa, p0, p1, w0, w2;
+    natural_params(e->g, x, &kappa, &p0, &p1, &w0, &w2);
+    if ((rc = gene_stage(e->g, e->f, x, kappa, p0, p1, w0, w2, nullptr, true))) return rc;
+    if ((rc = forest_run_part(e->f, 1))) return rc;
+    const double lnl = gene_collect(e->g, e->f);
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e->f.timing[6] = ms;
+    e->f.timing_sum[6] += ms;
+    e->f.n_eval += 1;
+    e->part_x.assign(x, x + np);
+    e->part_ready = true;
+    *neg_lnl = -lnl;
+    return BSA_OK;
+}
+
+int bsa_engine_grad_part2(bsa_engine *e, int np, double *grad)
+{
+    if (!e || !grad) return fail(BSA_EINVAL, "null argument");
+    if (!e->part_ready || (int)e->part_x.size() != np)
+        return fail(BSA_EINVAL, "no part-1 evaluation to differentiate");
+    HIP_SHARED;
+    HIPCHK(hipSetDevice(e->f.dev));
+    const int rc = forest_run_part(e->f, 2);
+    if (rc) return rc;
+    gene_chain_rule(e->g, e->f, e->part_x.data(), grad);
     return BSA_OK;
 }
 
@@ -434,12 +482,14 @@ extern "C" int bsa_set_graph(bsa_engine *e, int enable)
 extern "C" int bsa_set_shapes(bsa_engine *e, int tile, int tpb, int tpc)
 {
     if (!e) return fail(BSA_EINVAL, "null engine");
+    e->part_ready = false;
     return forest_set_shapes(e->f, tile, tpb, tpc);
 }
 
 extern "C" int bsa_set_prune(bsa_engine *e, int mode)
 {
     if (!e) return fail(BSA_EINVAL, "null engine");
+    e->part_ready = false;
     return forest_set_prune(e->f, mode);
 }
 

@@ -58,6 +58,11 @@
 
 int bsa_error(int code, const char *msg);   // sets bsa_last_error (engine)
 int bsa_engine_precision(const bsa_engine *eng);   // BSA_FP64 / BSA_FP32 (engine)
+bool bsa_engine_graph(const bsa_engine *eng);      // evaluations replayed from graphs
+// an evaluation with gradient in two calls (engine): -lnL at x, then the
+// gradient at that x
+int bsa_engine_lnl_part1(bsa_engine *eng, const double *x, int np, double *neg_lnl);
+int bsa_engine_grad_part2(bsa_engine *eng, int np, double *grad);
 
 namespace {
 inline double clampd(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
@@ -67,9 +72,10 @@ namespace bsa {
 
 // Newton iterations from x (f, g its value and gradient, updated in place);
 // *converged = 1 when the last Newton step was below 1e-6 max(1, |x_i|)
-template <class FG>
-static int newton_finish(FG &fg, int n, double *x, const double *lo, const double *hi,
-                         int iters, double *f, std::vector<double> &g, int *converged)
+template <class FG, class FV, class GR>
+static int newton_finish(FG &fg, FV &fv, GR &gr, int n, double *x, const double *lo,
+                         const double *hi, int iters, double *f, std::vector<double> &g,
+                         int *converged)
 {
     std::vector<double> xt(n), gt(n), gc(n);
     *converged = 0;
@@ -158,9 +164,10 @@ static int newton_finish(FG &fg, int n, double *x, const double *lo, const doubl
                 xt[i] = clampd(x[i] + step * dd[a], lo[i], hi[i]);
                 slope += g[i] * (xt[i] - x[i]);
             }
-            int rc = fg(xt.data(), &ft, gt.data());
+            int rc = fv(xt.data(), &ft);
             if (rc && rc != BSA_ENUMERIC) return rc;
             if (rc == BSA_OK && std::isfinite(ft) && ft <= *f + 1e-4 * slope) {
+                if ((rc = gr(gt.data()))) return rc;   // the accepted point's gradient
                 acc = true;
                 break;
             }
@@ -186,7 +193,7 @@ static int newton_finish(FG &fg, int n, double *x, const double *lo, const doubl
 // Newton iterations close the gap and the later ones show the step vanish
 constexpr int kVerifyNewton = 4;
 
-int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const double *hi,
+int minimize_box(const Objective &obj, int n, double *x, const double *lo, const double *hi,
                  const bsa_fit_opts *opts, FitResult *res, bool verify)
 {
     const int max_iter = !opts || opts->max_iter == 0 ? 2000 : std::max(0, opts->max_iter);
@@ -194,9 +201,14 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     const double e = opts && opts->e > 0 ? opts->e : 1e-8;
     const double max_step = opts && opts->max_step > 0 ? opts->max_step : 5.0;
     int nfev = 0;
-    auto fg = [&](const double *xx, double *ff, double *gg) {
+    auto fv = [&](const double *xx, double *ff) {
         nfev++;
-        return fgfun(xx, ff, gg);
+        return obj.value(xx, ff);
+    };
+    auto gr = [&](double *gg) { return obj.grad(gg); };
+    auto fg = [&](const double *xx, double *ff, double *gg) {
+        const int rc = fv(xx, ff);
+        return rc ? rc : gr(gg);
     };
     std::vector<double> g(n), xt(n), gt(n), d(n), s(n), y(n), Hy(n), H((size_t)n * n, 0.0);
     std::vector<char> act(n);
@@ -268,9 +280,10 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
                 dx = std::max(dx, std::fabs(xt[i] - x[i]));
             }
             if (dx == 0) break;
-            rc = fg(xt.data(), &ft, gt.data());
+            rc = fv(xt.data(), &ft);
             if (rc && rc != BSA_ENUMERIC) return rc;
             if (rc == BSA_OK && std::isfinite(ft) && ft <= f + 1e-4 * slope) {
+                if ((rc = gr(gt.data()))) return rc;   // the accepted point's gradient
                 ok = true;
                 break;
             }
@@ -356,12 +369,12 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
     }
     if (newton > 0) {
         int st = 0;
-        if ((rc = newton_finish(fg, n, x, lo, hi, newton, &f, g, &st))) return rc;
+        if ((rc = newton_finish(fg, fv, gr, n, x, lo, hi, newton, &f, g, &st))) return rc;
         if (st) stop = BSA_STOP_NEWTON;
         it += newton;
     } else if (verify && (stop == BSA_STOP_STALL || stop == BSA_STOP_NODESCENT)) {
         int st = 0;
-        if ((rc = newton_finish(fg, n, x, lo, hi, kVerifyNewton, &f, g, &st))) return rc;
+        if ((rc = newton_finish(fg, fv, gr, n, x, lo, hi, kVerifyNewton, &f, g, &st))) return rc;
         if (st) stop = BSA_STOP_NEWTON;
     }
     res->f = f;
@@ -375,6 +388,7 @@ int minimize_box(const ObjFn &fgfun, int n, double *x, const double *lo, const d
 
 using bsa::FitResult;
 using bsa::ObjFn;
+using bsa::Objective;
 using bsa::minimize_box;
 
 extern "C" int bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, const double *hi,
@@ -385,11 +399,33 @@ extern "C" int bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, con
     for (int i = 0; i < np; i++)
         if (!(lo[i] <= hi[i])) return bsa_error(BSA_EINVAL, "bounds with lo > hi");
     const bool fd = opts && opts->gradient == BSA_GRAD_FD;
-    ObjFn fg = [&](const double *xx, double *f, double *g) {
-        return fd ? bsa_lnl_grad_fd(eng, xx, np, 0, 1e-6, f, g) : bsa_lnl_grad(eng, xx, np, f, g);
-    };
+    // the gradient only where the optimiser accepts a point: a rejected
+    // trial costs one likelihood (the forward pass of the adjoint's
+    // evaluation, or the one likelihood of the differences), not a gradient
+    Objective obj;
+    std::vector<double> xl(np);
+    if (fd) {
+        obj.value = [&](const double *xx, double *f) {
+            std::copy(xx, xx + np, xl.begin());
+            return bsa_lnl(eng, xx, np, f);
+        };
+        obj.grad = [&](double *g) {
+            double f;
+            return bsa_lnl_grad_fd(eng, xl.data(), np, 0, 1e-6, &f, g);
+        };
+    } else if (bsa_engine_graph(eng)) {
+        // graph replay evaluates lnL and gradient together
+        obj = bsa::combined(
+            [&](const double *xx, double *f, double *g) { return bsa_lnl_grad(eng, xx, np, f, g); },
+            np);
+    } else {
+        obj.value = [&](const double *xx, double *f) {
+            return bsa_engine_lnl_part1(eng, xx, np, f);
+        };
+        obj.grad = [&](double *g) { return bsa_engine_grad_part2(eng, np, g); };
+    }
     FitResult r;
-    int rc = minimize_box(fg, np, x, lo, hi, opts, &r,
+    int rc = minimize_box(obj, np, x, lo, hi, opts, &r,
                           !fd && bsa_engine_precision(eng) == BSA_FP64);
     if (rc) return rc;
     // leave the engine's state (per-site values, parameters) at the returned x

@@ -1,6 +1,9 @@
 // Internal header: the bounded quasi-Newton minimiser behind bsa_fit.
 #pragma once
+#include <algorithm>
 #include <functional>
+#include <memory>
+#include <vector>
 
 #include "bsa.h"
 
@@ -8,6 +11,26 @@ namespace bsa {
 
 // objective: f and its gradient g at x; returns a bsa status code
 using ObjFn = std::function<int(const double *x, double *f, double *g)>;
+
+// objective in two calls, for an optimiser that needs the gradient only at
+// the points it accepts: value(x, &f) makes x the current point, grad(g)
+// writes the gradient at the current point; both return bsa status codes
+struct Objective {
+    std::function<int(const double *x, double *f)> value;
+    std::function<int(double *g)> grad;
+};
+
+// the two calls from one combined function: value computes both and keeps
+// the gradient for grad
+inline Objective combined(const ObjFn &fg, int n)
+{
+    auto buf = std::make_shared<std::vector<double>>(n);
+    return {[fg, buf](const double *x, double *f) { return fg(x, f, buf->data()); },
+            [buf](double *g) {
+                std::copy(buf->begin(), buf->end(), g);
+                return BSA_OK;
+            }};
+}
 
 struct FitResult {
     double f = 0;
@@ -21,7 +44,15 @@ struct FitResult {
 // iterations check whether x is the maximum (BSA_STOP_NEWTON if so); for
 // exact FP64 gradients (the adjoint on FP64 partials), whose differences give
 // the Hessian
-int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
+// The gradient is asked for at the start point, at every accepted point
+// and for the Newton finish's Hessian columns, never at a rejected trial
+// point; res->evaluations counts the value calls.
+int minimize_box(const Objective &obj, int n, double *x, const double *lo, const double *hi,
                  const bsa_fit_opts *opts, FitResult *res, bool verify = false);
+inline int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
+                        const bsa_fit_opts *opts, FitResult *res, bool verify = false)
+{
+    return minimize_box(combined(fg, n), n, x, lo, hi, opts, res, verify);
+}
 
 }  // namespace bsa

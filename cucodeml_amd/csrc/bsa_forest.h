@@ -195,6 +195,12 @@ int gene_stage_x(Gene &g, Forest &f, const double *x, bool adjoint);
 // device work of one evaluation of every gene (inputs staged), results
 // copied back; then gene_collect reads gene g's lnL
 int forest_run(Forest &f, bool adjoint);
+// an evaluation with gradient in two parts (inputs staged with adjoint):
+// part 1 the forward pass, keeping what the adjoint sweep needs (gene_collect
+// then reads lnL); part 2 the sweep of the last part 1 (then
+// gene_chain_rule).  Bitwise the results of forest_run(f, true); eager
+// launches only.  A fit asks for part 2 only at the points it accepts.
+int forest_run_part(Forest &f, int part);
 // the same for the listed genes only (ascending gene indices, inputs
 // staged): their jobs alone are launched and only their inputs and outputs
 // cross PCIe; other genes' staging buffers may be written meanwhile.  Eager

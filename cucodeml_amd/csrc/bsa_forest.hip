@@ -930,9 +930,11 @@ struct RoundSel {
     int nwitems = 0;
 };
 
+// P(t), pruning and root pass; with `adjoint`, the contributions and the
+// root's adjoint seeds that the sweep will need are stored too
 template <typename T>
-static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
-                          const RoundSel *rs = nullptr)
+static int enqueue_forward(Forest &f, hipStream_t s, bool adjoint, bool events,
+                           const RoundSel *rs = nullptr)
 {
 #define REC(i) \
     if (events) HIPCHK(hipEventRecord(f.ev[i], s))
@@ -941,11 +943,7 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
                 (double *)f.d_PT, f.d_mask_amb, f.ncodes, (double *)f.d_AT);
     HIPCHK(hipGetLastError());
     REC(1);
-    // BSA_KERNEL_MFMA32 moves the child adjoints and the interior children's
-    // branch matrices to the FP32 matrix cores, BSA_KERNEL_FP32MC the
-    // pruning's products too
-    const bool mfma32 = adjoint && sizeof(T) == 4 &&
-                        (f.kernel == BSA_KERNEL_MFMA32 || f.kernel == BSA_KERNEL_FP32MC);
+    // BSA_KERNEL_FP32MC: the pruning's products on the FP32 matrix cores
     const int pk = f.kernel == BSA_KERNEL_VALU ? 1 : f.kernel == BSA_KERNEL_FP32MC ? 2 : 0;
     if (f.prune_walk) {
         // every level in one persistent launch (FP64 matrix-core products)
@@ -969,38 +967,61 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
                    f.d_fh, f.d_site, f.d_partial, f.nroot_blk, f.d_out, (T *)f.d_A, f.d_EA);
     HIPCHK(hipGetLastError());
     REC(3);
-    if (adjoint) {
-        for (size_t d = 0; d + 1 < f.adj_off.size(); d++) {
-            if (mfma32)
-                launch_adjoint_level32(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
-                                       rs ? rs->adl[d] : nullptr,
-                                       rs ? rs->nadl[d] : f.adj_nint[d], f.npad, f.d_z,
-                                       (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
-                                       f.d_E, (const float *)f.d_C, (float *)f.d_A, f.d_EA,
-                                       f.tpb_set);
-            else
-                launch_adjoint_level<T>(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
-                                        rs ? rs->adl[d] : nullptr,
-                                        rs ? rs->nadl[d] : f.adj_nint[d], f.npad, f.d_z,
-                                        (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
-                                        f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA, f.tpb_set);
-        }
-        REC(6);
-        launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
-                            f.npad, f.nchunk, f.tpc, mfma32, f.d_z, (const double *)f.d_PT,
-                            (const double *)f.d_AT, f.ncodes, f.d_mask64, f.d_tperm,
-                            (const T *)f.d_L, f.d_E, (const T *)f.d_C, (const T *)f.d_A, f.d_EA,
-                            f.d_Mpart);
-        REC(4);
-        launch_adjoint_finish(s, f.d_Mpart, f.nchunk, f.d_ejobs, rs ? rs->ej : nullptr,
-                              rs ? rs->nej : (int)f.ejobs.size(), rs ? rs->sets : nullptr,
-                              rs ? rs->nsets : 3 * f.G, f.d_set_off, f.d_set_list, f.d_uvr, f.d_G,
-                              f.d_dtau, f.d_W, f.d_Z);
-        HIPCHK(hipGetLastError());
-        REC(5);
-    }
 #undef REC
     return BSA_OK;
+}
+
+// The adjoint sweep (child adjoints per depth, branch matrices, eigen-basis
+// finish) of the last forward pass run with `adjoint`
+template <typename T>
+static int enqueue_adjoint(Forest &f, hipStream_t s, bool events,
+                           const RoundSel *rs = nullptr)
+{
+#define REC(i) \
+    if (events) HIPCHK(hipEventRecord(f.ev[i], s))
+    // BSA_KERNEL_MFMA32 and _FP32MC: the child adjoints and the interior
+    // children's branch matrices on the FP32 matrix cores
+    const bool mfma32 = sizeof(T) == 4 &&
+                        (f.kernel == BSA_KERNEL_MFMA32 || f.kernel == BSA_KERNEL_FP32MC);
+    for (size_t d = 0; d + 1 < f.adj_off.size(); d++) {
+        if (mfma32)
+            launch_adjoint_level32(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
+                                   rs ? rs->adl[d] : nullptr,
+                                   rs ? rs->nadl[d] : f.adj_nint[d], f.npad, f.d_z,
+                                   (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
+                                   f.d_E, (const float *)f.d_C, (float *)f.d_A, f.d_EA,
+                                   f.tpb_set);
+        else
+            launch_adjoint_level<T>(s, rs ? f.d_adj : f.d_adj + f.adj_off[d],
+                                    rs ? rs->adl[d] : nullptr,
+                                    rs ? rs->nadl[d] : f.adj_nint[d], f.npad, f.d_z,
+                                    (const double *)f.d_PT, (const double *)f.d_AT, f.ncodes,
+                                    f.d_E, (const T *)f.d_C, (T *)f.d_A, f.d_EA, f.tpb_set);
+    }
+    REC(6);
+    launch_adjoint_m<T>(s, f.d_adj, rs ? rs->adm : nullptr, rs ? rs->nadm : (int)f.adj.size(),
+                        f.npad, f.nchunk, f.tpc, mfma32, f.d_z, (const double *)f.d_PT,
+                        (const double *)f.d_AT, f.ncodes, f.d_mask64, f.d_tperm,
+                        (const T *)f.d_L, f.d_E, (const T *)f.d_C, (const T *)f.d_A, f.d_EA,
+                        f.d_Mpart);
+    REC(4);
+    launch_adjoint_finish(s, f.d_Mpart, f.nchunk, f.d_ejobs, rs ? rs->ej : nullptr,
+                          rs ? rs->nej : (int)f.ejobs.size(), rs ? rs->sets : nullptr,
+                          rs ? rs->nsets : 3 * f.G, f.d_set_off, f.d_set_list, f.d_uvr, f.d_G,
+                          f.d_dtau, f.d_W, f.d_Z);
+    HIPCHK(hipGetLastError());
+    REC(5);
+#undef REC
+    return BSA_OK;
+}
+
+template <typename T>
+static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
+                          const RoundSel *rs = nullptr)
+{
+    int rc = enqueue_forward<T>(f, s, adjoint, events, rs);
+    if (!rc && adjoint) rc = enqueue_adjoint<T>(f, s, events, rs);
+    return rc;
 }
 
 // device times of the phases of the last evaluation from the stream's events
@@ -1073,6 +1094,62 @@ static int run_device(Forest &f, bool adjoint)
     int nl = 1;   // the branch-matrix launch, plus one per depth with interior children
     for (int v : f.adj_nint) nl += v > 0;
     return read_timing(f, adjoint, nl);
+}
+
+// The two parts of an evaluation with gradient, run one after the other:
+// part 1 the forward pass that keeps what the sweep needs (and the staged
+// inputs of every phase), part 2 the adjoint sweep of the last part 1.  The
+// device work and its order are those of run_device(f, true), so the
+// results are the same bitwise.
+template <typename T>
+static int run_part(Forest &f, int part)
+{
+    hipStream_t s = f.stream;
+    if (part == 1) {
+        HIPCHK(hipMemcpyAsync(f.d_uvr, f.h_uvr, sizeof(double) * f.model_block(),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(f.d_pjobs, f.h_pjobs, sizeof(PmatJob) * f.pjobs.size(),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(f.d_ejobs, f.h_ejobs, sizeof(EigJob) * f.ejobs.size(),
+                              hipMemcpyHostToDevice, s));
+        int rc = enqueue_forward<T>(f, s, true, true);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(f.h_out, f.d_out, sizeof(double) * kRootSums * f.G,
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        // the forward phases of this evaluation; the sweep's are added by part 2
+        rc = read_timing(f, false, 0);
+        return rc;
+    }
+    HIPCHK(hipEventRecord(f.ev[3], s));
+    int rc = enqueue_adjoint<T>(f, s, true);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(f.h_adjout, f.d_dtau, sizeof(double) * f.ejobs.size(),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(f.h_adjout + f.ejobs.size(), f.d_Z, sizeof(double) * 3 * f.G * 4096,
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int nl = 1;
+    for (int v : f.adj_nint) nl += v > 0;
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, f.ev[3], f.ev[6]));
+    f.timing[3] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, f.ev[6], f.ev[4]));
+    f.timing[4] = ms;
+    HIPCHK(hipEventElapsedTime(&ms, f.ev[4], f.ev[5]));
+    f.timing[5] = ms;
+    for (int i = 3; i < 6; i++) f.timing_sum[i] += f.timing[i];
+    f.n_adj_launch += nl;
+    return BSA_OK;
+}
+
+int forest_run_part(Forest &f, int part)
+{
+    if (part != 1 && part != 2) return fail(BSA_EINVAL, "evaluation part %d: 1 or 2", part);
+    if (!f.adj_ready) return fail(BSA_EINVAL, "adjoint not set up");
+    if (f.use_graph) return fail(BSA_EINVAL, "split evaluations need eager launches (no graph)");
+    HIPCHK(hipSetDevice(f.dev));
+    return f.precision == BSA_FP32 ? run_part<float>(f, part) : run_part<double>(f, part);
 }
 
 // copies of the per-gene ranges [base(g), base(g) + len(g)) of the listed

@@ -285,3 +285,33 @@ def test_driver_lnf_matches_reference():
         assert a[:2] == b[:2] and a[5:] == b[5:]
         assert abs(float(a[2]) - float(b[2])) <= 1e-10 * abs(float(b[2])) + 2e-10
         assert abs(float(a[4]) - float(b[4])) <= 1.5e-4
+
+
+@pytest.mark.parametrize("gradient", ["adjoint", "fd"])
+def test_fit_asks_gradients_only_at_accepted_points(gradient):
+    """bsa_fit evaluates a trial point's likelihood alone and asks for the
+    gradient only where it accepts the point (the adjoint sweep of the kept
+    forward pass, or gradientB's differences).  The trajectory is that of an
+    optimiser given value and gradient at every trial point, bitwise: the
+    same minimiser on a host objective that evaluates both (lnl_grad or
+    gradientB) stops at the same x and -lnL after the same iterations and
+    evaluations."""
+    from cucodeml_amd import engine
+    print(f"case: gradient only at accepted points, ml16 H1, {gradient}", flush=True)
+    fx, tree, data = fixtures.load("ml16")
+    eng = engine.BranchSiteEngine(tree, data)
+    try:
+        x0 = codeml.default_start(tree, 2.0, 1.5, False)
+        lo, hi = eng.default_bounds()
+        xs, fs, ins = eng.fit(x0, lo, hi, max_iter=40, gradient=gradient)
+        both = eng.lnl_grad if gradient == "adjoint" else eng.gradientB
+        xb, fb, inb = engine.minimize(both, x0, lo, hi, max_iter=40)
+    finally:
+        eng.close()
+    print(f"  fit: -lnL {fs:.10f}, {ins['iterations']} iterations, {ins['evaluations']} "
+          f"evaluations; both at every trial: -lnL {fb:.10f}, {inb['iterations']} iterations, "
+          f"{inb['evaluations']} evaluations", flush=True)
+    assert np.array_equal(xs, xb) and fs == fb
+    assert ins["iterations"] == inb["iterations"]
+    # bsa_fit counts its final evaluation at the returned x too
+    assert ins["evaluations"] == inb["evaluations"] + 1
