@@ -344,12 +344,10 @@ def scan_device_record(stats, wall):
 
 def large_fit(ns=512, ncodon=50000, gap=0.1, seed=4, e=1e-8):
     """Full ML fit (H1) of the large-gene workload, 512 taxa x 50,000 codons
-    with 10 % gaps, from codeml's start (all branches 0.1): with FP64
-    partials; with FP32 partials finished on FP64 (as the driver does); and
-    with FP32 partials started at the FP64 estimates, which shows where FP32
-    partials put the same maximum.  The engine's bounded BFGS runs on a
-    Python objective that prints its progress, so that a long fit reports
-    as it goes."""
+    with 10 % gaps, from codeml's start (all branches 0.1), by the driver's
+    own fit (codeml.fit_model_a): with FP64 partials; with FP32 partials
+    finished on FP64; and, from the FP64 estimates, the FP32 partials' own
+    fit, which shows where FP32 partials put the same maximum."""
     from cucodeml_amd import codeml
     rng = np.random.default_rng(seed)
     tree = simulate.yule_tree(ns, 0.02, rng)
@@ -360,61 +358,49 @@ def large_fit(ns=512, ncodon=50000, gap=0.1, seed=4, e=1e-8):
     data = formats.load_codon_data(tree.names, seqs)
     x0 = codeml.default_start(tree, 2.0, 1.5, False)
     x0[:tree.nbranch] = 0.1
+    names = ["kappa", "p0", "p1", "w0", "w2"]
 
-    def fit(precision, x, label):
-        eng = engine.BranchSiteEngine(tree, data, precision=precision)
-        n = [0]
+    def fit(precision, x, label, polish=True):
         t0 = time.perf_counter()
+        r = codeml.fit_model_a(tree, data, precision=precision, start=x, e=e, polish=polish)
+        dt = time.perf_counter() - t0
+        print(f"large fit {label}: {r.evaluations} evaluations, {r.iterations} iterations, "
+              f"-lnL {-r.lnL:.6f}, {dt:.1f} s", file=sys.stderr, flush=True)
+        return r, dt
 
-        def fun(v):
-            f, g = eng.lnl_grad(v)
-            n[0] += 1
-            if n[0] % 100 == 0:
-                print(f"large fit {label}: {n[0]} evaluations, -lnL {f:.6f}, "
-                      f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
-            return f, g
-        try:
-            lo, hi = eng.default_bounds()
-            xs, f, info = engine.minimize(fun, x, lo, hi, e=e, max_iter=5000)
-            return xs, f, info, n[0], time.perf_counter() - t0
-        finally:
-            eng.close()
+    def est(r):
+        return dict(zip(names, [r.kappa, r.p0, r.p1, r.w0, r.w2]))
 
-    x64, f64, i64, n64, t64 = fit("fp64", x0, "fp64")
-    x32, f32, i32, n32, t32 = fit("fp32", x0, "fp32")
-    x32p, f32p, i32p, n32p, t32p = fit("fp64", x32, "fp32 finished on fp64")
+    r64, t64 = fit("fp64", x0, "fp64")
+    r32, t32 = fit("fp32", x0, "fp32 finished on fp64")
     # the FP32 fit started at the FP64 estimates: where FP32 partials put
     # the same maximum
-    x32m, f32m, i32m, n32m, t32m = fit("fp32", x64, "fp32 from the fp64 estimates")
+    r32m, _ = fit("fp32", r64.x, "fp32 from the fp64 estimates", polish=False)
     e64 = engine.BranchSiteEngine(tree, data, precision="fp64")
     try:
-        f32m_64 = e64.lnl(x32m)
+        f32m_64 = e64.lnl(r32m.x)
     finally:
         e64.close()
     nb = tree.nbranch
-    par = lambda x: np.array([x[nb], *engine_ps(x, nb), x[nb + 3], x[nb + 4]])   # noqa: E731
-    names = ["kappa", "p0", "p1", "w0", "w2"]
+    par = lambda r: np.array([r.kappa, r.p0, r.p1, r.w0, r.w2])   # noqa: E731
+    ex = r32.extra
     return {"workload": f"{ns} taxa x {ncodon} codons ({data.npatt} patterns), {gap:.0%} gaps, "
                         f"H1 from all branches 0.1", "e": e,
-            "fp64": {"neg_lnl": f64, "evaluations": n64, "iterations": i64["iterations"],
-                     "seconds": t64, "estimates": dict(zip(names, par(x64).tolist()))},
-            "fp32_then_fp64": {"neg_lnl_fp32": f32, "neg_lnl_after_fp64_finish": f32p,
-                               "evaluations": n32 + n32p,
-                               "iterations": i32["iterations"] + i32p["iterations"],
-                               "seconds": t32 + t32p, "fp32_part_seconds": t32,
-                               "estimates": dict(zip(names, par(x32p).tolist())),
-                               "delta_neg_lnl_vs_fp64": f32p - f64},
+            "fp64": {"neg_lnl": -r64.lnL, "evaluations": r64.evaluations,
+                     "iterations": r64.iterations, "stop": r64.stop, "seconds": t64,
+                     "estimates": est(r64)},
+            "fp32_then_fp64": {"neg_lnl_fp32": -ex["fp32_lnL"], "fp32_stop": ex["fp32_stop"],
+                               "fp32_evaluations": ex["fp32_evaluations"],
+                               "neg_lnl_after_fp64_finish": -r32.lnL,
+                               "evaluations": r32.evaluations, "iterations": r32.iterations,
+                               "stop": r32.stop, "seconds": t32,
+                               "estimates": est(r32),
+                               "delta_neg_lnl_vs_fp64": r64.lnL - r32.lnL},
             "fp32_at_the_fp64_maximum": {
-                "neg_lnl_fp32": f32m, "neg_lnl_fp64_at_its_estimates": f32m_64,
-                "delta_neg_lnl_vs_fp64": f32m_64 - f64, "evaluations": n32m,
-                "max_abs_param_diff": float(np.abs(par(x32m) - par(x64)).max()),
-                "max_abs_branch_diff": float(np.abs(x32m[:nb] - x64[:nb]).max())}}
-
-
-def engine_ps(x, nb):
-    """p0, p1 from codeml's logistic variables"""
-    a, b = np.exp(x[nb + 1]), np.exp(x[nb + 2])
-    return a / (1 + a + b), b / (1 + a + b)
+                "neg_lnl_fp32": -r32m.lnL, "neg_lnl_fp64_at_its_estimates": f32m_64,
+                "delta_neg_lnl_vs_fp64": f32m_64 + r64.lnL, "evaluations": r32m.evaluations,
+                "max_abs_param_diff": float(np.abs(par(r32m) - par(r64)).max()),
+                "max_abs_branch_diff": float(np.abs(r32m.x[:nb] - r64.x[:nb]).max())}}
 
 
 def main():

@@ -131,6 +131,14 @@ def _bounds(eng_bounds, x0, nb, fixed_branches):
 
 # Newton iterations of the last phase of an FP64 finish
 FINISH_NEWTON = 8
+# Newton iterations of an FP64 finish on more free variables than
+# DENSE_NEWTON_MAX, whose Newton steps come from preconditioned conjugate
+# gradients (about 40 gradients each, where the Hessian costs one per
+# variable): there Newton iterations alone are the cheaper finish (512 taxa
+# x 50,000 codons: the FP32 estimates 0.78 below the maximum in lnL, along
+# the p2 w2 ridge; 7 Newton iterations, 14 s, reach it)
+FINISH_NEWTON_LARGE = 16
+DENSE_NEWTON_MAX = 160
 
 
 def finish_fp64(fit, x, lo, hi, e, gradient="adjoint"):
@@ -144,10 +152,13 @@ def finish_fp64(fit, x, lo, hi, e, gradient="adjoint"):
     a curved, nearly flat ridge (p2 w2 ~ const: one scan gene stopped at
     w2 = 2.58 where the maximum has 1.91, 2e-4 below it in lnL); Newton
     steps follow such a ridge at a linear rate and BFGS crawls, so the
-    last phase gives Newton the iterations to get there.  ``fit`` is an
-    engine's fit method."""
+    last phase gives Newton the iterations to get there.  With more than
+    DENSE_NEWTON_MAX free variables the first phase is FINISH_NEWTON_LARGE
+    Newton iterations.  ``fit`` is an engine's fit method."""
     if gradient == "adjoint":
-        x1, f1, i1 = fit(x, lo, hi, e=e, max_iter=-1, newton=3)
+        large = np.count_nonzero(np.asarray(lo) < np.asarray(hi)) > DENSE_NEWTON_MAX
+        x1, f1, i1 = fit(x, lo, hi, e=e, max_iter=-1,
+                         newton=FINISH_NEWTON_LARGE if large else 3)
         if i1["converged"]:
             return x1, f1, i1
         x2, f2, i2 = fit(x1, lo, hi, e=e, newton=4)

@@ -324,8 +324,9 @@ static int run_fit_chains(bsa_batch *b, const bsa_fit_opts *opts,
                 gene_chain_rule(g, f, xx, gv);
                 return BSA_OK;
             };
-            return minimize_box(fg, gene_num_params(g), x0, lo, hi, opts, res,
-                                f.precision == BSA_FP64);
+            const int np = gene_num_params(g);
+            return minimize_box(fg, np, x0, lo, hi, opts, res, f.precision == BSA_FP64,
+                                np - g.nbranch);
         };
         const int r = chain(i, fit);
         if (r) {

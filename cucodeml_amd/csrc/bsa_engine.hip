@@ -26,6 +26,7 @@ struct bsa_engine {
 
 int bsa_engine_precision(const bsa_engine *eng) { return eng->f.precision; }
 bool bsa_engine_graph(const bsa_engine *eng) { return eng->f.use_graph; }
+int bsa_engine_nbranch(const bsa_engine *eng) { return eng->g.nbranch; }
 
 extern "C" int bsa_create(const bsa_problem *p, bsa_engine **out)
 {

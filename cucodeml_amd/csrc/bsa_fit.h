@@ -44,15 +44,20 @@ struct FitResult {
 // iterations check whether x is the maximum (BSA_STOP_NEWTON if so); for
 // exact FP64 gradients (the adjoint on FP64 partials), whose differences give
 // the Hessian
+// nblock: the last nblock variables (in codeml's x, the model parameters
+// after the branch lengths), whose Hessian block Newton steps on more than
+// 160 free variables form exactly (their conjugate-gradient preconditioner)
 // The gradient is asked for at the start point, at every accepted point
-// and for the Newton finish's Hessian columns, never at a rejected trial
+// and for the Newton finish's Hessian products, never at a rejected trial
 // point; res->evaluations counts the value calls.
 int minimize_box(const Objective &obj, int n, double *x, const double *lo, const double *hi,
-                 const bsa_fit_opts *opts, FitResult *res, bool verify = false);
+                 const bsa_fit_opts *opts, FitResult *res, bool verify = false,
+                 int nblock = 0);
 inline int minimize_box(const ObjFn &fg, int n, double *x, const double *lo, const double *hi,
-                        const bsa_fit_opts *opts, FitResult *res, bool verify = false)
+                        const bsa_fit_opts *opts, FitResult *res, bool verify = false,
+                        int nblock = 0)
 {
-    return minimize_box(combined(fg, n), n, x, lo, hi, opts, res, verify);
+    return minimize_box(combined(fg, n), n, x, lo, hi, opts, res, verify, nblock);
 }
 
 }  // namespace bsa

@@ -111,7 +111,7 @@ def lib():
         L.bsa_fit.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, C.POINTER(_FitOpts), dp,
                               C.POINTER(C.c_int)]
         L.bsa_debug_minimize.argtypes = [C.c_int, _OBJECTIVE, C.c_void_p, dp, dp, dp,
-                                         C.POINTER(_FitOpts), dp, C.POINTER(C.c_int)]
+                                         C.POINTER(_FitOpts), C.c_int, dp, C.POINTER(C.c_int)]
         pp, dpp = C.POINTER(C.POINTER(_Problem)), C.POINTER(dp)
         ip = C.POINTER(C.c_int)
         L.bsa_batch_create.argtypes = [pp, C.c_int, C.POINTER(C.c_void_p)]
@@ -570,9 +570,11 @@ def forest_plan(genes, precision="fp64"):
     return {k: int(v) for k, v in zip(keys, o)}
 
 
-def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0, newton=0):
+def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0, newton=0, block=0):
     """The engine's bounded BFGS on a Python objective fun(x) -> (f, grad)
-    (diagnostic; runs on the host).  Returns (x, f, info)."""
+    (diagnostic; runs on the host).  block: the last `block` variables,
+    whose Hessian block Newton steps on more than 160 free variables form
+    exactly (bsa_debug_minimize).  Returns (x, f, info)."""
     n = len(x0)
 
     def cb(xp, fp, gp, _ctx):
@@ -596,7 +598,7 @@ def minimize(fun, x0, lo, hi, e=1e-8, max_iter=2000, max_step=5.0, newton=0):
     f = C.c_double()
     info = (C.c_int * 3)()
     _check(lib().bsa_debug_minimize(n, cfun, None, _dptr(x), _dptr(lo), _dptr(hi), C.byref(o),
-                                    C.byref(f), info))
+                                    int(block), C.byref(f), info))
     return x, f.value, fit_info(info[0], info[1], info[2])
 
 

@@ -274,10 +274,15 @@ int  bsa_fit(bsa_engine *eng, double *x, int np, const double *lo, const double 
 
 /* diagnostic: the same optimiser on a caller-supplied objective (returns
  * nonzero where it is undefined), for testing it without a GPU; info as in
- * bsa_fit (no counterpart in the reference) */
+ * bsa_fit.  nblock (0..n): the last nblock variables, whose Hessian block
+ * Newton iterations on more than 160 free variables form exactly, as the
+ * preconditioner of their conjugate-gradient step (bsa_fit names codeml's
+ * model parameters, after the branch lengths).  No counterpart in the
+ * reference. */
 typedef int (*bsa_objective)(const double *x, double *f, double *grad, void *ctx);
 int  bsa_debug_minimize(int n, bsa_objective fn, void *ctx, double *x, const double *lo,
-                        const double *hi, const bsa_fit_opts *opts, double *fmin, int *info3);
+                        const double *hi, const bsa_fit_opts *opts, int nblock, double *fmin,
+                        int *info3);
 
 /* ------------------------------------------------------------------------
  * Batch engine: many genes, each with its own tree, alignment and model

@@ -112,3 +112,57 @@ def test_newton_only_from_a_near_point_and_on_a_bound():
     _, g = fun(x)
     free = np.arange(n) != 3
     assert np.max(np.abs(g[free])) < 1e-6
+
+
+def _ridge_large(n=300, nblock=5, seed=3):
+    """A tree-likelihood-like Hessian on n variables: the bulk of the
+    spectrum log-normal about 3e6 (1e4.3 .. 1e8.5), and two flat directions
+    (curvature 0.2 and 7.8) inside the last nblock coordinates, as the
+    p2 w2 ridge lies in codeml's model parameters after the branch lengths.
+    The start is 0.6 and 0.3 along the flat directions from the minimum c
+    and 1e-4 off it in the steep ones, as where an FP32 fit stops."""
+    rng = np.random.default_rng(seed)
+    U = np.zeros((n, n))
+    F, _ = np.linalg.qr(rng.normal(size=(nblock, 2)))
+    U[n - nblock:, :2] = F
+    R = rng.normal(size=(n, n - 2))
+    R -= U[:, :2] @ (U[:, :2].T @ R)
+    U[:, 2:], _ = np.linalg.qr(R)
+    lam = np.concatenate([[0.2, 7.8], 10 ** np.clip(rng.normal(6.5, 0.45, n - 2), 4.3, 8.5)])
+    A = (U * lam) @ U.T
+    A = 0.5 * (A + A.T)
+    c = rng.uniform(-0.5, 0.5, n)
+    x0 = c + U[:, :2] @ np.array([0.6, 0.3]) + U[:, 2:] @ (rng.normal(size=n - 2) * 1e-4)
+    return (lambda x: (0.5 * (x - c) @ A @ (x - c) + 5e6, A @ (x - c))), A, c, x0
+
+
+def test_newton_on_many_variables_preconditions_with_the_trailing_block():
+    """More than 160 free variables: Newton steps from preconditioned
+    conjugate gradients (the Hessian block of the last `block` variables
+    formed exactly, a scalar on the rest), at most 5 + 32 gradients each
+    plus the line search.  With the block they land on the minimum along
+    the flat directions; plain conjugate gradients (block 0) on the same
+    budget barely move along them."""
+    fun, A, c, x0 = _ridge_large()
+    n = len(c)
+    lo, hi = -np.ones(n), np.ones(n)
+    x, _, info = engine.minimize(fun, x0, lo, hi, max_iter=-1, newton=6, block=5)
+    assert info["converged"] and info["stop"] == "newton"
+    assert np.max(np.abs(x - c)) < 1e-8
+    assert info["evaluations"] <= 1 + 6 * (5 + 32 + 30)
+    xp, _, ip = engine.minimize(fun, x0, lo, hi, max_iter=-1, newton=6, block=0)
+    assert not ip["converged"] and np.max(np.abs(xp - c)) > 0.1
+
+
+def test_newton_on_many_variables_with_a_bound():
+    # the minimum of the last variable lies outside the box: it stays on
+    # its bound and the other variables minimise with it held there
+    fun, A, c, x0 = _ridge_large()
+    n = len(c)
+    lo, hi = -np.ones(n), np.ones(n)
+    hi[-1] = c[-1] - 0.2
+    x, _, info = engine.minimize(fun, np.clip(x0, lo, hi), lo, hi, max_iter=-1, newton=6,
+                                 block=5)
+    assert info["converged"] and x[-1] == hi[-1]
+    _, g = fun(x)
+    assert np.max(np.abs(g[:-1])) < 1e-3 and g[-1] < 0
