@@ -73,11 +73,15 @@ int parallel_for(int n, Fn fn)
 // points its own fit asks for, and a gene's result does not depend on which
 // other genes share its round.
 struct RoundServer {
+    // what a request asks for: the forward pass at the gene's staged point
+    // (kept for a later sweep), the adjoint sweep of its last forward pass,
+    // or both
+    enum { kForward = 1, kSweep = 2, kBoth = 3 };
     std::mutex mu;
     std::condition_variable cv_server;
     std::unique_ptr<std::condition_variable[]> cv_gene;
     Forest *f = nullptr;
-    std::vector<int> queue;          // staged genes waiting for a round
+    std::vector<std::pair<int, int>> queue;   // (gene, kind) waiting for a round
     std::vector<char> ready;         // per gene: the results of its request are in
     std::vector<int> grc;            // per gene: status of its last round
     std::vector<std::string> gerr;
@@ -91,11 +95,11 @@ struct RoundServer {
         gerr.assign(f->G, std::string());
         fitting = f->G;
     }
-    int request(int g)
+    int request(int g, int kind)
     {
         std::unique_lock<std::mutex> lk(mu);
         ready[g] = 0;
-        queue.push_back(g);
+        queue.emplace_back(g, kind);
         cv_server.notify_one();
         cv_gene[g].wait(lk, [&] { return ready[g] != 0; });
         return grc[g] ? fail(grc[g], "%s", gerr[g].c_str()) : BSA_OK;
@@ -112,15 +116,22 @@ struct RoundServer {
         for (;;) {
             cv_server.wait(lk, [&] { return !queue.empty() || fitting == 0; });
             if (queue.empty()) return;   // every fit has finished
-            std::vector<int> gl;
-            gl.swap(queue);
+            std::vector<std::pair<int, int>> q;
+            q.swap(queue);
             lk.unlock();
-            std::sort(gl.begin(), gl.end());
-            const int rc = forest_run_round(*f, gl, true);
+            std::vector<int> fwd, adj;
+            for (const auto &r : q) {
+                if (r.second & kForward) fwd.push_back(r.first);
+                if (r.second & kSweep) adj.push_back(r.first);
+            }
+            std::sort(fwd.begin(), fwd.end());
+            std::sort(adj.begin(), adj.end());
+            const int rc = forest_run_round(*f, fwd, adj, true);
             const std::string err = rc ? last_error_text() : std::string();
             lk.lock();
             rounds++;
-            for (int g : gl) {
+            for (const auto &r : q) {
+                const int g = r.first;
                 grc[g] = rc;
                 gerr[g] = err;
                 ready[g] = 1;
@@ -316,16 +327,35 @@ static int run_fit_chains(bsa_batch *b, const bsa_fit_opts *opts,
     auto chain_one = [&](int i) {
         Gene &g = *b->genes[i];
         const GeneFit fit = [&](double *x0, const double *lo, const double *hi, FitResult *res) {
-            ObjFn fg = [&](const double *xx, double *fv, double *gv) {
+            // a line-search trial asks for the forward pass alone; the sweep
+            // follows in a later round where the optimiser accepts the point
+            const int np = gene_num_params(g);
+            std::vector<double> xk(np);   // point of the gene's last forward pass
+            auto run = [&](const double *xx, double *fv, int kind) {
                 int r = gene_stage_x(g, f, xx, true);
                 if (r) return r;
-                if ((r = srv.request(i))) return r;
+                if ((r = srv.request(i, kind))) return r;
+                std::copy(xx, xx + np, xk.begin());
                 *fv = -gene_collect(g, f);
+                return BSA_OK;
+            };
+            Objective obj;
+            obj.value = [&](const double *xx, double *fv) {
+                return run(xx, fv, RoundServer::kForward);
+            };
+            obj.grad = [&](double *gv) {
+                const int r = srv.request(i, RoundServer::kSweep);
+                if (r) return r;
+                gene_chain_rule(g, f, xk.data(), gv);
+                return BSA_OK;
+            };
+            obj.both = [&](const double *xx, double *fv, double *gv) {
+                const int r = run(xx, fv, RoundServer::kBoth);
+                if (r) return r;
                 gene_chain_rule(g, f, xx, gv);
                 return BSA_OK;
             };
-            const int np = gene_num_params(g);
-            return minimize_box(fg, np, x0, lo, hi, opts, res, f.precision == BSA_FP64,
+            return minimize_box(obj, np, x0, lo, hi, opts, res, f.precision == BSA_FP64,
                                 np - g.nbranch);
         };
         const int r = chain(i, fit);

@@ -430,6 +430,10 @@ int minimize_box(const Objective &obj, int n, double *x, const double *lo, const
     };
     auto gr = [&](double *gg) { return obj.grad(gg); };
     auto fg = [&](const double *xx, double *ff, double *gg) {
+        if (obj.both) {
+            nfev++;
+            return obj.both(xx, ff, gg);
+        }
         const int rc = fv(xx, ff);
         return rc ? rc : gr(gg);
     };

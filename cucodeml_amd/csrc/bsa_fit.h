@@ -14,10 +14,14 @@ using ObjFn = std::function<int(const double *x, double *f, double *g)>;
 
 // objective in two calls, for an optimiser that needs the gradient only at
 // the points it accepts: value(x, &f) makes x the current point, grad(g)
-// writes the gradient at the current point; both return bsa status codes
+// writes the gradient at the current point; both return bsa status codes.
+// both (optional): value and gradient at x in one call, where the caller
+// knows it will want both (the start, a Hessian column); it too makes x
+// the current point.
 struct Objective {
     std::function<int(const double *x, double *f)> value;
     std::function<int(double *g)> grad;
+    ObjFn both;
 };
 
 // the two calls from one combined function: value computes both and keeps

@@ -206,6 +206,14 @@ int forest_run_part(Forest &f, int part);
 // cross PCIe; other genes' staging buffers may be written meanwhile.  Eager
 // launches (no graph).
 int forest_run_round(Forest &f, const std::vector<int> &genes, bool adjoint);
+// a round of forward passes for the genes fwd and adjoint sweeps for the
+// genes adj (ascending gene indices each; either list may be empty).  With
+// keep the forward passes store what a sweep needs; a gene in adj gets the
+// sweep of its last forward pass run with keep: this round's where it is in
+// fwd too, else an earlier round's (gene regions of the device buffers are
+// disjoint, so other genes' rounds leave it intact).
+int forest_run_round(Forest &f, const std::vector<int> &fwd, const std::vector<int> &adj,
+                     bool keep);
 double gene_collect(Gene &g, const Forest &f);
 // d(-lnL)/dx of gene g from the adjoint outputs of the last forest_run
 void gene_chain_rule(const Gene &g, const Forest &f, const double *x, double *grad);

@@ -1025,7 +1025,8 @@ static int enqueue_device(Forest &f, hipStream_t s, bool adjoint, bool events,
 }
 
 // device times of the phases of the last evaluation from the stream's events
-static int read_timing(Forest &f, bool adjoint, int nadj_launch)
+// (forward: whether the pruning ran, for the launch count)
+static int read_timing(Forest &f, bool adjoint, int nadj_launch, bool forward = true)
 {
     float ms;
     HIPCHK(hipEventElapsedTime(&ms, f.ev[0], f.ev[1]));
@@ -1045,7 +1046,7 @@ static int read_timing(Forest &f, bool adjoint, int nadj_launch)
         f.n_adj_launch += nadj_launch;
     }
     for (int i = 0; i < 6; i++) f.timing_sum[i] += f.timing[i];
-    f.n_launch += f.prune_walk ? 1.0 : (double)(f.level_off.size() - 1);
+    if (forward) f.n_launch += f.prune_walk ? 1.0 : (double)(f.level_off.size() - 1);
     return BSA_OK;
 }
 
@@ -1172,10 +1173,16 @@ static int copy_runs(hipStream_t s, void *dst, const void *src, size_t unit,
     return BSA_OK;
 }
 
+// A round: the forward passes of the genes gl (with keep, storing what a
+// sweep needs) and the adjoint sweeps of the genes ga, each of the last
+// forward pass run with keep for that gene (this round's where the gene is
+// in both lists).  Gene regions of the device buffers are disjoint, so a
+// gene's kept forward pass survives other genes' rounds.
 template <typename T>
-static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
+static int run_round(Forest &f, const std::vector<int> &gl, const std::vector<int> &ga, bool keep)
 {
     hipStream_t s = f.stream;
+    const bool adjoint = !ga.empty();
     const size_t nlev = f.level_off.size() - 1, nd = adjoint ? f.adj_off.size() - 1 : 0;
     // job lists: P(t), pruning levels, genes, child adjoints per depth,
     // branch matrices, eigen-basis jobs, eigen sets
@@ -1226,27 +1233,27 @@ static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
     if (adjoint) {
         for (size_t d = 0; d < nd; d++) {
             off.push_back(n);
-            for (int g : gl) put_range(f.genes[g]->adj_int_rng[d][0], f.genes[g]->adj_int_rng[d][1]);
+            for (int g : ga) put_range(f.genes[g]->adj_int_rng[d][0], f.genes[g]->adj_int_rng[d][1]);
             rs.nadl.push_back((int)(n - off.back()));
         }
         off.push_back(n);
         for (size_t d = 0; d < nd; d++)
-            for (int g : gl) {
+            for (int g : ga) {
                 const Gene &ge = *f.genes[g];
                 put_range(ge.adj_int_rng[d][0], ge.adj_int_rng[d][1]);
                 put_range(ge.adj_tip_rng[d][0], ge.adj_tip_rng[d][1]);
             }
         rs.nadm = (int)(n - off.back());
         off.push_back(n);
-        for (int g : gl) {
+        for (int g : ga) {
             const Gene &ge = *f.genes[g];
             put_range(ge.ej_base, ge.ej_base + (int)ge.ejobs.size());
         }
         rs.nej = (int)(n - off.back());
         off.push_back(n);
-        for (int g : gl)
+        for (int g : ga)
             for (int k = 0; k < 3; k++) h[n++] = 3 * g + k;
-        rs.nsets = 3 * (int)gl.size();
+        rs.nsets = 3 * (int)ga.size();
     }
     if (n > f.sel_cap) return fail(BSA_EINVAL, "round job lists exceed their buffer");
     HIPCHK(hipMemcpyAsync(f.d_sel, h, sizeof(int) * n, hipMemcpyHostToDevice, s));
@@ -1271,18 +1278,19 @@ static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
                        [&](int g) { return (size_t)G[g]->pj_base; },
                        [&](int g) { return G[g]->pjobs.size(); }, hipMemcpyHostToDevice);
     if (!rc && adjoint)
-        rc = copy_runs(s, f.d_ejobs, f.h_ejobs, sizeof(EigJob), gl,
+        rc = copy_runs(s, f.d_ejobs, f.h_ejobs, sizeof(EigJob), ga,
                        [&](int g) { return (size_t)G[g]->ej_base; },
                        [&](int g) { return G[g]->ejobs.size(); }, hipMemcpyHostToDevice);
     if (rc) return rc;
-    if ((rc = enqueue_device<T>(f, s, adjoint, true, &rs))) return rc;
+    if ((rc = enqueue_forward<T>(f, s, keep, true, &rs))) return rc;
+    if (adjoint && (rc = enqueue_adjoint<T>(f, s, true, &rs))) return rc;
     // their outputs
     if (adjoint) {
-        rc = copy_runs(s, f.h_adjout, f.d_dtau, sizeof(double), gl,
+        rc = copy_runs(s, f.h_adjout, f.d_dtau, sizeof(double), ga,
                        [&](int g) { return (size_t)G[g]->ej_base; },
                        [&](int g) { return G[g]->ejobs.size(); }, hipMemcpyDeviceToHost);
         if (!rc)
-            rc = copy_runs(s, f.h_adjout + f.ejobs.size(), f.d_Z, sizeof(double), gl,
+            rc = copy_runs(s, f.h_adjout + f.ejobs.size(), f.d_Z, sizeof(double), ga,
                            [](int g) { return (size_t)g * 3 * 4096; },
                            [](int) { return (size_t)3 * 4096; }, hipMemcpyDeviceToHost);
         if (rc) return rc;
@@ -1294,19 +1302,26 @@ static int run_round(Forest &f, const std::vector<int> &gl, bool adjoint)
     HIPCHK(hipStreamSynchronize(s));
     int nl = 1;
     for (int v : rs.nadl) nl += v > 0;
-    return read_timing(f, adjoint, nl);
+    return read_timing(f, adjoint, nl, !gl.empty());
+}
+
+int forest_run_round(Forest &f, const std::vector<int> &fwd, const std::vector<int> &adj,
+                     bool keep)
+{
+    for (const std::vector<int> *v : {&fwd, &adj})
+        for (size_t i = 0; i < v->size(); i++)
+            if ((*v)[i] < 0 || (*v)[i] >= f.G || (i > 0 && (*v)[i] <= (*v)[i - 1]))
+                return fail(BSA_EINVAL, "round genes must be ascending gene indices");
+    if (fwd.empty() && adj.empty()) return BSA_OK;
+    if ((keep || !adj.empty()) && !f.adj_ready) return fail(BSA_EINVAL, "adjoint not set up");
+    HIPCHK(hipSetDevice(f.dev));
+    return f.precision == BSA_FP32 ? run_round<float>(f, fwd, adj, keep)
+                                   : run_round<double>(f, fwd, adj, keep);
 }
 
 int forest_run_round(Forest &f, const std::vector<int> &genes, bool adjoint)
 {
-    for (size_t i = 0; i < genes.size(); i++)
-        if (genes[i] < 0 || genes[i] >= f.G || (i > 0 && genes[i] <= genes[i - 1]))
-            return fail(BSA_EINVAL, "round genes must be ascending gene indices");
-    if (genes.empty()) return BSA_OK;
-    if (adjoint && !f.adj_ready) return fail(BSA_EINVAL, "adjoint not set up");
-    HIPCHK(hipSetDevice(f.dev));
-    return f.precision == BSA_FP32 ? run_round<float>(f, genes, adjoint)
-                                   : run_round<double>(f, genes, adjoint);
+    return forest_run_round(f, genes, adjoint ? genes : std::vector<int>(), adjoint);
 }
 
 int forest_run(Forest &f, bool adjoint)

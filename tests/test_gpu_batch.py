@@ -275,3 +275,46 @@ def test_chained_lrt_fits_equal_separate_batch_fits():
             assert info[i]["h1_evaluations"] == i1[i]["evaluations"]
         # the engine is left at the H1 estimates
         assert abs(par_c[i][4] - (x1c[i][-1])) <= 1e-12 * max(1.0, abs(x1c[i][-1]))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_fit_asks_sweeps_only_at_accepted_points(genes, precision):
+    """A batch fit's line-search trials ask for the forward pass alone; the
+    adjoint sweep of that pass follows in a later round where the optimiser
+    accepts the point (other genes' rounds in between leave the kept pass
+    intact).  The trajectory is that of the same minimiser given value and
+    gradient at every trial, here the gene evaluated alone on the same batch
+    (eval_genes), bitwise: same estimates, -lnL and iterations, one more
+    evaluation (the fit's last, at its estimates).  With FP64 partials a
+    stall exit is checked by Newton iterations that the host minimiser does
+    not run, so there only fits that end otherwise are compared."""
+    say(f"case: batch fit, sweeps only at accepted points, {precision}")
+    sub = genes[:5]
+    starts = []
+    for tree, _, _ in sub:
+        s = codeml.default_start(tree, 2.0, 1.5, False)
+        s[:tree.nbranch] = 0.1
+        starts.append(s)
+    b = engine.BranchSiteBatch([(t, d) for t, d, _ in sub], precision=precision)
+    try:
+        xb, fb, ib = b.fit(starts)
+        compared = 0
+        for i in range(len(sub)):
+            if precision == "fp64" and ib[i]["stop"] in ("stall", "nodescent", "newton"):
+                continue
+
+            def fun(x, i=i):
+                f, g = b.eval_genes([i], [x])
+                return f[0], g[0]
+            lo, hi = b.default_bounds(i)
+            xs, fs, ins = engine.minimize(fun, starts[i], lo, hi)
+            say(f"  gene {i}: batch -lnL {fb[i]:.10f} ({ib[i]['iterations']} it, "
+                f"{ib[i]['evaluations']} ev, {ib[i]['stop']}), every trial both "
+                f"{fs:.10f} ({ins['iterations']} it, {ins['evaluations']} ev, {ins['stop']})")
+            assert np.array_equal(xb[i], xs) and fb[i] == fs
+            assert ib[i]["iterations"] == ins["iterations"]
+            assert ib[i]["evaluations"] == ins["evaluations"] + 1
+            compared += 1
+        assert compared >= 3
+    finally:
+        b.close()
