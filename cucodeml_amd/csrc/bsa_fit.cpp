@@ -14,7 +14,8 @@
 //     in N&W eq. 6.20);
 //   * projected backtracking line search: x(a) = clamp(x + a d), Armijo
 //     condition f(x(a)) <= f + 1e-4 g'(x(a) - x), step cut by safeguarded
-//     quadratic interpolation;
+//     quadratic interpolation; a trial costs f alone, the gradient is asked
+//     for only where the trial is accepted;
 //   * BFGS inverse update restricted to the free variables, skipped when the
 //     curvature s'y is not positive;
 //   * stop (the reason is returned, BSA_STOP_*):
@@ -42,11 +43,13 @@
 //     Newton step with a projected Armijo line search.  With more than 160
 //     free variables the step comes from conjugate gradients on
 //     Hessian-vector products instead (forward differences of the gradient
-//     along each direction, at most 64 per step).  Where the last step
-//     is below 1e-6 max(1, |x_i|) in every variable, the fit stops with
-//     BSA_STOP_NEWTON: the point is the maximum to that precision, also in
-//     flat directions (w2 on a ridge) that BFGS's curvature estimate, and the
-//     H_end rule, resolve poorly.
+//     along each direction), preconditioned by the exact Hessian block of
+//     the last nblock variables (codeml's model parameters) and a scalar on
+//     the rest: at most nblock + 32 gradient differences per step.  Where
+//     the last step is below 1e-6 max(1, |x_i|) in every variable, the fit
+//     stops with BSA_STOP_NEWTON: the point is the maximum to that
+//     precision, also in flat directions (w2 on a ridge) that BFGS's
+//     curvature estimate, and the H_end rule, resolve poorly.
 //     With `verify` (exact FP64 gradients), a stall or no-descent exit is
 //     checked the same way by up to four Newton iterations.
 //     BSA_STOP_HEND, BSA_STOP_PGRAD and BSA_STOP_NEWTON are convergence.
