@@ -315,3 +315,37 @@ def test_fit_asks_gradients_only_at_accepted_points(gradient):
     assert ins["iterations"] == inb["iterations"]
     # bsa_fit counts its final evaluation at the returned x too
     assert ins["evaluations"] == inb["evaluations"] + 1
+
+
+def test_fp64_finish_on_a_large_tree_takes_preconditioned_newton_steps():
+    """A tree of 100 taxa has 202 variables (H1): more than 160, so the
+    Newton iterations of the FP64 finish take their steps from conjugate
+    gradients preconditioned by the model parameters' exact Hessian block
+    (bsa_fit.cpp pcg_step), and codeml.finish_fp64 runs them alone
+    (FINISH_NEWTON_LARGE).  From the FP32 fit's estimates they must land on
+    the maximum that the FP64 BFGS fit from the same start finds (or above
+    it), and say so (stop "newton")."""
+    from cucodeml_amd import engine, formats, simulate
+    print("case: FP32 fit + FP64 Newton finish (preconditioned steps), 100 taxa x 400 codons",
+          flush=True)
+    rng = np.random.default_rng(31)
+    tree = simulate.yule_tree(100, 0.05, rng)
+    _, pi = simulate.dirichlet_f3x4(rng)
+    p = simulate.uniform_params(rng)
+    seqs, _ = simulate.simulate_model_a(tree, 400, p["kappa"], p["w0"], p["w2"], p["p0"],
+                                        p["p1"], pi, rng)
+    data = formats.load_codon_data(tree.names, seqs)
+    start = codeml.default_start(tree, 2.0, 1.5, False)
+    assert len(start) > codeml.DENSE_NEWTON_MAX
+    f64 = codeml.fit_model_a(tree, data, precision="fp64", start=start)
+    f32 = codeml.fit_model_a(tree, data, precision="fp32", start=start)
+    ex = f32.extra
+    print(f"  fp64 BFGS fit: lnL {f64.lnL:.8f} ({f64.iterations} iterations, stop {f64.stop}); "
+          f"fp32 fit {ex['fp32_lnL']:.8f} ({ex['fp32_stop']}), after the FP64 Newton finish "
+          f"{f32.lnL:.8f} ({f32.iterations} iterations in all, {f32.evaluations} evaluations, "
+          f"stop {f32.stop}); w2 {f64.w2:.6f} / {f32.w2:.6f}", flush=True)
+    assert f32.stop == "newton" and f32.converged
+    assert f32.lnL >= f64.lnL - 1e-6
+    assert abs(f32.lnL - f64.lnL) <= 1e-4
+    for name in ("kappa", "p0", "p1", "w0"):
+        assert abs(getattr(f32, name) - getattr(f64, name)) <= 1e-3, name
