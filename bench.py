@@ -452,8 +452,8 @@ def main():
                     help="skip timing one Bayes empirical Bayes analysis beside")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding over site patterns "
-              "is described in DESIGN.md but not built", file=sys.stderr)
+        print("bench.py: the engine runs on one GPU; sharding a gene's site patterns "
+              "over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     args.kernel = args.kernel or engine.default_kernel(args.precision)
