@@ -166,3 +166,14 @@ def test_newton_on_many_variables_with_a_bound():
     assert info["converged"] and x[-1] == hi[-1]
     _, g = fun(x)
     assert np.max(np.abs(g[:-1])) < 1e-3 and g[-1] < 0
+
+
+def test_block_argument_is_checked():
+    # nblock names the last variables of x: 0..n
+    fun, A, c, x0 = _ridge_large(n=170)
+    lo, hi = -np.ones(len(c)), np.ones(len(c))
+    for bad in (-1, len(c) + 1):
+        with pytest.raises(engine.BsaError):
+            engine.minimize(fun, x0, lo, hi, max_iter=-1, newton=1, block=bad)
+    x, _, info = engine.minimize(fun, x0, lo, hi, max_iter=-1, newton=1, block=len(c))
+    assert np.all(np.isfinite(x)) and info["iterations"] == 1
