@@ -62,6 +62,7 @@ extern "C" int bsa_set_model(bsa_engine *e, const bsa_model_opts *o)
     if (!e || !o) return fail(BSA_EINVAL, "null argument");
     if (o->fix_omega && !(o->omega_fix > 0)) return fail(BSA_EINVAL, "omega_fix must be > 0");
     e->g.opts = *o;
+    e->part_ready = false;   // x means something else now: no sweep of a kept forward pass
     return BSA_OK;
 }
 
