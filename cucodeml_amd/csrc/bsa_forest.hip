@@ -1268,37 +1268,39 @@ static int run_round(Forest &f, const std::vector<int> &gl, const std::vector<in
         rs.ej = f.d_sel + off[k++];
         rs.sets = f.d_sel + off[k++];
     }
-    // the listed genes' inputs
-    const std::vector<Gene *> &G = f.genes;
+    // Inputs and outputs.  The per-gene parameter blocks and Z matrices (about
+    // 200 and 100 KB a gene) cross for the listed genes only, one copy per run
+    // of consecutive genes.  The small per-gene lists (P(t) jobs and
+    // eigen-basis jobs in, lnL sums and d lnL/d tau out) cross whole, one
+    // copy each: a round of scattered genes would otherwise take a copy per
+    // gene and list (80 a round in a scan).  That is safe: kernels read only
+    // the listed genes' entries; a gene awaiting the sweep of a kept forward
+    // pass does not stage (its host lists are those of that pass); and a
+    // gene's device outputs change only in rounds that list it, which copy
+    // them back, so the other genes' host values are rewritten unchanged.
     int rc = copy_runs(s, f.d_uvr, f.h_uvr, sizeof(double), gl,
                        [](int g) { return (size_t)g * kGeneBlock; },
                        [](int) { return (size_t)kGeneBlock; }, hipMemcpyHostToDevice);
-    if (!rc)
-        rc = copy_runs(s, f.d_pjobs, f.h_pjobs, sizeof(PmatJob), gl,
-                       [&](int g) { return (size_t)G[g]->pj_base; },
-                       [&](int g) { return G[g]->pjobs.size(); }, hipMemcpyHostToDevice);
-    if (!rc && adjoint)
-        rc = copy_runs(s, f.d_ejobs, f.h_ejobs, sizeof(EigJob), ga,
-                       [&](int g) { return (size_t)G[g]->ej_base; },
-                       [&](int g) { return G[g]->ejobs.size(); }, hipMemcpyHostToDevice);
     if (rc) return rc;
+    if (!gl.empty())
+        HIPCHK(hipMemcpyAsync(f.d_pjobs, f.h_pjobs, sizeof(PmatJob) * f.pjobs.size(),
+                              hipMemcpyHostToDevice, s));
+    if (adjoint)
+        HIPCHK(hipMemcpyAsync(f.d_ejobs, f.h_ejobs, sizeof(EigJob) * f.ejobs.size(),
+                              hipMemcpyHostToDevice, s));
     if ((rc = enqueue_forward<T>(f, s, keep, true, &rs))) return rc;
     if (adjoint && (rc = enqueue_adjoint<T>(f, s, true, &rs))) return rc;
-    // their outputs
     if (adjoint) {
-        rc = copy_runs(s, f.h_adjout, f.d_dtau, sizeof(double), ga,
-                       [&](int g) { return (size_t)G[g]->ej_base; },
-                       [&](int g) { return G[g]->ejobs.size(); }, hipMemcpyDeviceToHost);
-        if (!rc)
-            rc = copy_runs(s, f.h_adjout + f.ejobs.size(), f.d_Z, sizeof(double), ga,
-                           [](int g) { return (size_t)g * 3 * 4096; },
-                           [](int) { return (size_t)3 * 4096; }, hipMemcpyDeviceToHost);
+        HIPCHK(hipMemcpyAsync(f.h_adjout, f.d_dtau, sizeof(double) * f.ejobs.size(),
+                              hipMemcpyDeviceToHost, s));
+        rc = copy_runs(s, f.h_adjout + f.ejobs.size(), f.d_Z, sizeof(double), ga,
+                       [](int g) { return (size_t)g * 3 * 4096; },
+                       [](int) { return (size_t)3 * 4096; }, hipMemcpyDeviceToHost);
         if (rc) return rc;
     }
-    rc = copy_runs(s, f.h_out, f.d_out, sizeof(double), gl,
-                   [](int g) { return (size_t)g * kRootSums; }, [](int) { return (size_t)kRootSums; },
-                   hipMemcpyDeviceToHost);
-    if (rc) return rc;
+    if (!gl.empty())
+        HIPCHK(hipMemcpyAsync(f.h_out, f.d_out, sizeof(double) * kRootSums * f.G,
+                              hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     int nl = 1;
     for (int v : rs.nadl) nl += v > 0;
