@@ -40,13 +40,17 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
 
-def make_workload(ns, ncodon, seed):
+def make_workload(ns, ncodon, seed, branch_mean=0.05, gap=0.0):
+    """Seeded Yule tree (branch lengths exponential of the given mean) and a
+    model-A alignment simulated on it, with a fraction `gap` of the codons
+    missing (512 taxa, 50,000 codons, mean 0.02, gap 0.1, seed 4 is the
+    large-gene workload, the same draws as tests/test_gpu_large.py)."""
     rng = np.random.default_rng(seed)
-    tree = simulate.yule_tree(ns, 0.05, rng)
+    tree = simulate.yule_tree(ns, branch_mean, rng)
     _, pi = simulate.dirichlet_f3x4(rng)
     p = simulate.uniform_params(rng)
     seqs, _ = simulate.simulate_model_a(tree, ncodon, p["kappa"], p["w0"], p["w2"], p["p0"],
-                                        p["p1"], pi, rng)
+                                        p["p1"], pi, rng, gap_frac=gap)
     data = formats.load_codon_data(tree.names, seqs)
     return tree, data, p, seqs
 
@@ -411,6 +415,10 @@ def main():
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--codons", type=int, default=10000)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--branch-mean", type=float, default=0.05,
+                    help="mean branch length of the simulated tree")
+    ap.add_argument("--gap", type=float, default=0.0,
+                    help="fraction of missing codons in the simulated alignment")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32")
     ap.add_argument("--gradient", choices=["adjoint", "fd"], default="adjoint")
     ap.add_argument("--kernel", choices=["mfma", "mfma32", "fp32mc", "valu"], default=None,
@@ -457,7 +465,8 @@ def main():
         sys.exit(2)
 
     args.kernel = args.kernel or engine.default_kernel(args.precision)
-    tree, data, p, seqs = make_workload(args.taxa, args.codons, args.seed)
+    tree, data, p, seqs = make_workload(args.taxa, args.codons, args.seed, args.branch_mean,
+                                        args.gap)
     x = engine.x_from_params(tree.branch_vector(), p["kappa"], p["p0"], p["p1"], p["w0"],
                              p["w2"])
     npar = len(x)
@@ -510,9 +519,10 @@ def main():
                          "fp32" if m32 else "fp64")
         rls += [(tm["sum_adjoint"], adj_rl), (tm["sum_branch"], br_rl)]
     rl = max(rls, key=lambda r: r[0])[1]
-    default_workload = (args.taxa, args.codons, args.seed, args.precision, args.gradient,
-                        args.kernel, args.tile, args.tune) == (128, 10000, 2, "fp32", "adjoint",
-                                                               "mfma32", 0, True)
+    default_workload = (args.taxa, args.codons, args.seed, args.branch_mean, args.gap,
+                        args.precision, args.gradient, args.kernel, args.tile,
+                        args.tune) == (128, 10000, 2, 0.05, 0.0, "fp32", "adjoint", "mfma32", 0,
+                                       True)
     tr = measured_traffic(rl["kernel"]) if default_workload else None
     if tr:
         rl["traffic"], rl["traffic_source"] = tr
@@ -537,6 +547,7 @@ def main():
                                f"({data.npatt} patterns), {args.precision} partials, "
                                f"{args.gradient} gradient of {npar} parameters",
                    "taxa": args.taxa, "codons": args.codons, "patterns": data.npatt,
+                   "branch_mean": args.branch_mean, "gap": args.gap, "seed": args.seed,
                    "launch_shapes": shapes,
                    "np": npar, "gradient": args.gradient, "kernel": args.kernel,
                    "tile": args.tile or 64},
